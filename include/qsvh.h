@@ -1,0 +1,216 @@
+/*
+ * qsvh.h -- C ABI of the MI355X (gfx950) statevector engine.
+ *
+ * This is the drop-in boundary for the reference's `Backend<R>` trait
+ * (nez0b/rustatevec crates/qsv-core/src/backend/mod.rs:31-74): a Rust `qsv-hip` crate
+ * implements `Backend` by calling these functions (see INTEGRATION.md for the binding).
+ * All signatures use plain pointers and sizes; no exceptions or aborts cross the ABI --
+ * every call returns a qsvh_status.
+ *
+ * Conventions (crates/qsv-core/src/gate.rs:8-12): a gate acting on the ordered qubit
+ * list qs[0..m) is a row-major 2^m x 2^m complex matrix whose internal index has bit j
+ * equal to global qubit qs[j]; controlled gates list the control first.  Amplitudes are
+ * Structure-of-Arrays (crates/qsv-core/src/state/mod.rs:25-29): re[2^n], im[2^n] of f64
+ * (QSVH_C128) or f32 (QSVH_C64).  Matrices always cross the ABI in f64 and are rounded to
+ * the state's precision on the device side, as `DenseGate<R>` is built with
+ * `Cplx::from_f64` (crates/qsv-core/src/complex.rs:43-50).
+ *
+ * Threading (crates/qsv-cuda/src/lib.rs:59,199 for the reference's CUDA backend): one HIP
+ * stream per context; calls on one context must be serialised by the caller; separate
+ * contexts are independent.  Work is asynchronous until a call that returns host data
+ * (amplitude, probabilities, norm_sqr, expectation_z, download) or qsvh_synchronize.
+ */
+#ifndef QSVH_H
+#define QSVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSVH_ABI_VERSION 1
+#define QSVH_MAX_GATE_QUBITS 6 /* apply_mq limit, crates/qsv-core/src/backend/cpu.rs:34,180 */
+
+/* Status codes.  The reference panics where these are returned:
+ * Circuit::push asserts (crates/qsv-core/src/circuit.rs:55-71), apply_mq's m <= 6 assert
+ * (crates/qsv-core/src/backend/cpu.rs:180), `.expect` on device calls
+ * (crates/qsv-cuda/src/lib.rs:90-227). */
+typedef enum qsvh_status {
+  QSVH_OK = 0,
+  QSVH_ERR_INVALID_ARG = 1,     /* null pointer, unknown enum/name, bad size          */
+  QSVH_ERR_INVALID_QUBIT = 2,   /* qubit >= n_qubits               (circuit.rs:60-66) */
+  QSVH_ERR_REPEATED_QUBIT = 3,  /* same qubit twice in one gate    (circuit.rs:67-71) */
+  QSVH_ERR_ARITY = 4,           /* matrix size != 2^m x 2^m        (circuit.rs:55-59) */
+  QSVH_ERR_TOO_WIDE = 5,        /* gate on more than 6 qubits      (cpu.rs:180)       */
+  QSVH_ERR_OOM = 6,             /* device (or host) allocation failed                 */
+  QSVH_ERR_HIP = 7,             /* a HIP runtime call failed; see qsvh_last_error     */
+  QSVH_ERR_DTYPE = 8,           /* operation not available for this state dtype       */
+  QSVH_ERR_NO_DEVICE = 9        /* no HIP device with that ordinal                    */
+} qsvh_status;
+
+typedef enum qsvh_dtype { QSVH_C128 = 0, QSVH_C64 = 1 } qsvh_dtype;
+
+/* How fused (non-diagonal, k >= 2) groups are applied by qsvh_program_*.
+ * EXACT: the state's own precision on the vector ALUs (f64 for C128: the plain path).
+ * MFMA_F32: C64 states; k-qubit groups as FP32 matrix-core contractions.
+ * MFMA_BF16X3: C64 states; split-BF16 (hi+lo) 3-product matrix-core contractions
+ * accumulated in FP32. */
+typedef enum qsvh_precision {
+  QSVH_PREC_EXACT = 0,
+  QSVH_PREC_MFMA_F32 = 1,
+  QSVH_PREC_MFMA_BF16X3 = 2
+} qsvh_precision;
+
+typedef struct qsvh_ctx qsvh_ctx;         /* device + stream (CudaBackend, lib.rs:41-55)  */
+typedef struct qsvh_state qsvh_state;     /* device state (Backend::State, mod.rs:33)     */
+typedef struct qsvh_circuit qsvh_circuit; /* host circuit (Circuit<R>, circuit.rs:30-34)  */
+typedef struct qsvh_program qsvh_program; /* a circuit bound to a state, ready to replay   */
+
+/* ------------------------------------------------------------------ context */
+/* CudaBackend::new(ordinal) (crates/qsv-cuda/src/lib.rs:57-77). */
+int qsvh_ctx_create(int device, qsvh_ctx** out);
+void qsvh_ctx_destroy(qsvh_ctx* ctx);
+/* Text of the last error recorded on ctx ("" if none).  No reference counterpart:
+ * the reference panics with the message instead. */
+const char* qsvh_last_error(const qsvh_ctx* ctx);
+/* Device name, compute-unit count and HBM bytes (for reports). */
+int qsvh_device_info(qsvh_ctx* ctx, char* name, size_t name_cap, int* compute_units,
+                     uint64_t* hbm_bytes);
+int qsvh_abi_version(void);
+
+/* ------------------------------------------------------------------ gates */
+/* The gate library (crates/qsv-core/src/gate.rs:96-180): name is one of h x y z s t sx
+ * rx ry rz p|phase cx cz cp|cphase swap rzz, plus u3 (OpenQASM 3 U(theta,phi,lambda)),
+ * ccx and ccz (crates/qsv-core/tests/equivalence.rs:182-193).  Writes the row-major
+ * matrix into re/im (capacity 64 entries each) and its arity into *m. */
+int qsvh_gate_matrix(const char* name, const double* params, uint32_t n_params, double* re,
+                     double* im, uint32_t* m);
+
+/* ------------------------------------------------------------------ circuits */
+/* Circuit::new (crates/qsv-core/src/circuit.rs:37-42). */
+int qsvh_circuit_create(uint32_t n_qubits, qsvh_circuit** out);
+void qsvh_circuit_destroy(qsvh_circuit* c);
+/* Circuit::push (circuit.rs:54-78): validated, returns ARITY/INVALID_QUBIT/
+ * REPEATED_QUBIT/TOO_WIDE instead of panicking.  re/im: row-major 2^m x 2^m. */
+int qsvh_circuit_push(qsvh_circuit* c, uint32_t m, const uint32_t* qubits, const double* re,
+                      const double* im);
+/* The fluent builders (circuit.rs:81-130): push the named library gate. */
+int qsvh_circuit_push_gate(qsvh_circuit* c, const char* name, const uint32_t* qubits,
+                           const double* params, uint32_t n_params);
+uint32_t qsvh_circuit_n_qubits(const qsvh_circuit* c);
+size_t qsvh_circuit_len(const qsvh_circuit* c);
+/* Circuit::ops()[i] (circuit.rs:48-51): arity, qubits (capacity 6), matrix (capacity 64
+ * complex entries when the gate has m <= 3; 4^m entries in general). */
+int qsvh_circuit_op(const qsvh_circuit* c, size_t i, uint32_t* m, uint32_t* qubits,
+                    double* re, double* im);
+
+/* Generators (crates/qsv-core/src/circuits.rs:50-113) and the measured workloads. */
+int qsvh_circuit_ghz(uint32_t n, qsvh_circuit** out);       /* circuits.rs:50-57        */
+int qsvh_circuit_ghz_chain(uint32_t n, qsvh_circuit** out); /* H(0); CX(i,i+1)          */
+int qsvh_circuit_qft(uint32_t n, qsvh_circuit** out);       /* circuits.rs:61-74        */
+int qsvh_circuit_random(uint32_t n, uint64_t n_gates, uint64_t seed,
+                        qsvh_circuit** out);                /* circuits.rs:79-113       */
+/* Random layered circuit: per layer U3 on every qubit, RZ on every qubit, CZ brickwork. */
+int qsvh_circuit_layered(uint32_t n, uint32_t depth, uint64_t seed, qsvh_circuit** out);
+/* Hardware-efficient ansatz: per layer RY, RZ on every qubit, then CX(i,i+1). */
+int qsvh_circuit_hea(uint32_t n, uint32_t depth, uint64_t seed, qsvh_circuit** out);
+/* Random gate mix: 50% U3, 30% CX/CZ, 15% RZZ, 5% CCX/CCZ on random qubits. */
+int qsvh_circuit_mix(uint32_t n, uint64_t n_gates, uint64_t seed, qsvh_circuit** out);
+
+/* fusion::fuse (crates/qsv-core/src/fusion.rs:39-77) with FusionConfig{max_qubits}:
+ * greedy consecutive grouping on the host; every multi-op group's matrix is composed in
+ * f64 on the device (same product order as build_fused, fusion.rs:102-139) and the group
+ * is emitted on its sorted qubits (fusion.rs:91-94).  *out is a new circuit. */
+int qsvh_fuse(qsvh_ctx* ctx, const qsvh_circuit* in, uint32_t max_qubits,
+              qsvh_circuit** out);
+
+/* The greedy grouping of fusion::fuse alone, on the host (fusion.rs:39-99): for output
+ * op g, the input op range [begin[g], end[g]), its arity m[g] and its qubits at
+ * qubits[6*g ..] (sorted when the group holds more than one op).  Writes at most `cap`
+ * groups and the total count to *n_groups. */
+int qsvh_fuse_groups(const qsvh_circuit* in, uint32_t max_qubits, uint64_t* begin,
+                     uint64_t* end, uint32_t* m, uint32_t* qubits, size_t cap,
+                     size_t* n_groups);
+
+/* ------------------------------------------------------------------ state (Backend) */
+/* Backend::alloc (crates/qsv-core/src/backend/mod.rs:36-37); contents unspecified. */
+int qsvh_state_alloc(qsvh_ctx* ctx, uint32_t n_qubits, int dtype, qsvh_state** out);
+void qsvh_state_free(qsvh_state* st);
+uint32_t qsvh_state_n_qubits(const qsvh_state* st);
+int qsvh_state_dtype(const qsvh_state* st);
+/* Backend::init_basis (mod.rs:39-40). */
+int qsvh_init_basis(qsvh_state* st, uint64_t basis);
+/* Backend::apply (mod.rs:42-43): dispatch diagonal -> 1q -> mq as CpuBackend::apply
+ * (crates/qsv-core/src/backend/cpu.rs:281-290). */
+int qsvh_apply(qsvh_state* st, uint32_t m, const uint32_t* qubits, const double* re,
+               const double* im);
+/* Backend::amplitude (mod.rs:45-46): single-element device->host copy. */
+int qsvh_amplitude(qsvh_state* st, uint64_t index, double* re, double* im);
+/* Backend::probabilities (mod.rs:48-49): |psi_i|^2 in the state's precision (f64 for
+ * C128, f32 for C64) into host_out[2^n]. */
+int qsvh_probabilities(qsvh_state* st, void* host_out);
+/* Device-side probabilities into a caller-owned device buffer (no host copy). */
+int qsvh_probabilities_device(qsvh_state* st, void* device_out);
+/* <psi|psi> as an f64 tree reduction (StateVector::norm_sqr, state/mod.rs:112-117). */
+int qsvh_norm_sqr(qsvh_state* st, double* out);
+/* Backend::expectation_z (mod.rs:57-59, sample.rs:177-200): sum_i |psi_i|^2 *
+ * (-1)^popcount(i & mask), f64 accumulation on the device. */
+int qsvh_expectation_z(qsvh_state* st, uint64_t mask, double* out);
+/* Backend::download (mod.rs:61-62): the full SoA state to host re/im (2^n each). */
+int qsvh_download(qsvh_state* st, void* re, void* im);
+/* Host -> device copy of a full state.  No reference counterpart (the trait has no
+ * upload); needed to start from a prepared state such as a random normalised vector. */
+int qsvh_upload(qsvh_state* st, const void* re, const void* im);
+/* Random normalised start state, generated on the device: psi_i = (a_i + i b_i) / norm
+ * with a_i, b_i ~ N(0,1) by Box-Muller from outputs 2i and 2i+1 of SplitMix64(seed)
+ * (crates/qsv-core/src/circuits.rs:12-47): u = (out >> 11) * 2^-53,
+ * a + ib = sqrt(-2 ln(1 - u1)) * e^{2 pi i u2}.  No reference counterpart (the trait has
+ * no upload); it is the start state of the QFT workload. */
+int qsvh_init_random_normal(qsvh_state* st, uint64_t seed);
+/* Compare two states of equal size (any dtypes): *max_abs_err = max_i |a_i - b_i|,
+ * *fidelity = |<a|b>|^2 / (<a|a><b|b>), both accumulated in f64 on the device. */
+int qsvh_compare(qsvh_state* a, qsvh_state* b, double* max_abs_err, double* fidelity);
+/* Wait until all work queued on the state's context has finished. */
+int qsvh_synchronize(qsvh_state* st);
+
+/* Backend::execute (mod.rs:66-73): alloc, init_basis(0), apply every op; one submission
+ * (no per-gate synchronisation). */
+int qsvh_execute(qsvh_ctx* ctx, const qsvh_circuit* c, int dtype, qsvh_state** out);
+
+/* ------------------------------------------------------------------ programs */
+/* A program binds a circuit to a state: gate classification, device-resident matrix
+ * arena (uploaded once) and, optionally, a captured HIP graph.  No reference
+ * counterpart: it is how the default execute loop (mod.rs:66-73) becomes one submission
+ * that can be replayed. */
+typedef struct qsvh_program_opts {
+  int precision;    /* qsvh_precision; EXACT is the plain path                          */
+  int use_graph;    /* capture the launches in a hipGraph and replay it                */
+  int tile_qubits;  /* >0: apply runs of gates on a 2^tile_qubits LDS tile per HBM pass */
+  int reserved[5];
+} qsvh_program_opts;
+
+int qsvh_program_create(qsvh_state* st, const qsvh_circuit* c, const qsvh_program_opts* opts,
+                        qsvh_program** out);
+/* Apply the program's gates to the state's current contents (asynchronous). */
+int qsvh_program_launch(qsvh_program* p);
+/* Launch once and return the device time of that launch in milliseconds, measured with
+ * HIP events recorded on the context's stream. */
+int qsvh_program_time(qsvh_program* p, double* ms);
+/* Number of device kernels one launch issues (passes over the state). */
+int qsvh_program_num_kernels(const qsvh_program* p, uint64_t* out);
+void qsvh_program_destroy(qsvh_program* p);
+
+/* Apply one gate `reps` times back to back and return the mean device time per launch
+ * (HIP events on the context stream).  Measurement helper for single-kernel rooflines. */
+int qsvh_time_gate(qsvh_state* st, uint32_t m, const uint32_t* qubits, const double* re,
+                   const double* im, uint32_t reps, double* ms_per_launch);
+/* Device-to-device copy of the whole state, timed like qsvh_time_gate: the measured
+ * bandwidth ceiling (2 * bytes(state) moved per launch). */
+int qsvh_time_copy(qsvh_state* st, uint32_t reps, double* ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSVH_H */

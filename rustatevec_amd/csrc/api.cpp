@@ -1,0 +1,741 @@
+// api.cpp -- the C ABI (include/qsvh.h): contexts, device states, circuits, fusion and
+// programs (a circuit bound to a state: classified launches, a device matrix arena
+// uploaded once, optionally a captured HIP graph).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/qsvh.h"
+#include "engine.hpp"
+#include "ir.hpp"
+#include "program.hpp"
+
+struct qsvh_ctx {
+  qsvh::Ctx c;
+};
+struct qsvh_state {
+  qsvh::State s;
+};
+struct qsvh_circuit {
+  qsvh::Circuit c;
+};
+struct qsvh_program {
+  qsvh::Program p;
+};
+
+namespace qsvh {
+
+int set_err(Ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->last_error = msg;
+  return code;
+}
+
+int hip_err(Ctx* ctx, hipError_t e, const char* where) {
+  if (e == hipSuccess) return kOk;
+  return set_err(ctx, e == hipErrorOutOfMemory ? kOom : kHip,
+                 std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// Round an f64 matrix to the state's precision (DenseGate<R> is built with
+// Cplx::from_f64, crates/qsv-core/src/complex.rs:43-50) and keep it as f64, interleaved.
+std::vector<double> rounded(const Op& op, int dtype) {
+  const size_t n = op.re.size();
+  std::vector<double> g(2 * n);
+  for (size_t e = 0; e < n; ++e) {
+    g[2 * e] = dtype == 0 ? op.re[e] : double(float(op.re[e]));
+    g[2 * e + 1] = dtype == 0 ? op.im[e] : double(float(op.im[e]));
+  }
+  return g;
+}
+
+bool diag_of(const std::vector<double>& g, uint32_t m) {
+  const uint32_t d = 1u << m;
+  for (uint32_t r = 0; r < d; ++r)
+    for (uint32_t c = 0; c < d; ++c)
+      if (r != c &&
+          (g[2 * (size_t(r) * d + c)] != 0.0 || g[2 * (size_t(r) * d + c) + 1] != 0.0))
+        return false;
+  return true;
+}
+
+// CpuBackend::apply dispatch (crates/qsv-core/src/backend/cpu.rs:281-290):
+// diagonal -> 1q -> mq, with the gate in the state's precision.
+Launch classify(const Op& op, int dtype) {
+  Launch l;
+  l.m = op.m;
+  std::copy(op.q, op.q + op.m, l.q);
+  std::vector<double> g = rounded(op, dtype);
+  if (diag_of(g, op.m)) {
+    l.kind = Launch::kDiag;
+    const uint32_t d = 1u << op.m;
+    l.mat.resize(2 * d);
+    for (uint32_t s = 0; s < d; ++s) {
+      l.mat[2 * s] = g[2 * (size_t(s) * d + s)];
+      l.mat[2 * s + 1] = g[2 * (size_t(s) * d + s) + 1];
+    }
+  } else if (op.m == 1) {
+    l.kind = Launch::k1q;
+    l.mat = std::move(g);
+  } else {
+    l.kind = Launch::kMq;
+    l.mat = std::move(g);
+  }
+  return l;
+}
+
+hipError_t issue(const State& st, const Launch& l, const void* arena, const Tuning& t,
+                 hipStream_t s) {
+  const char* base = static_cast<const char*>(arena);
+  const void* dmat = base ? base + l.arena_off * st.elem() : nullptr;
+  switch (l.kind) {
+    case Launch::k1q:
+      return launch_1q(st, l.q[0], l.mat.data(), t, s);
+    case Launch::kDiag:
+      return launch_diag(st, l.m, l.q, l.mat.data(), t, s);
+    case Launch::kMq:
+      return launch_mq(st, l.m, l.q, l.mat.data(), dmat, t, s);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+// Upload, once, the matrices of launches that read them from device memory (m >= 4),
+// converted to the state's precision.
+int build_arena(Program* p) {
+  State& st = *p->st;
+  Ctx* ctx = st.ctx;
+  std::vector<double> host;
+  for (Launch& l : p->launches) {
+    if (l.kind == Launch::kMq && l.m >= 4) {
+      l.arena_off = host.size();
+      host.insert(host.end(), l.mat.begin(), l.mat.end());
+    }
+  }
+  if (host.empty()) return kOk;
+  p->arena_bytes = host.size() * st.elem();
+  hipError_t e = hipMalloc(&p->arena, p->arena_bytes);
+  if (e != hipSuccess) return hip_err(ctx, e, "arena alloc");
+  if (st.dtype == 0) {
+    e = hipMemcpy(p->arena, host.data(), p->arena_bytes, hipMemcpyHostToDevice);
+  } else {
+    std::vector<float> f(host.begin(), host.end());
+    e = hipMemcpy(p->arena, f.data(), p->arena_bytes, hipMemcpyHostToDevice);
+  }
+  return hip_err(ctx, e, "arena upload");
+}
+
+hipError_t issue_all(Program* p) {
+  State& st = *p->st;
+  for (const Launch& l : p->launches) {
+    const hipError_t e = issue(st, l, p->arena, st.ctx->tune, st.ctx->stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+int program_launch(Program* p) {
+  Ctx* ctx = p->st->ctx;
+  if (!p->use_graph) return hip_err(ctx, issue_all(p), "launch");
+  if (!p->graph) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_err(ctx, e, "begin capture");
+    const hipError_t le = issue_all(p);
+    e = hipStreamEndCapture(ctx->stream, &g);
+    if (le != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      return hip_err(ctx, le, "capture launch");
+    }
+    if (e != hipSuccess) return hip_err(ctx, e, "end capture");
+    e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_err(ctx, e, "graph instantiate");
+  }
+  return hip_err(ctx, hipGraphLaunch(p->graph, ctx->stream), "graph launch");
+}
+
+int program_build(State* st, const Circuit& c, const qsvh_program_opts* opts, Program* p) {
+  p->st = st;
+  p->use_graph = opts && opts->use_graph;
+  p->precision = opts ? opts->precision : 0;
+  if (c.n != st->n)
+    return set_err(st->ctx, kInvalidArg, "circuit and state qubit counts differ");
+  if (p->precision != QSVH_PREC_EXACT && st->dtype != QSVH_C64)
+    return set_err(st->ctx, kDtype, "MFMA precisions need a complex64 state");
+  p->launches.reserve(c.ops.size());
+  for (const Op& op : c.ops) p->launches.push_back(classify(op, st->dtype));
+  int rc = plan_tiles(p, opts ? opts->tile_qubits : 0);
+  if (rc != kOk) return rc;
+  rc = plan_mfma(p);
+  if (rc != kOk) return rc;
+  return build_arena(p);
+}
+
+void program_free(Program* p) {
+  // Device buffers may still be read by queued launches: wait before releasing them.
+  // A program that owns nothing (gates of m <= 3 travel as kernel arguments) returns
+  // without synchronising, so qsvh_apply stays asynchronous.
+  const bool owns = p->graph || p->arena || !p->owned.empty();
+  if (owns && p->st && p->st->ctx) hipStreamSynchronize(p->st->ctx->stream);
+  if (p->graph) hipGraphExecDestroy(p->graph);
+  hipFree(p->arena);
+  release_extra(p);
+  p->graph = nullptr;
+  p->arena = nullptr;
+}
+
+}  // namespace qsvh
+
+using namespace qsvh;
+
+// ======================================================================= C ABI
+extern "C" {
+
+int qsvh_abi_version(void) { return QSVH_ABI_VERSION; }
+
+int qsvh_ctx_create(int device, qsvh_ctx** out) {
+  if (!out) return kInvalidArg;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return kNoDevice;
+  qsvh_ctx* h = new (std::nothrow) qsvh_ctx;
+  if (!h) return kOom;
+  Ctx& c = h->c;
+  c.device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c.d_partials, Ctx::kMaxPartials * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c.d_result, sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(&c.h_result, sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c.cus = prop.multiProcessorCount;
+  }
+  if (e != hipSuccess) {
+    qsvh_ctx_destroy(h);
+    return e == hipErrorOutOfMemory ? kOom : kHip;
+  }
+  *out = h;
+  return kOk;
+}
+
+void qsvh_ctx_destroy(qsvh_ctx* h) {
+  if (!h) return;
+  Ctx& c = h->c;
+  if (c.stream) hipStreamSynchronize(c.stream);
+  hipFree(c.d_partials);
+  hipFree(c.d_result);
+  if (c.h_result) hipHostFree(c.h_result);
+  if (c.stream) hipStreamDestroy(c.stream);
+  delete h;
+}
+
+const char* qsvh_last_error(const qsvh_ctx* h) { return h ? h->c.last_error.c_str() : ""; }
+
+int qsvh_device_info(qsvh_ctx* h, char* name, size_t cap, int* cus, uint64_t* hbm) {
+  if (!h) return kInvalidArg;
+  hipDeviceProp_t prop;
+  const hipError_t e = hipGetDeviceProperties(&prop, h->c.device);
+  if (e != hipSuccess) return hip_err(&h->c, e, "device properties");
+  if (name && cap) {
+    std::strncpy(name, prop.name, cap - 1);
+    name[cap - 1] = '\0';
+  }
+  if (cus) *cus = prop.multiProcessorCount;
+  if (hbm) *hbm = prop.totalGlobalMem;
+  return kOk;
+}
+
+// ------------------------------------------------------------------ gates / circuits
+int qsvh_gate_matrix(const char* name, const double* params, uint32_t n_params, double* re,
+                     double* im, uint32_t* m) {
+  if (!name || !re || !im || !m) return kInvalidArg;
+  Op op;
+  const int st = make_gate(name, params, n_params, &op);
+  if (st != kOk) return st;
+  std::copy(op.re.begin(), op.re.end(), re);
+  std::copy(op.im.begin(), op.im.end(), im);
+  *m = op.m;
+  return kOk;
+}
+
+int qsvh_circuit_create(uint32_t n_qubits, qsvh_circuit** out) {
+  if (!out) return kInvalidArg;
+  *out = new (std::nothrow) qsvh_circuit;
+  if (!*out) return kOom;
+  (*out)->c.n = n_qubits;
+  return kOk;
+}
+
+void qsvh_circuit_destroy(qsvh_circuit* c) { delete c; }
+
+int qsvh_circuit_push(qsvh_circuit* c, uint32_t m, const uint32_t* qubits, const double* re,
+                      const double* im) {
+  if (!c) return kInvalidArg;
+  return push(&c->c, m, qubits, re, im);
+}
+
+int qsvh_circuit_push_gate(qsvh_circuit* c, const char* name, const uint32_t* qubits,
+                           const double* params, uint32_t n_params) {
+  if (!c) return kInvalidArg;
+  return push_gate(&c->c, name, qubits, params, n_params);
+}
+
+uint32_t qsvh_circuit_n_qubits(const qsvh_circuit* c) { return c ? c->c.n : 0; }
+size_t qsvh_circuit_len(const qsvh_circuit* c) { return c ? c->c.ops.size() : 0; }
+
+int qsvh_circuit_op(const qsvh_circuit* c, size_t i, uint32_t* m, uint32_t* qubits, double* re,
+                    double* im) {
+  if (!c || i >= c->c.ops.size()) return kInvalidArg;
+  const Op& op = c->c.ops[i];
+  if (m) *m = op.m;
+  if (qubits) std::copy(op.q, op.q + op.m, qubits);
+  if (re) std::copy(op.re.begin(), op.re.end(), re);
+  if (im) std::copy(op.im.begin(), op.im.end(), im);
+  return kOk;
+}
+
+static int wrap(Circuit&& c, qsvh_circuit** out) {
+  if (!out) return kInvalidArg;
+  *out = new (std::nothrow) qsvh_circuit;
+  if (!*out) return kOom;
+  (*out)->c = std::move(c);
+  return kOk;
+}
+
+int qsvh_circuit_ghz(uint32_t n, qsvh_circuit** out) {
+  if (n < 1) return kInvalidArg;
+  return wrap(gen_ghz(n), out);
+}
+int qsvh_circuit_ghz_chain(uint32_t n, qsvh_circuit** out) {
+  if (n < 1) return kInvalidArg;
+  return wrap(gen_ghz_chain(n), out);
+}
+int qsvh_circuit_qft(uint32_t n, qsvh_circuit** out) {
+  if (n < 1) return kInvalidArg;
+  return wrap(gen_qft(n), out);
+}
+int qsvh_circuit_random(uint32_t n, uint64_t n_gates, uint64_t seed, qsvh_circuit** out) {
+  if (n < 2) return kInvalidArg;  // circuits.rs:80
+  return wrap(gen_random(n, n_gates, seed), out);
+}
+int qsvh_circuit_layered(uint32_t n, uint32_t depth, uint64_t seed, qsvh_circuit** out) {
+  if (n < 1) return kInvalidArg;
+  return wrap(gen_layered(n, depth, seed), out);
+}
+int qsvh_circuit_hea(uint32_t n, uint32_t depth, uint64_t seed, qsvh_circuit** out) {
+  if (n < 1) return kInvalidArg;
+  return wrap(gen_hea(n, depth, seed), out);
+}
+int qsvh_circuit_mix(uint32_t n, uint64_t n_gates, uint64_t seed, qsvh_circuit** out) {
+  if (n < 3) return kInvalidArg;
+  return wrap(gen_mix(n, n_gates, seed), out);
+}
+
+int qsvh_fuse(qsvh_ctx* h, const qsvh_circuit* in, uint32_t max_qubits, qsvh_circuit** out) {
+  if (!h || !in || !out) return kInvalidArg;
+  *out = nullptr;
+  if (max_qubits > kMaxGateQubits)
+    return set_err(&h->c, kTooWide, "fusion groups are limited to 6 qubits");
+  const Circuit& c = in->c;
+  const std::vector<Group> groups = greedy_groups(c, max_qubits);
+  std::vector<ComposeGroup> cg;
+  std::vector<ComposeGate> gates;
+  std::vector<double> mats;
+  uint64_t out_off = 0;
+  for (const Group& g : groups) {
+    if (g.end - g.begin == 1) continue;
+    ComposeGroup x;
+    x.k = g.m;
+    x.first_gate = uint32_t(gates.size());
+    x.n_gates = uint32_t(g.end - g.begin);
+    x.out_off = out_off;
+    out_off += uint64_t(1) << (2 * g.m);
+    for (uint64_t i = g.begin; i < g.end; ++i) {
+      const Op& op = c.ops[i];
+      ComposeGate y;
+      y.m = op.m;
+      for (uint32_t j = 0; j < op.m; ++j)
+        y.local[j] = uint32_t(std::find(g.q, g.q + g.m, op.q[j]) - g.q);
+      y.mat_off = mats.size() / 2;
+      for (size_t e = 0; e < op.re.size(); ++e) {
+        mats.push_back(op.re[e]);
+        mats.push_back(op.im[e]);
+      }
+      gates.push_back(y);
+    }
+    cg.push_back(x);
+  }
+  std::vector<double> composed;
+  const int rc = compose_groups(&h->c, cg, gates, mats, &composed);
+  if (rc != kOk) return rc;
+  Circuit res;
+  res.n = c.n;
+  size_t gi = 0;
+  for (const Group& g : groups) {
+    if (g.end - g.begin == 1) {
+      res.ops.push_back(c.ops[g.begin]);
+      continue;
+    }
+    const ComposeGroup& x = cg[gi++];
+    Op op;
+    op.m = g.m;
+    std::copy(g.q, g.q + g.m, op.q);
+    const size_t n = size_t(1) << (2 * g.m);
+    op.re.resize(n);
+    op.im.resize(n);
+    for (size_t e = 0; e < n; ++e) {
+      op.re[e] = composed[2 * (x.out_off + e)];
+      op.im[e] = composed[2 * (x.out_off + e) + 1];
+    }
+    res.ops.push_back(std::move(op));
+  }
+  return wrap(std::move(res), out);
+}
+
+int qsvh_fuse_groups(const qsvh_circuit* in, uint32_t max_qubits, uint64_t* begin,
+                     uint64_t* end, uint32_t* m, uint32_t* qubits, size_t cap,
+                     size_t* n_groups) {
+  if (!in || !n_groups) return kInvalidArg;
+  if (max_qubits > kMaxGateQubits) return kTooWide;
+  const std::vector<Group> groups = greedy_groups(in->c, max_qubits);
+  *n_groups = groups.size();
+  for (size_t g = 0; g < groups.size() && g < cap; ++g) {
+    if (begin) begin[g] = groups[g].begin;
+    if (end) end[g] = groups[g].end;
+    if (m) m[g] = groups[g].m;
+    if (qubits)
+      for (uint32_t j = 0; j < kMaxGateQubits; ++j) qubits[6 * g + j] = j < groups[g].m ? groups[g].q[j] : 0;
+  }
+  return kOk;
+}
+
+// ------------------------------------------------------------------ states
+int qsvh_state_alloc(qsvh_ctx* h, uint32_t n, int dtype, qsvh_state** out) {
+  if (!h || !out) return kInvalidArg;
+  *out = nullptr;
+  if (dtype != QSVH_C128 && dtype != QSVH_C64) return set_err(&h->c, kDtype, "unknown dtype");
+  if (n > 40) return set_err(&h->c, kOom, "state too large");
+  qsvh_state* st = new (std::nothrow) qsvh_state;
+  if (!st) return kOom;
+  State& s = st->s;
+  s.ctx = &h->c;
+  s.n = n;
+  s.dtype = dtype;
+  const size_t bytes = size_t(s.dim()) * s.elem();
+  hipError_t e = hipSetDevice(h->c.device);
+  if (e == hipSuccess) e = hipMalloc(&s.re, bytes);
+  if (e == hipSuccess) e = hipMalloc(&s.im, bytes);
+  if (e != hipSuccess) {
+    hipFree(s.re);
+    hipFree(s.im);
+    delete st;
+    return hip_err(&h->c, e, "state alloc");
+  }
+  *out = st;
+  return kOk;
+}
+
+void qsvh_state_free(qsvh_state* st) {
+  if (!st) return;
+  hipStreamSynchronize(st->s.ctx->stream);
+  hipFree(st->s.re);
+  hipFree(st->s.im);
+  delete st;
+}
+
+uint32_t qsvh_state_n_qubits(const qsvh_state* st) { return st ? st->s.n : 0; }
+int qsvh_state_dtype(const qsvh_state* st) { return st ? st->s.dtype : -1; }
+
+int qsvh_init_basis(qsvh_state* st, uint64_t basis) {
+  if (!st) return kInvalidArg;
+  State& s = st->s;
+  if (basis >= s.dim()) return set_err(s.ctx, kInvalidArg, "basis index out of range");
+  return hip_err(s.ctx, launch_init_basis(s, basis, s.ctx->stream), "init_basis");
+}
+
+int qsvh_apply(qsvh_state* st, uint32_t m, const uint32_t* qubits, const double* re,
+               const double* im) {
+  if (!st || !re || !im) return kInvalidArg;
+  State& s = st->s;
+  Circuit c;
+  c.n = s.n;
+  const int v = push(&c, m, qubits, re, im);
+  if (v != kOk) return set_err(s.ctx, v, "invalid gate");
+  Program p;
+  const int rc = program_build(&s, c, nullptr, &p);
+  if (rc == kOk) {
+    const int lr = program_launch(&p);
+    program_free(&p);  // waits for the launch only when it owns device matrices
+    return lr;
+  }
+  program_free(&p);
+  return rc;
+}
+
+int qsvh_amplitude(qsvh_state* st, uint64_t index, double* re, double* im) {
+  if (!st || !re || !im) return kInvalidArg;
+  State& s = st->s;
+  if (index >= s.dim()) return set_err(s.ctx, kInvalidArg, "amplitude index out of range");
+  const size_t es = s.elem();
+  alignas(8) unsigned char br[8], bi[8];
+  hipStream_t str = s.ctx->stream;
+  hipError_t e = hipMemcpyAsync(br, static_cast<char*>(s.re) + index * es, es,
+                                hipMemcpyDeviceToHost, str);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(bi, static_cast<char*>(s.im) + index * es, es, hipMemcpyDeviceToHost, str);
+  if (e == hipSuccess) e = hipStreamSynchronize(str);
+  if (e != hipSuccess) return hip_err(s.ctx, e, "amplitude");
+  if (s.dtype == 0) {
+    std::memcpy(re, br, 8);
+    std::memcpy(im, bi, 8);
+  } else {
+    float fr, fi;
+    std::memcpy(&fr, br, 4);
+    std::memcpy(&fi, bi, 4);
+    *re = fr;
+    *im = fi;
+  }
+  return kOk;
+}
+
+int qsvh_probabilities_device(qsvh_state* st, void* dout) {
+  if (!st || !dout) return kInvalidArg;
+  return hip_err(st->s.ctx, launch_probs(st->s, dout, st->s.ctx->stream), "probabilities");
+}
+
+int qsvh_probabilities(qsvh_state* st, void* host_out) {
+  if (!st || !host_out) return kInvalidArg;
+  State& s = st->s;
+  void* d = nullptr;
+  const size_t bytes = size_t(s.dim()) * s.elem();
+  hipError_t e = hipMalloc(&d, bytes);
+  if (e != hipSuccess) return hip_err(s.ctx, e, "probabilities alloc");
+  e = launch_probs(s, d, s.ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(host_out, d, bytes, hipMemcpyDeviceToHost, s.ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s.ctx->stream);
+  hipFree(d);
+  return hip_err(s.ctx, e, "probabilities");
+}
+
+static int reduce(State& s, uint64_t mask, double* out) {
+  Ctx* c = s.ctx;
+  hipError_t e = launch_reduce(s, mask, c->d_partials, Ctx::kMaxPartials, c->d_result, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->h_result, c->d_result, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_err(c, e, "reduction");
+  *out = *c->h_result;
+  return kOk;
+}
+
+int qsvh_norm_sqr(qsvh_state* st, double* out) {
+  if (!st || !out) return kInvalidArg;
+  return reduce(st->s, 0, out);
+}
+
+int qsvh_expectation_z(qsvh_state* st, uint64_t mask, double* out) {
+  if (!st || !out) return kInvalidArg;
+  return reduce(st->s, mask & (st->s.dim() - 1), out);
+}
+
+int qsvh_download(qsvh_state* st, void* re, void* im) {
+  if (!st || !re || !im) return kInvalidArg;
+  State& s = st->s;
+  const size_t bytes = size_t(s.dim()) * s.elem();
+  hipError_t e = hipStreamSynchronize(s.ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(re, s.re, bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(im, s.im, bytes, hipMemcpyDeviceToHost);
+  return hip_err(s.ctx, e, "download");
+}
+
+int qsvh_upload(qsvh_state* st, const void* re, const void* im) {
+  if (!st || !re || !im) return kInvalidArg;
+  State& s = st->s;
+  const size_t bytes = size_t(s.dim()) * s.elem();
+  hipError_t e = hipStreamSynchronize(s.ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(s.re, re, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(s.im, im, bytes, hipMemcpyHostToDevice);
+  return hip_err(s.ctx, e, "upload");
+}
+
+int qsvh_init_random_normal(qsvh_state* st, uint64_t seed) {
+  if (!st) return kInvalidArg;
+  State& s = st->s;
+  Ctx* c = s.ctx;
+  return hip_err(c, launch_init_random(s, seed, c->d_partials, Ctx::kMaxPartials, c->d_result,
+                                       c->stream),
+                 "init_random_normal");
+}
+
+int qsvh_compare(qsvh_state* a, qsvh_state* b, double* max_abs_err, double* fidelity) {
+  if (!a || !b) return kInvalidArg;
+  Ctx* c = a->s.ctx;
+  if (a->s.n != b->s.n) return set_err(c, kInvalidArg, "compare: qubit counts differ");
+  if (a->s.ctx != b->s.ctx) {
+    // Different contexts: order b's queue before a's by waiting for it.
+    const hipError_t e = hipStreamSynchronize(b->s.ctx->stream);
+    if (e != hipSuccess) return hip_err(c, e, "compare");
+  }
+  double* d_out = nullptr;
+  hipError_t e = hipMalloc(&d_out, 5 * sizeof(double));
+  if (e != hipSuccess) return hip_err(c, e, "compare alloc");
+  double h[5] = {0, 0, 0, 0, 0};
+  e = launch_compare(a->s, b->s, c->d_partials, d_out, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(d_out);
+  if (e != hipSuccess) return hip_err(c, e, "compare");
+  if (max_abs_err) *max_abs_err = h[0];
+  if (fidelity) {
+    const double den = h[3] * h[4];
+    *fidelity = den > 0 ? (h[1] * h[1] + h[2] * h[2]) / den : 0.0;
+  }
+  return kOk;
+}
+
+int qsvh_synchronize(qsvh_state* st) {
+  if (!st) return kInvalidArg;
+  return hip_err(st->s.ctx, hipStreamSynchronize(st->s.ctx->stream), "synchronize");
+}
+
+int qsvh_execute(qsvh_ctx* h, const qsvh_circuit* c, int dtype, qsvh_state** out) {
+  if (!h || !c || !out) return kInvalidArg;
+  int rc = qsvh_state_alloc(h, c->c.n, dtype, out);
+  if (rc != kOk) return rc;
+  rc = qsvh_init_basis(*out, 0);
+  if (rc == kOk) {
+    Program p;
+    rc = program_build(&(*out)->s, c->c, nullptr, &p);
+    if (rc == kOk) rc = program_launch(&p);
+    program_free(&p);
+  }
+  if (rc != kOk) {
+    qsvh_state_free(*out);
+    *out = nullptr;
+  }
+  return rc;
+}
+
+// ------------------------------------------------------------------ programs
+int qsvh_program_create(qsvh_state* st, const qsvh_circuit* c, const qsvh_program_opts* opts,
+                        qsvh_program** out) {
+  if (!st || !c || !out) return kInvalidArg;
+  *out = nullptr;
+  qsvh_program* p = new (std::nothrow) qsvh_program;
+  if (!p) return kOom;
+  const int rc = program_build(&st->s, c->c, opts, &p->p);
+  if (rc != kOk) {
+    program_free(&p->p);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return kOk;
+}
+
+int qsvh_program_launch(qsvh_program* p) {
+  if (!p) return kInvalidArg;
+  return program_launch(&p->p);
+}
+
+int qsvh_program_time(qsvh_program* p, double* ms) {
+  if (!p || !ms) return kInvalidArg;
+  Ctx* c = p->p.st->ctx;
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return set_err(c, kHip, "event");
+  if (hipEventCreate(&b) != hipSuccess) {
+    hipEventDestroy(a);
+    return set_err(c, kHip, "event");
+  }
+  hipError_t e = hipEventRecord(a, c->stream);
+  int rc = kOk;
+  if (e == hipSuccess) rc = program_launch(&p->p);
+  if (e == hipSuccess && rc == kOk) e = hipEventRecord(b, c->stream);
+  if (e == hipSuccess && rc == kOk) e = hipEventSynchronize(b);
+  float f = 0.f;
+  if (e == hipSuccess && rc == kOk) e = hipEventElapsedTime(&f, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (rc != kOk) return rc;
+  if (e != hipSuccess) return hip_err(c, e, "program time");
+  *ms = f;
+  return kOk;
+}
+
+int qsvh_program_num_kernels(const qsvh_program* p, uint64_t* out) {
+  if (!p || !out) return kInvalidArg;
+  *out = p->p.launches.size();
+  return kOk;
+}
+
+void qsvh_program_destroy(qsvh_program* p) {
+  if (!p) return;
+  program_free(&p->p);
+  delete p;
+}
+
+static int time_launches(Ctx* c, uint32_t reps, double* ms, const std::function<hipError_t()>& f) {
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return set_err(c, kHip, "event");
+  if (hipEventCreate(&b) != hipSuccess) {
+    hipEventDestroy(a);
+    return set_err(c, kHip, "event");
+  }
+  hipError_t e = f();  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(a, c->stream);
+  for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) e = f();
+  if (e == hipSuccess) e = hipEventRecord(b, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (e != hipSuccess) return hip_err(c, e, "timing");
+  *ms = double(t) / double(reps ? reps : 1);
+  return kOk;
+}
+
+int qsvh_time_gate(qsvh_state* st, uint32_t m, const uint32_t* qubits, const double* re,
+                   const double* im, uint32_t reps, double* ms) {
+  if (!st || !re || !im || !ms || reps == 0) return kInvalidArg;
+  State& s = st->s;
+  Circuit c;
+  c.n = s.n;
+  const int v = push(&c, m, qubits, re, im);
+  if (v != kOk) return set_err(s.ctx, v, "invalid gate");
+  Program p;
+  int rc = program_build(&s, c, nullptr, &p);
+  if (rc == kOk)
+    rc = time_launches(s.ctx, reps, ms, [&]() { return issue_all(&p); });
+  program_free(&p);
+  return rc;
+}
+
+int qsvh_time_copy(qsvh_state* st, uint32_t reps, double* ms) {
+  if (!st || !ms || reps == 0) return kInvalidArg;
+  State& s = st->s;
+  const size_t bytes = size_t(s.dim()) * s.elem();
+  void *dr = nullptr, *di = nullptr;
+  hipError_t e = hipMalloc(&dr, bytes);
+  if (e == hipSuccess) e = hipMalloc(&di, bytes);
+  if (e != hipSuccess) {
+    hipFree(dr);
+    hipFree(di);
+    return hip_err(s.ctx, e, "copy alloc");
+  }
+  const int rc =
+      time_launches(s.ctx, reps, ms, [&]() { return launch_copy(s, dr, di, s.ctx->stream); });
+  hipStreamSynchronize(s.ctx->stream);
+  hipFree(dr);
+  hipFree(di);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// engine.hpp -- internal types shared by the C ABI (api.cpp) and the kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ir.hpp"
+
+namespace qsvh {
+
+// Launch-shape knobs; defaults chosen by measurement on MI355X, overridable per context
+// (the autotuner writes them).
+struct Tuning {
+  int stream_block = 256;      // threads per block for the streaming kernels
+  int stream_max_blocks = 0;   // 0: one thread per work item; >0: grid-stride cap
+  int unroll = 2;              // independent vector items per thread per iteration
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  Tuning tune;
+  int cus = 0;
+  // Reduction scratch: per-block f64 partials and the final value.
+  double* d_partials = nullptr;
+  double* d_result = nullptr;
+  double* h_result = nullptr;  // pinned
+  static constexpr int kMaxPartials = 4096;
+};
+
+struct State {
+  Ctx* ctx = nullptr;
+  uint32_t n = 0;
+  int dtype = 0;  // 0 = c128 (f64 SoA), 1 = c64 (f32 SoA)
+  void* re = nullptr;
+  void* im = nullptr;
+  uint64_t dim() const { return 1ull << n; }
+  size_t elem() const { return dtype == 0 ? sizeof(double) : sizeof(float); }
+};
+
+// ---- gate descriptors passed by value to the kernels -------------------------------
+// Qubit index data for an m-qubit gate: sorted positions (for insert_zero_bits) and
+// the offset of each internal sub-index s, scatter_bits(s, q) (layout.rs:54-60).
+struct MqIndex {
+  uint32_t m;
+  uint32_t sorted[kMaxGateQubits];
+  uint64_t off[64];
+};
+
+// One kernel launch of the plain executor.
+struct Launch {
+  enum Kind { k1q, kDiag, kMq, kFusedMfma } kind;
+  uint32_t m = 0;
+  uint32_t q[kMaxGateQubits] = {0, 0, 0, 0, 0, 0};
+  // matrix in f64 (interleaved re, im), row-major; for diagonal launches the 2^m phases.
+  std::vector<double> mat;
+  uint64_t arena_off = 0;  // element offset of the matrix in the device arena
+};
+
+// ---- launchers (kernels_plain.hip) ---------------------------------------------------
+hipError_t launch_init_basis(const State& st, uint64_t basis, hipStream_t s);
+hipError_t launch_1q(const State& st, uint32_t q, const double* g /*8: re,im interleaved*/,
+                     const Tuning& t, hipStream_t s);
+hipError_t launch_diag(const State& st, uint32_t m, const uint32_t* q,
+                       const double* d /*2*2^m interleaved*/, const Tuning& t,
+                       hipStream_t s);
+// m-qubit dense gate; `dmat` is the device copy of the matrix in the state's precision
+// (interleaved re, im), needed for m >= 4 (smaller gates travel as kernel arguments).
+hipError_t launch_mq(const State& st, uint32_t m, const uint32_t* q, const double* g,
+                     const void* dmat, const Tuning& t, hipStream_t s);
+hipError_t launch_probs(const State& st, void* out, hipStream_t s);
+// f64 sum of |psi_i|^2 * sign(i & mask) (mask 0: the norm) into *d_result.
+hipError_t launch_reduce(const State& st, uint64_t mask, double* d_partials, int max_partials,
+                         double* d_result, hipStream_t s);
+hipError_t launch_copy(const State& src, void* dst_re, void* dst_im, hipStream_t s);
+
+// ---- kernels_util.hip ------------------------------------------------------------------
+// Random normalised start state (counter-based SplitMix64 + Box-Muller); `d_norm` receives
+// the pre-normalisation <psi|psi>.
+hipError_t launch_init_random(const State& st, uint64_t seed, double* part, int max_part,
+                              double* d_norm, hipStream_t s);
+// out[0..5) = max |a-b|, Re<a|b>, Im<a|b>, <a|a>, <b|b>; `part` holds 5 * 512 doubles.
+hipError_t launch_compare(const State& a, const State& b, double* part, double* out,
+                          hipStream_t s);
+
+// ---- fusion (kernels_fused.hip) ------------------------------------------------------
+// Compose every group's product in f64 on the device.  `ops` lists, per group, the
+// member gates (local qubit positions + matrices).  Output: row-major 4^k entries per
+// group at the given offsets (interleaved re, im).
+struct ComposeGate {
+  uint32_t m;
+  uint32_t local[kMaxGateQubits];  // positions within the group's sorted qubit list
+  uint64_t mat_off;                // offset (complex entries) into the packed matrices
+};
+struct ComposeGroup {
+  uint32_t k;
+  uint32_t first_gate, n_gates;
+  uint64_t out_off;  // offset (complex entries) into the output
+};
+int compose_groups(Ctx* ctx, const std::vector<ComposeGroup>& groups,
+                   const std::vector<ComposeGate>& gates, const std::vector<double>& mats,
+                   std::vector<double>* out);
+
+}  // namespace qsvh

@@ -1,0 +1,663 @@
+// kernels_plain.hip -- the plain (unfused, exact-precision) gate kernels for gfx950.
+//
+// Amplitudes are SoA: re[2^n], im[2^n] of T (double for complex128, float for complex64).
+// Every streaming kernel moves 16-byte vectors (W = 2 doubles or 4 floats per access),
+// so a wave64 load instruction covers 1 KiB of contiguous HBM for each component array.
+//
+// Arithmetic is the reference's, operation for operation and without contraction (this
+// file is compiled with -ffp-contract=off): complex products as (ac - bd, ad + bc), sums
+// in the reference's order.  The complex128 results are therefore bit-identical to
+// CpuBackend (crates/qsv-core/src/backend/cpu.rs:71-114,221-247).
+#include <hip/hip_runtime.h>
+
+#include "engine.hpp"
+
+namespace qsvh {
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename T, int W>
+struct Vec {
+  using type = T __attribute__((ext_vector_type(W)));
+};
+
+template <typename T, int W>
+__device__ __forceinline__ void vload(const T* p, T (&x)[W]) {
+  if constexpr (W == 1) {
+    x[0] = p[0];
+  } else {
+    const typename Vec<T, W>::type v = *reinterpret_cast<const typename Vec<T, W>::type*>(p);
+#pragma unroll
+    for (int j = 0; j < W; ++j) x[j] = v[j];
+  }
+}
+
+template <typename T, int W>
+__device__ __forceinline__ void vstore(T* p, const T (&x)[W]) {
+  if constexpr (W == 1) {
+    p[0] = x[0];
+  } else {
+    typename Vec<T, W>::type v;
+#pragma unroll
+    for (int j = 0; j < W; ++j) v[j] = x[j];
+    *reinterpret_cast<typename Vec<T, W>::type*>(p) = v;
+  }
+}
+
+// 2x2 gate, row-major entries g00 g01 g10 g11.
+template <typename T>
+struct G2 {
+  T r[4], i[4];
+};
+
+// one_qubit_block body (cpu.rs:84-93): y0 = g00 x0 + g01 x1, y1 = g10 x0 + g11 x1.
+template <typename T>
+__device__ __forceinline__ void pair_update(T& x0r, T& x0i, T& x1r, T& x1i, const G2<T>& g) {
+  const T ar = g.r[0] * x0r - g.i[0] * x0i, ai = g.r[0] * x0i + g.i[0] * x0r;
+  const T br = g.r[1] * x1r - g.i[1] * x1i, bi = g.r[1] * x1i + g.i[1] * x1r;
+  const T cr = g.r[2] * x0r - g.i[2] * x0i, ci = g.r[2] * x0i + g.i[2] * x0r;
+  const T dr = g.r[3] * x1r - g.i[3] * x1i, di = g.r[3] * x1i + g.i[3] * x1r;
+  x0r = ar + br;
+  x0i = ai + bi;
+  x1r = cr + dr;
+  x1i = ci + di;
+}
+
+// ---------------------------------------------------------------------------- 1q
+// Target q >= log2(W): each work item is W consecutive "lower" amplitudes, i.e. the
+// nested-block walk of cpu.rs:148-175 with the 2^(n-1) pairs enumerated as
+// lo = insert_zero_bit(x, q), hi = lo + 2^q (layout.rs:22-26).  Consecutive lanes read
+// consecutive 16-byte vectors of both halves: every access is coalesced for any q.
+template <typename T, int W, int U>
+__global__ __launch_bounds__(kBlock) void k_1q_wide(T* __restrict__ re, T* __restrict__ im,
+                                                    uint64_t n_items, uint32_t q, G2<T> g) {
+  const uint64_t lowmask = (1ull << q) - 1;
+  const uint64_t half = 1ull << q;
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v0 < n_items;
+       v0 += nthr * U) {
+    T ar[U][W], ai[U][W], br[U][W], bi[U][W];
+    uint64_t lo[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t v = v0 + uint64_t(u) * nthr;
+      const uint64_t x = v * W;
+      lo[u] = ((x & ~lowmask) << 1) | (x & lowmask);
+      if (v < n_items) {
+        vload<T, W>(re + lo[u], ar[u]);
+        vload<T, W>(im + lo[u], ai[u]);
+        vload<T, W>(re + lo[u] + half, br[u]);
+        vload<T, W>(im + lo[u] + half, bi[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t v = v0 + uint64_t(u) * nthr;
+      if (v < n_items) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) pair_update(ar[u][j], ai[u][j], br[u][j], bi[u][j], g);
+        vstore<T, W>(re + lo[u], ar[u]);
+        vstore<T, W>(im + lo[u], ai[u]);
+        vstore<T, W>(re + lo[u] + half, br[u]);
+        vstore<T, W>(im + lo[u] + half, bi[u]);
+      }
+    }
+  }
+}
+
+// Target q < log2(W): both partners of every pair sit inside one 16-byte vector, so each
+// work item is one vector of W amplitudes holding W/2 complete pairs.  Q is the target
+// qubit (a compile-time constant so the pairing stays in registers).
+template <typename T, int W, int Q, int U>
+__global__ __launch_bounds__(kBlock) void k_1q_intra(T* __restrict__ re, T* __restrict__ im,
+                                                     uint64_t n_items, G2<T> g) {
+  constexpr int h = 1 << Q;
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v0 < n_items;
+       v0 += nthr * U) {
+    T xr[U][W], xi[U][W];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t v = v0 + uint64_t(u) * nthr;
+      if (v < n_items) {
+        vload<T, W>(re + v * W, xr[u]);
+        vload<T, W>(im + v * W, xi[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t v = v0 + uint64_t(u) * nthr;
+      if (v < n_items) {
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+          if (((j >> Q) & 1) == 0) pair_update(xr[u][j], xi[u][j], xr[u][j + h], xi[u][j + h], g);
+        vstore<T, W>(re + v * W, xr[u]);
+        vstore<T, W>(im + v * W, xi[u]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------- diagonal
+// diag_chunk (cpu.rs:99-114): psi_i <- d[gather_bits(i, qs)] * psi_i in one sequential
+// pass (layout.rs:43-49).  The 2^m phases are staged in LDS.
+template <typename T>
+struct DiagArg {
+  uint32_t m;
+  uint32_t q[kMaxGateQubits];
+  T dr[64], di[64];
+};
+
+template <typename T, int W, int U>
+__global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ re, T* __restrict__ im,
+                                                 uint64_t n_items, DiagArg<T> a) {
+  __shared__ T sdr[64], sdi[64];
+  const uint32_t sub = 1u << a.m;
+  if (threadIdx.x < sub) {
+    sdr[threadIdx.x] = a.dr[threadIdx.x];
+    sdi[threadIdx.x] = a.di[threadIdx.x];
+  }
+  __syncthreads();
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v0 < n_items;
+       v0 += nthr * U) {
+    T xr[U][W], xi[U][W];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t v = v0 + uint64_t(u) * nthr;
+      if (v < n_items) {
+        vload<T, W>(re + v * W, xr[u]);
+        vload<T, W>(im + v * W, xi[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t v = v0 + uint64_t(u) * nthr;
+      if (v < n_items) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          const uint64_t i = v * W + j;
+          uint32_t s = 0;
+          for (uint32_t b = 0; b < a.m; ++b) s |= uint32_t((i >> a.q[b]) & 1u) << b;
+          const T dr = sdr[s], di = sdi[s];
+          const T x_r = xr[u][j], x_i = xi[u][j];
+          xr[u][j] = dr * x_r - di * x_i;
+          xi[u][j] = dr * x_i + di * x_r;
+        }
+        vstore<T, W>(re + v * W, xr[u]);
+        vstore<T, W>(im + v * W, xi[u]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------- m-qubit
+// insert_zero_bits over strictly ascending positions (layout.rs:70-87).
+template <int M>
+__device__ __forceinline__ uint64_t insert_zero_bits(uint64_t x, const uint32_t* sorted) {
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    const uint32_t b = sorted[j];
+    x = ((x >> b) << (b + 1)) | (x & ((1ull << b) - 1));
+  }
+  return x;
+}
+
+template <int M>
+struct MqIdx {
+  uint32_t sorted[M];
+  uint64_t off[1 << M];  // scatter_bits(s, qs) for every internal sub-index s
+};
+
+template <typename T, int M>
+struct MatArg {
+  T r[1 << (2 * M)], i[1 << (2 * M)];
+};
+
+// mq_block (cpu.rs:221-247) for m = 2, 3 with the matrix in kernel arguments (scalar
+// registers).  A work item is VW consecutive blocks o..o+VW-1; when the lowest gate qubit
+// is >= log2(VW) their amplitudes are consecutive, so every gather is a 16-byte vector.
+template <typename T, int M, int VW>
+__global__ __launch_bounds__(kBlock) void k_mq_reg(T* __restrict__ re, T* __restrict__ im,
+                                                   uint64_t n_items, MqIdx<M> idx,
+                                                   MatArg<T, M> g) {
+  constexpr int D = 1 << M;
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+    const uint64_t base = insert_zero_bits<M>(v * VW, idx.sorted);
+    T ar[D][VW], ai[D][VW];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+      vload<T, VW>(re + base + idx.off[s], ar[s]);
+      vload<T, VW>(im + base + idx.off[s], ai[s]);
+    }
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      T yr[VW], yi[VW];
+#pragma unroll
+      for (int w = 0; w < VW; ++w) {
+        T accr = T(0), acci = T(0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          const T er = g.r[r * D + c], ei = g.i[r * D + c];
+          const T pr = er * ar[c][w] - ei * ai[c][w];
+          const T pi = er * ai[c][w] + ei * ar[c][w];
+          accr = accr + pr;
+          acci = acci + pi;
+        }
+        yr[w] = accr;
+        yi[w] = acci;
+      }
+      vstore<T, VW>(re + base + idx.off[r], yr);
+      vstore<T, VW>(im + base + idx.off[r], yi);
+    }
+  }
+}
+
+// mq_block for m = 4..6 (fused groups): a workgroup stages a tile of C consecutive blocks
+// (2^m x C amplitudes) and the gate matrix in LDS, then each lane forms outputs
+// out[s][c] = sum_j U[s][j] X[j][c] in ascending j (the reference's order).  Loads run
+// along c, the direction in which the blocks' amplitudes are consecutive.
+template <typename T, int M>
+__global__ __launch_bounds__(kBlock) void k_mq_lds(T* __restrict__ re, T* __restrict__ im,
+                                                   uint64_t n_blocks, MqIdx<M> idx,
+                                                   const T* __restrict__ mat) {
+  constexpr int D = 1 << M;
+  constexpr int C = 1024 / D;
+  __shared__ T ur[D * D], ui[D * D];
+  __shared__ T xr[D * C], xi[D * C];
+  for (int e = threadIdx.x; e < D * D; e += kBlock) {
+    ur[e] = mat[2 * e];
+    ui[e] = mat[2 * e + 1];
+  }
+  const uint64_t n_tiles = (n_blocks + C - 1) / C;
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < D * C; e += kBlock) {
+      const int c = e % C, s = e / C;
+      const uint64_t o = t * C + c;
+      if (o < n_blocks) {
+        const uint64_t a = insert_zero_bits<M>(o, idx.sorted) + idx.off[s];
+        xr[e] = re[a];
+        xi[e] = im[a];
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < D * C; e += kBlock) {
+      const int c = e % C, s = e / C;
+      const uint64_t o = t * C + c;
+      if (o >= n_blocks) continue;
+      T accr = T(0), acci = T(0);
+      for (int j = 0; j < D; ++j) {
+        const T er = ur[s * D + j], ei = ui[s * D + j];
+        const T x_r = xr[j * C + c], x_i = xi[j * C + c];
+        const T pr = er * x_r - ei * x_i;
+        const T pi = er * x_i + ei * x_r;
+        accr = accr + pr;
+        acci = acci + pi;
+      }
+      const uint64_t a = insert_zero_bits<M>(o, idx.sorted) + idx.off[s];
+      re[a] = accr;
+      im[a] = acci;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------- init / reduce
+template <typename T, int W>
+__global__ __launch_bounds__(kBlock) void k_init_basis(T* __restrict__ re, T* __restrict__ im,
+                                                       uint64_t n_items, uint64_t basis) {
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+    T zr[W], zi[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      zr[j] = (v * W + j == basis) ? T(1) : T(0);
+      zi[j] = T(0);
+    }
+    vstore<T, W>(re + v * W, zr);
+    vstore<T, W>(im + v * W, zi);
+  }
+}
+
+// StateVector::probabilities (state/mod.rs:130-134): p_i = re^2 + im^2 in T.
+template <typename T, int W>
+__global__ __launch_bounds__(kBlock) void k_probs(const T* __restrict__ re,
+                                                  const T* __restrict__ im,
+                                                  T* __restrict__ p, uint64_t n_items) {
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+    T xr[W], xi[W], y[W];
+    vload<T, W>(re + v * W, xr);
+    vload<T, W>(im + v * W, xi);
+#pragma unroll
+    for (int j = 0; j < W; ++j) y[j] = xr[j] * xr[j] + xi[j] * xi[j];
+    vstore<T, W>(p + v * W, y);
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// f64 partial sums of sign(i & mask) * |psi_i|^2 (norm when mask == 0; the expectation
+// of sample.rs:177-200 otherwise), one partial per block; fixed order, so repeatable.
+template <typename T, int W>
+__global__ __launch_bounds__(kBlock) void k_reduce(const T* __restrict__ re,
+                                                   const T* __restrict__ im, uint64_t n_items,
+                                                   uint64_t mask, double* __restrict__ part) {
+  __shared__ double ws[kBlock / 64];
+  double acc = 0.0;
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+    T xr[W], xi[W];
+    vload<T, W>(re + v * W, xr);
+    vload<T, W>(im + v * W, xi);
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const double a = double(xr[j]), b = double(xi[j]);
+      const double pj = a * a + b * b;
+      acc += (__popcll((v * W + j) & mask) & 1) ? -pj : pj;
+    }
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += ws[w];
+    part[blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_final(const double* __restrict__ part, int n,
+                                                         double* __restrict__ out) {
+  __shared__ double ws[kBlock / 64];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock) acc += part[i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += ws[w];
+    *out = s;
+  }
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(kBlock) void k_copy(const T* __restrict__ sr,
+                                                 const T* __restrict__ si, T* __restrict__ dr,
+                                                 T* __restrict__ di, uint64_t n_items) {
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+    T a[W], b[W];
+    vload<T, W>(sr + v * W, a);
+    vload<T, W>(si + v * W, b);
+    vstore<T, W>(dr + v * W, a);
+    vstore<T, W>(di + v * W, b);
+  }
+}
+
+// ------------------------------------------------------------------- host helpers
+unsigned grid_for(uint64_t items, int unroll, const Tuning& t) {
+  const uint64_t per = uint64_t(kBlock) * uint64_t(unroll);
+  uint64_t g = (items + per - 1) / per;
+  if (t.stream_max_blocks > 0 && g > uint64_t(t.stream_max_blocks)) g = t.stream_max_blocks;
+  if (g < 1) g = 1;
+  if (g > 0x7fffffffull) g = 0x7fffffffull;
+  return unsigned(g);
+}
+
+template <typename T>
+constexpr int vec_width() {
+  return 16 / int(sizeof(T));
+}
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+
+template <typename T, int W>
+hipError_t launch_1q_t(const State& st, uint32_t q, const G2<T>& g, const Tuning& t,
+                       hipStream_t s) {
+  const uint64_t dim = st.dim();
+  T* re = static_cast<T*>(st.re);
+  T* im = static_cast<T*>(st.im);
+  if (int(q) >= ilog2(W)) {
+    const uint64_t items = (dim >> 1) / W;
+    switch (t.unroll) {
+      case 1:
+        k_1q_wide<T, W, 1><<<grid_for(items, 1, t), kBlock, 0, s>>>(re, im, items, q, g);
+        break;
+      case 4:
+        k_1q_wide<T, W, 4><<<grid_for(items, 4, t), kBlock, 0, s>>>(re, im, items, q, g);
+        break;
+      default:
+        k_1q_wide<T, W, 2><<<grid_for(items, 2, t), kBlock, 0, s>>>(re, im, items, q, g);
+    }
+  } else {
+    const uint64_t items = dim / W;
+    if constexpr (W >= 2) {
+      if (q == 0)
+        k_1q_intra<T, W, 0, 2><<<grid_for(items, 2, t), kBlock, 0, s>>>(re, im, items, g);
+    }
+    if constexpr (W >= 4) {
+      if (q == 1)
+        k_1q_intra<T, W, 1, 2><<<grid_for(items, 2, t), kBlock, 0, s>>>(re, im, items, g);
+    }
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_1q_any(const State& st, uint32_t q, const double* gd, const Tuning& t,
+                         hipStream_t s) {
+  G2<T> g;
+  for (int e = 0; e < 4; ++e) {
+    g.r[e] = T(gd[2 * e]);
+    g.i[e] = T(gd[2 * e + 1]);
+  }
+  constexpr int W = vec_width<T>();
+  if (st.dim() >= uint64_t(2 * W)) return launch_1q_t<T, W>(st, q, g, t, s);
+  return launch_1q_t<T, 1>(st, q, g, t, s);
+}
+
+template <typename T>
+hipError_t launch_diag_any(const State& st, uint32_t m, const uint32_t* q, const double* d,
+                           const Tuning& t, hipStream_t s) {
+  DiagArg<T> a;
+  a.m = m;
+  for (uint32_t j = 0; j < kMaxGateQubits; ++j) a.q[j] = j < m ? q[j] : 0;
+  for (uint32_t e = 0; e < 64; ++e) {
+    a.dr[e] = e < (1u << m) ? T(d[2 * e]) : T(0);
+    a.di[e] = e < (1u << m) ? T(d[2 * e + 1]) : T(0);
+  }
+  T* re = static_cast<T*>(st.re);
+  T* im = static_cast<T*>(st.im);
+  constexpr int W = vec_width<T>();
+  if (st.dim() >= uint64_t(W)) {
+    const uint64_t items = st.dim() / W;
+    k_diag<T, W, 2><<<grid_for(items, 2, t), kBlock, 0, s>>>(re, im, items, a);
+  } else {
+    const uint64_t items = st.dim();
+    k_diag<T, 1, 2><<<grid_for(items, 2, t), kBlock, 0, s>>>(re, im, items, a);
+  }
+  return hipGetLastError();
+}
+
+template <int M>
+MqIdx<M> make_idx(const uint32_t* q) {
+  MqIdx<M> idx;
+  uint32_t sorted[M];
+  for (int j = 0; j < M; ++j) sorted[j] = q[j];
+  for (int a = 1; a < M; ++a)
+    for (int b = a; b > 0 && sorted[b - 1] > sorted[b]; --b) {
+      const uint32_t x = sorted[b];
+      sorted[b] = sorted[b - 1];
+      sorted[b - 1] = x;
+    }
+  for (int j = 0; j < M; ++j) idx.sorted[j] = sorted[j];
+  for (int sidx = 0; sidx < (1 << M); ++sidx) {
+    uint64_t off = 0;
+    for (int j = 0; j < M; ++j) off |= uint64_t((sidx >> j) & 1) << q[j];
+    idx.off[sidx] = off;
+  }
+  return idx;
+}
+
+template <typename T, int M>
+hipError_t launch_mq_reg(const State& st, const uint32_t* q, const double* gd,
+                         const Tuning& t, hipStream_t s) {
+  const MqIdx<M> idx = make_idx<M>(q);
+  MatArg<T, M> g;
+  for (int e = 0; e < (1 << (2 * M)); ++e) {
+    g.r[e] = T(gd[2 * e]);
+    g.i[e] = T(gd[2 * e + 1]);
+  }
+  T* re = static_cast<T*>(st.re);
+  T* im = static_cast<T*>(st.im);
+  constexpr int W = vec_width<T>();
+  const uint64_t blocks = st.dim() >> M;
+  if (int(idx.sorted[0]) >= ilog2(W) && blocks >= uint64_t(W)) {
+    const uint64_t items = blocks / W;
+    k_mq_reg<T, M, W><<<grid_for(items, 1, t), kBlock, 0, s>>>(re, im, items, idx, g);
+  } else {
+    k_mq_reg<T, M, 1><<<grid_for(blocks, 1, t), kBlock, 0, s>>>(re, im, blocks, idx, g);
+  }
+  return hipGetLastError();
+}
+
+template <typename T, int M>
+hipError_t launch_mq_lds(const State& st, const uint32_t* q, const void* dmat,
+                         hipStream_t s) {
+  const MqIdx<M> idx = make_idx<M>(q);
+  constexpr int C = 1024 >> M;
+  const uint64_t blocks = st.dim() >> M;
+  uint64_t tiles = (blocks + C - 1) / C;
+  if (tiles > 65536) tiles = 65536;
+  k_mq_lds<T, M><<<unsigned(tiles), kBlock, 0, s>>>(static_cast<T*>(st.re),
+                                                     static_cast<T*>(st.im), blocks, idx,
+                                                     static_cast<const T*>(dmat));
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_mq_any(const State& st, uint32_t m, const uint32_t* q, const double* g,
+                         const void* dmat, const Tuning& t, hipStream_t s) {
+  switch (m) {
+    case 2: return launch_mq_reg<T, 2>(st, q, g, t, s);
+    case 3: return launch_mq_reg<T, 3>(st, q, g, t, s);
+    case 4: return launch_mq_lds<T, 4>(st, q, dmat, s);
+    case 5: return launch_mq_lds<T, 5>(st, q, dmat, s);
+    case 6: return launch_mq_lds<T, 6>(st, q, dmat, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t launch_init_t(const State& st, uint64_t basis, hipStream_t s) {
+  constexpr int W = vec_width<T>();
+  T* re = static_cast<T*>(st.re);
+  T* im = static_cast<T*>(st.im);
+  Tuning t;
+  if (st.dim() >= uint64_t(W)) {
+    const uint64_t items = st.dim() / W;
+    k_init_basis<T, W><<<grid_for(items, 1, t), kBlock, 0, s>>>(re, im, items, basis);
+  } else {
+    k_init_basis<T, 1><<<1, kBlock, 0, s>>>(re, im, st.dim(), basis);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_probs_t(const State& st, void* out, hipStream_t s) {
+  constexpr int W = vec_width<T>();
+  const T* re = static_cast<const T*>(st.re);
+  const T* im = static_cast<const T*>(st.im);
+  Tuning t;
+  if (st.dim() >= uint64_t(W)) {
+    const uint64_t items = st.dim() / W;
+    k_probs<T, W><<<grid_for(items, 1, t), kBlock, 0, s>>>(re, im, static_cast<T*>(out), items);
+  } else {
+    k_probs<T, 1><<<1, kBlock, 0, s>>>(re, im, static_cast<T*>(out), st.dim());
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_reduce_t(const State& st, uint64_t mask, double* part, int max_part,
+                           double* result, hipStream_t s) {
+  constexpr int W = vec_width<T>();
+  const T* re = static_cast<const T*>(st.re);
+  const T* im = static_cast<const T*>(st.im);
+  const bool vec = st.dim() >= uint64_t(W);
+  const uint64_t items = vec ? st.dim() / W : st.dim();
+  uint64_t g = (items + kBlock * 8 - 1) / (kBlock * 8);
+  if (g < 1) g = 1;
+  if (g > uint64_t(max_part)) g = max_part;
+  if (vec)
+    k_reduce<T, W><<<unsigned(g), kBlock, 0, s>>>(re, im, items, mask, part);
+  else
+    k_reduce<T, 1><<<unsigned(g), kBlock, 0, s>>>(re, im, items, mask, part);
+  k_reduce_final<<<1, kBlock, 0, s>>>(part, int(g), result);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_copy_t(const State& st, void* dr, void* di, hipStream_t s) {
+  constexpr int W = vec_width<T>();
+  Tuning t;
+  if (st.dim() >= uint64_t(W)) {
+    const uint64_t items = st.dim() / W;
+    k_copy<T, W><<<grid_for(items, 1, t), kBlock, 0, s>>>(
+        static_cast<const T*>(st.re), static_cast<const T*>(st.im), static_cast<T*>(dr),
+        static_cast<T*>(di), items);
+  } else {
+    k_copy<T, 1><<<1, kBlock, 0, s>>>(static_cast<const T*>(st.re),
+                                      static_cast<const T*>(st.im), static_cast<T*>(dr),
+                                      static_cast<T*>(di), st.dim());
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_init_basis(const State& st, uint64_t basis, hipStream_t s) {
+  return st.dtype == 0 ? launch_init_t<double>(st, basis, s) : launch_init_t<float>(st, basis, s);
+}
+
+hipError_t launch_1q(const State& st, uint32_t q, const double* g, const Tuning& t,
+                     hipStream_t s) {
+  return st.dtype == 0 ? launch_1q_any<double>(st, q, g, t, s)
+                       : launch_1q_any<float>(st, q, g, t, s);
+}
+
+hipError_t launch_diag(const State& st, uint32_t m, const uint32_t* q, const double* d,
+                       const Tuning& t, hipStream_t s) {
+  return st.dtype == 0 ? launch_diag_any<double>(st, m, q, d, t, s)
+                       : launch_diag_any<float>(st, m, q, d, t, s);
+}
+
+hipError_t launch_mq(const State& st, uint32_t m, const uint32_t* q, const double* g,
+                     const void* dmat, const Tuning& t, hipStream_t s) {
+  return st.dtype == 0 ? launch_mq_any<double>(st, m, q, g, dmat, t, s)
+                       : launch_mq_any<float>(st, m, q, g, dmat, t, s);
+}
+
+hipError_t launch_probs(const State& st, void* out, hipStream_t s) {
+  return st.dtype == 0 ? launch_probs_t<double>(st, out, s) : launch_probs_t<float>(st, out, s);
+}
+
+hipError_t launch_reduce(const State& st, uint64_t mask, double* part, int max_part,
+                         double* result, hipStream_t s) {
+  return st.dtype == 0 ? launch_reduce_t<double>(st, mask, part, max_part, result, s)
+                       : launch_reduce_t<float>(st, mask, part, max_part, result, s);
+}
+
+hipError_t launch_copy(const State& st, void* dr, void* di, hipStream_t s) {
+  return st.dtype == 0 ? launch_copy_t<double>(st, dr, di, s) : launch_copy_t<float>(st, dr, di, s);
+}
+
+}  // namespace qsvh

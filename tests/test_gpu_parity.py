@@ -1,0 +1,279 @@
+"""GPU parity: the HIP engine through the C ABI against the CPU oracle and the golden
+fixtures, on the reference's own differential recipes
+(crates/qsv-core/tests/equivalence.rs, crates/qsv-cuda/tests/equivalence.rs).
+
+Tolerances: the plain complex128 path must match to <= 1e-12 max amplitude error; it is
+in fact bit-identical to the CpuBackend restatement (same operations, same order, no
+contraction), which is asserted as well.  complex64 is compared with the oracle's f32
+CpuBackend (also bit-identical) and with the f64 result to 1e-5."""
+import numpy as np
+import pytest
+
+import oracle
+import rustatevec_amd as rs
+from tests.helpers import max_abs_diff, records_of
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def gpu_state(backend, circ, dtype=rs.C128):
+    st = backend.execute(circ, dtype)
+    out = backend.download(st)
+    st.free()
+    return out
+
+
+def cpu_state(n, recs_or_ops, dtype=np.float64, backend=oracle.CPU_SERIAL):
+    return oracle.execute(backend, n, recs_or_ops, dtype=dtype)
+
+
+def test_device_is_present(backend):
+    info = backend.device_info()
+    print(info)
+    assert info["compute_units"] > 0
+
+
+def test_golden_states(backend, golden):
+    z, meta = golden
+    for case in meta["cases"]:
+        recs = records_of(case)
+        c = rs.Circuit.from_ops(case["n"], oracle.records_to_ops(recs))
+        got = gpu_state(backend, c)
+        assert max_abs_diff(got, z["state_" + case["name"]]) < TOL, case["name"]
+
+
+@pytest.mark.parametrize("chunk", range(4))
+def test_random_circuits_bitwise(backend, chunk):
+    # equivalence.rs:28-70: 200 random circuits, 3..8 qubits, depth 20..59.
+    for seed in range(50 * chunk, 50 * (chunk + 1)):
+        n = 3 + seed % 6
+        depth = 20 + seed % 40
+        recs = oracle.gen_random(n, depth, seed)
+        want_ref = cpu_state(n, recs, backend=oracle.REF)
+        want_cpu = cpu_state(n, recs)
+        got = gpu_state(backend, rs.random_circuit(n, depth, seed))
+        assert max_abs_diff(got, want_ref) < TOL, seed
+        assert max_abs_diff(got, want_cpu) == 0.0, seed
+
+
+def test_larger_registers(backend):
+    # equivalence.rs:72-106 (10q depth 80, 14q depth 60), plus sizes where every kernel
+    # takes its vectorised, multi-block path.
+    for n, depth, seed in [(10, 80, 9000), (14, 60, 4242), (16, 200, 7), (18, 120, 8)]:
+        recs = oracle.gen_random(n, depth, seed)
+        want = cpu_state(n, recs, backend=oracle.CPU_PARALLEL)
+        got = gpu_state(backend, rs.random_circuit(n, depth, seed))
+        assert max_abs_diff(got, want) == 0.0, (n, seed)
+
+
+def test_qft_and_ghz(backend):  # qsv-cuda/tests/equivalence.rs:45-62
+    for n in (1, 2, 3, 5, 8, 12):
+        got = gpu_state(backend, rs.qft(n))
+        assert max_abs_diff(got, cpu_state(n, oracle.gen_qft(n), backend=oracle.REF)) < TOL
+        assert np.abs(np.abs(got) ** 2 - 2.0 ** -n).max() < 1e-12
+    for n in (3, 6, 10):
+        got = gpu_state(backend, rs.ghz(n))
+        assert max_abs_diff(got, cpu_state(n, oracle.gen_ghz_star(n), backend=oracle.REF)) < TOL
+
+
+def test_ghz20_chain_probabilities(backend):
+    # GHZ(20) workload: probabilities nonzero only at 0 and 2^20-1 (0.5 each).
+    st = backend.execute(rs.ghz_chain(20))
+    p = backend.probabilities(st)
+    assert p.dtype == np.float64 and p.shape == (1 << 20,)
+    assert abs(p[0] - 0.5) < 1e-15 and abs(p[-1] - 0.5) < 1e-15
+    assert np.count_nonzero(p) == 2
+    st.free()
+
+
+def test_three_qubit_gates(backend):  # equivalence.rs:195-225
+    ccx = rs.gate_matrix("ccx")
+    for seed, qs in [(1, [0, 1, 2]), (2, [2, 0, 4]), (3, [4, 3, 1]), (4, [5, 2, 0])]:
+        recs = oracle.gen_random(6, 40, seed)
+        ops = oracle.records_to_ops(recs) + [(ccx, qs), (oracle.gate_matrix("h"), (0,)),
+                                              (oracle.gate_matrix("rx", [0.9]), (3,)),
+                                              (ccx, (1, 4, 5))]
+        c = rs.Circuit.from_ops(6, ops)
+        got = gpu_state(backend, c)
+        assert max_abs_diff(got, cpu_state(6, ops, backend=oracle.REF)) < TOL
+        assert max_abs_diff(got, cpu_state(6, ops)) == 0.0
+
+
+def test_mix_and_layered_workloads(backend):
+    for n, g, s in [(9, 200, 1), (12, 300, 2), (15, 200, 3)]:
+        got = gpu_state(backend, rs.mix(n, g, s))
+        assert max_abs_diff(got, cpu_state(n, oracle.gen_mix(n, g, s))) == 0.0
+    for n, d, s in [(8, 4, 0xC0FFEE), (13, 3, 5)]:
+        got = gpu_state(backend, rs.layered(n, d, s))
+        assert max_abs_diff(got, cpu_state(n, oracle.gen_layered(n, d, s))) == 0.0
+
+
+def test_apply_every_target_and_size(backend):
+    # Every 1q target, both vector paths (q below/above the 16-byte vector width) and the
+    # tiny-state scalar path; dense 2q/3q on low and high qubits.
+    rng = np.random.default_rng(1)
+    for n in (1, 2, 3, 5, 9):
+        psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        st = backend.alloc(n)
+        backend.upload(st, psi)
+        want = psi.copy()
+        for q in range(n):
+            g = oracle.gate_matrix("u3", [0.3 + q, 1.1, -0.4])
+            backend.apply(st, g, [q])
+            want = oracle.apply(oracle.CPU_SERIAL, n, want, g, [q])
+        for qs in ([0, 1], [1, 0], [n - 1, 0]) if n >= 2 else ():
+            g = oracle.gate_matrix("u3", [0.2, 0.3, 0.4])
+            g2 = np.kron(g, oracle.gate_matrix("rx", [0.7])) @ oracle.gate_matrix("cx")
+            backend.apply(st, g2, qs)
+            want = oracle.apply(oracle.CPU_SERIAL, n, want, g2, qs)
+        assert max_abs_diff(backend.download(st), want) == 0.0, n
+        st.free()
+
+
+def test_upload_download_roundtrip_and_qft_from_random_start(backend, golden):
+    z, _ = golden
+    v = z["start_random7"]
+    st = backend.alloc(7)
+    backend.upload(st, v)
+    assert max_abs_diff(backend.download(st), v) == 0.0
+    p = rs.Program(st, rs.qft(7))
+    p.launch()
+    assert max_abs_diff(backend.download(st), z["state_qft7_from_random"]) < TOL
+    st.free()
+
+
+def test_fusion_composes_like_build_fused(backend):
+    # fusion.rs:102-139: group matrices composed in f64 on the device, bit-identical.
+    for cap in (2, 3, 4, 5, 6):
+        for seed in range(6):
+            n = 5 + seed % 4
+            recs = oracle.gen_random(n, 60, 70000 + seed)
+            flat = oracle.flatten(recs)
+            want = oracle.fuse(flat, cap)
+            got = rs.fuse(backend, rs.random_circuit(n, 60, 70000 + seed), cap).ops()
+            assert len(got) == len(want)
+            for (ga, qa), (gb, qb) in zip(got, want):
+                assert tuple(qa) == tuple(qb)
+                assert np.array_equal(ga, gb), (cap, seed)
+
+
+def test_fusion_preserves_results(backend):  # equivalence.rs:148-178
+    for cap in (2, 3, 4, 5, 6):
+        for seed in range(10):
+            n = 4 + seed % 5
+            recs = oracle.gen_random(n, 50, 70000 + seed)
+            want = cpu_state(n, recs, backend=oracle.REF)
+            fused = rs.fuse(backend, rs.random_circuit(n, 50, 70000 + seed), cap)
+            assert len(fused) <= 50
+            got = gpu_state(backend, fused)
+            assert max_abs_diff(got, want) < TOL, (cap, seed)
+            # the fused circuit run on the oracle's CpuBackend: identical bits
+            assert max_abs_diff(got, cpu_state(n, fused.ops())) == 0.0
+    for n in (4, 6, 8, 12):
+        fused = rs.fuse(backend, rs.qft(n), 4)
+        got = gpu_state(backend, fused)
+        assert max_abs_diff(got, cpu_state(n, oracle.gen_qft(n), backend=oracle.REF)) < TOL
+
+
+def test_wide_fused_groups_on_large_states(backend):
+    # k = 4..6 groups over the LDS-tiled kernel at sizes with many tiles.
+    for n, cap, seed in [(14, 4, 1), (15, 5, 2), (16, 6, 3)]:
+        fused = rs.fuse(backend, rs.random_circuit(n, 150, seed), cap)
+        got = gpu_state(backend, fused)
+        assert max_abs_diff(got, cpu_state(n, fused.ops(), backend=oracle.CPU_PARALLEL)) == 0.0
+
+
+def test_complex64(backend):
+    for n, g, s in [(6, 50, 1), (11, 120, 2), (15, 200, 3)]:
+        got = gpu_state(backend, rs.random_circuit(n, g, s), rs.C64)
+        want32 = cpu_state(n, oracle.gen_random(n, g, s), dtype=np.float32)
+        want64 = cpu_state(n, oracle.gen_random(n, g, s))
+        assert max_abs_diff(got, want32) == 0.0
+        assert max_abs_diff(got, want64) < 1e-5
+    fused = rs.fuse(backend, rs.random_circuit(12, 100, 9), 5)
+    got = gpu_state(backend, fused, rs.C64)
+    assert max_abs_diff(got, cpu_state(12, fused.ops(), dtype=np.float32)) == 0.0
+
+
+def test_reductions(backend):
+    st = backend.execute(rs.ghz(5))
+    # sampling.rs:48-58
+    assert abs(backend.expectation_z(st, 0b00001)) < 1e-12
+    assert abs(backend.expectation_z(st, 0b00011) - 1) < 1e-12
+    assert abs(backend.expectation_z(st, 0b01111) - 1) < 1e-12
+    assert abs(backend.expectation_z(st, 0b11111)) < 1e-12
+    st.free()
+    # probabilities sum to one (qsv-cuda/tests/equivalence.rs:64-71)
+    st = backend.execute(rs.qft(10))
+    assert abs(backend.probabilities(st).sum() - 1) < 1e-9
+    st.free()
+    for n, dtype in [(16, rs.C128), (18, rs.C64), (3, rs.C128), (1, rs.C64)]:
+        circ = rs.random_circuit(max(n, 2), 80, n) if n >= 2 else rs.Circuit(1).h(0)
+        st = backend.execute(circ, dtype)
+        psi = backend.download(st)
+        p = oracle.probabilities(psi.astype(np.complex128))
+        for mask in (0, 1, 0b101, (1 << n) - 1):
+            want = oracle.expectation_z(p, mask)
+            got = backend.expectation_z(st, mask) if mask else backend.norm_sqr(st)
+            assert abs(got - want) < 1e-12, (n, mask)
+        probs = backend.probabilities(st)
+        want_p = psi.real ** 2 + psi.imag ** 2
+        assert np.array_equal(probs, want_p.astype(probs.dtype)) or \
+            np.abs(probs - want_p).max() < 1e-7
+        for i in (0, (1 << n) - 1, (1 << n) // 3):
+            assert backend.amplitude(st, i) == complex(psi[i])
+        st.free()
+
+
+def test_errors_are_status_codes(backend):
+    st = backend.alloc(3)
+    backend.init_basis(st, 0)
+    with pytest.raises(rs.QsvhError) as e:
+        backend.apply(st, rs.gate_matrix("cx"), [0, 3])
+    assert e.value.status == "INVALID_QUBIT"
+    with pytest.raises(rs.QsvhError) as e:
+        backend.apply(st, rs.gate_matrix("cx"), [2, 2])
+    assert e.value.status == "REPEATED_QUBIT"
+    with pytest.raises(rs.QsvhError) as e:
+        backend.apply(st, np.eye(2), [0, 1])
+    assert e.value.status == "ARITY"
+    with pytest.raises(rs.QsvhError) as e:
+        backend.amplitude(st, 8)
+    assert e.value.status == "INVALID_ARG"
+    with pytest.raises(rs.QsvhError) as e:
+        backend.init_basis(st, 9)
+    assert e.value.status == "INVALID_ARG"
+    with pytest.raises(rs.QsvhError) as e:
+        rs.Program(st, rs.qft(4))
+    assert e.value.status == "INVALID_ARG"
+    with pytest.raises(rs.QsvhError) as e:
+        backend.alloc(3, dtype=7)
+    assert e.value.status == "DTYPE"
+    # the state is untouched by rejected calls
+    assert backend.amplitude(st, 0) == 1
+    st.free()
+
+
+def test_graph_replays_match_eager(backend):
+    # Replay a captured program several times with eager work in between; every replay
+    # must equal the eager application of the same gates.
+    n = 12
+    circ = rs.random_circuit(n, 100, 77)
+    fused = rs.fuse(backend, circ, 5)
+    a = backend.alloc(n)
+    b = backend.alloc(n)
+    backend.init_basis(a, 0)
+    backend.init_basis(b, 0)
+    pg = rs.Program(a, fused, use_graph=True)
+    pe = rs.Program(b, fused)
+    for rep in range(4):
+        pg.launch()
+        pe.launch()
+        backend.synchronize(a)
+        assert backend.norm_sqr(b) > 0.99  # eager work on the same stream in between
+        assert max_abs_diff(backend.download(a), backend.download(b)) == 0.0, rep
+    del pg, pe
+    a.free()
+    b.free()
