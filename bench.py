@@ -4,13 +4,18 @@ Workload (one "step"): QFT(30) -- H + controlled-phase ladder + final swaps, 480
 applied in complex128 to a random-normal 30-qubit state already resident in HBM
 (re, im ~ N(0,1) per amplitude, normalised; generated on the device).  The plain engine
 applies one gate per HBM pass, so a step moves 480 x 2^30 x 32 B = 16.49 TB of
-algorithmic traffic; ``value`` is that traffic divided by the measured step time.
+algorithmic traffic; ``value`` is that traffic divided by the measured step time (the
+reference's own convention: GB/s = amplitude-updates/s x 32 B).
 
-Also reported on the same line: the dominant kernel's roofline (HIP-event timing of the
-controlled-phase diagonal kernel on the engine's stream), a copy kernel's bandwidth on
-the same state, GHZ(20) latency, the random layered 28-qubit circuit, and the CPU
-baseline (the oracle's CpuBackend restatement on the host cores, on a bounded sample of
-the same workload).
+The same line reports, for the same start state:
+* ``fused``: fusion::fuse with k in {4, 5, 6}, run on a complex64 copy with the wide
+  groups on the matrix cores in FP32 and in split-BF16x3 -- circuit seconds, speed-up over
+  plain, max amplitude error and 1 - fidelity against the plain complex128 result;
+* ``tiled``: the exact complex128 engine with LDS tile passes (many gates per pass);
+* the dominant plain kernel's roofline (HIP-event timing on the engine's stream), a copy
+  kernel's bandwidth on the same state, GHZ(20) latency, the random layered 28-qubit
+  circuit, and the CPU baseline (the oracle's CpuBackend restatement on the host cores, on
+  a bounded sample of the same workload).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
@@ -27,23 +32,29 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
-METRIC = ("circuit sec & HBM GB/s (% roofline) at 30q; fused-MFMA vs plain at err<=1e-6")
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+FP32_MFMA_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+METRIC = "circuit sec & HBM GB/s (% roofline) at 30q; fused-MFMA vs plain at err<=1e-6"
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def timed_steps(backend, state, prog, steps, warmup):
-    for _ in range(warmup):
+def timed_steps(backend, state, prog, steps, warmup, reset=None):
+    """Mean wall seconds per launch; `reset` (untimed) restores the start state."""
+    total = 0.0
+    for i in range(warmup + steps):
+        if reset:
+            reset()
+        backend.synchronize(state)
+        t0 = time.perf_counter()
         prog.launch()
-    backend.synchronize(state)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        prog.launch()
-    backend.synchronize(state)
-    return (time.perf_counter() - t0) / steps
+        backend.synchronize(state)
+        if i >= warmup:
+            total += time.perf_counter() - t0
+    return total / steps
 
 
 def cpu_baseline(n_sample_gates: int, n: int = 30):
@@ -51,15 +62,13 @@ def cpu_baseline(n_sample_gates: int, n: int = 30):
     import oracle
 
     recs = oracle.gen_qft(n)[:n_sample_gates]
-    flat = oracle.flatten(recs)
     re, im = oracle.basis(n, 0)
-    oracle.run(oracle.CPU_PARALLEL, n, oracle.flatten(recs[:1]), re, im)  # warm-up touch
+    oracle.run(oracle.CPU_PARALLEL, n, oracle.flatten(recs[:1]), re, im)  # touch pages
     t0 = time.perf_counter()
-    oracle.run(oracle.CPU_PARALLEL, n, flat, re, im)
+    oracle.run(oracle.CPU_PARALLEL, n, oracle.flatten(recs), re, im)
     dt = time.perf_counter() - t0
     del re, im
-    gbs = n_sample_gates * (1 << n) * 32 / dt / 1e9
-    return gbs, dt, oracle.num_threads()
+    return n_sample_gates * (1 << n) * 32 / dt / 1e9, dt, oracle.num_threads()
 
 
 def main():
@@ -69,7 +78,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller sizes (debugging)")
-    ap.add_argument("--cpu-gates", type=int, default=40)
+    ap.add_argument("--cpu-gates", type=int, default=30)
+    ap.add_argument("--tile", type=int, default=12)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -80,68 +90,132 @@ def main():
 
     n = 24 if args.quick else 30
     n_layered = 22 if args.quick else 28
+    seed = 0xC0FFEE
     b = rs.HipBackend(0)
-    info = b.device_info()
-    log(f"device {info}")
+    log(f"device {b.device_info()}; tuning {b.tune()}")
 
-    # ---------------------------------------------------------------- QFT(30) plain
+    # ------------------------------------------------------------ QFT(30) plain c128
     qft = rs.qft(n)
     st = b.alloc(n, rs.C128)
-    b.init_random_normal(st, 0xC0FFEE)
-    log(f"QFT({n}) c128 state ready; norm^2 = {b.norm_sqr(st):.15f}")
+    reset = lambda: b.init_random_normal(st, seed)  # noqa: E731
     prog = rs.Program(st, qft)
     passes = prog.num_kernels()
-    step_s = timed_steps(b, st, prog, args.steps, args.warmup)
+    step_s = timed_steps(b, st, prog, args.steps, args.warmup, reset)
     bytes_step = passes * (1 << n) * 32
     value = bytes_step / step_s / 1e9
     log(f"QFT({n}) plain c128: {passes} passes, {step_s:.4f} s/step, {value:.1f} GB/s")
+    del prog
+    reset()
+    rs.Program(st, qft).launch()  # the complex128 reference result stays resident
 
     # dominant kernel: the controlled-phase diagonal pass (435 of the 480 launches)
-    cp = rs.gate_matrix("cp", np.pi / 8)
-    reps = 10
-    ms_cp = b.time_gate(st, cp, [n - 1, n // 2], reps)
-    h = rs.gate_matrix("h")
-    ms_h = {q: b.time_gate(st, h, [q], reps) for q in (0, 1, n // 2, n - 1)}
-    ms_copy = b.time_copy(st, reps)
     per_launch = (1 << n) * 32
+    ms_cp = b.time_gate(st, rs.gate_matrix("cp", np.pi / 8), [n - 1, n // 2], 10)
+    ms_h = {q: b.time_gate(st, rs.gate_matrix("h"), [q], 10) for q in (0, 1, n // 2, n - 1)}
+    ms_copy = b.time_copy(st, 10)
     achieved = per_launch / (ms_cp * 1e-3) / 1e9
     copy_gbs = per_launch / (ms_copy * 1e-3) / 1e9
-    log(f"diag CP kernel {ms_cp:.3f} ms -> {achieved:.1f} GB/s; copy {copy_gbs:.1f} GB/s; "
-        + ", ".join(f"H(q={q}) {per_launch / (t * 1e-3) / 1e9:.0f} GB/s" for q, t in ms_h.items()))
-    del prog
+    log(f"diag CP kernel {ms_cp:.3f} ms -> {achieved:.1f} GB/s; copy {copy_gbs:.1f} GB/s")
+    # undo the timing gates' effect on the reference: recompute it
+    reset()
+    rs.Program(st, qft).launch()
+
+    # ------------------------------------------------------------ tiled exact c128
+    tiled = {}
+    st2 = b.alloc(n, rs.C128)
+    reset2 = lambda: b.init_random_normal(st2, seed)  # noqa: E731
+    p2 = rs.Program(st2, qft, tile_qubits=args.tile)
+    t_s = timed_steps(b, st2, p2, max(1, args.steps), 1, reset2)
+    reset2()
+    p2.launch()
+    err, fid = b.compare(st, st2)
+    tiled = {"tile_qubits": args.tile, "passes": p2.num_kernels(), "circuit_s": round(t_s, 5),
+             "speedup_vs_plain": round(step_s / t_s, 2), "max_abs_err": err,
+             "one_minus_fidelity": 1 - fid}
+    log(f"QFT({n}) tiled c128 t={args.tile}: {p2.num_kernels()} passes, {t_s:.4f} s "
+        f"({step_s / t_s:.1f}x), err {err:.2e}")
+    del p2
+    st2.free()
+
+    # ------------------------------------------------------------ fused c64 on MFMA
+    fused_out = []
+    mfma_roof = None
+    st64 = b.alloc(n, rs.C64)
+    reset64 = lambda: b.init_random_normal(st64, seed)  # noqa: E731
+    for k in (4, 5, 6):
+        fq = rs.fuse(b, qft, k)
+        for prec, name in ((rs.PREC_MFMA_F32, "mfma_f32"), (rs.PREC_MFMA_BF16X3, "mfma_bf16x3")):
+            p = rs.Program(st64, fq, precision=prec)
+            f_s = timed_steps(b, st64, p, max(1, args.steps), 1, reset64)
+            reset64()
+            p.launch()
+            err, fid = b.compare(st, st64)
+            rec = {"k": k, "precision": name, "passes": p.num_kernels(),
+                   "circuit_s": round(f_s, 5), "speedup_vs_plain": round(step_s / f_s, 2),
+                   "max_abs_err": err, "one_minus_fidelity": 1 - fid}
+            fused_out.append(rec)
+            log(f"QFT({n}) fused k={k} {name}: {rec}")
+            del p
+    # matrix-core roofline: one dense 6-qubit group, FP32 MFMA, on the 30q c64 state
+    U = np.linalg.qr(np.random.default_rng(1).normal(size=(64, 64))
+                     + 1j * np.random.default_rng(2).normal(size=(64, 64)))[0]
+    g6 = rs.Circuit(n).push(U, [3, 9, 14, 20, 25, 29])
+    for prec, peak, name in ((rs.PREC_MFMA_F32, FP32_MFMA_PEAK_TFLOPS, "f32"),
+                             (rs.PREC_MFMA_BF16X3, BF16_MFMA_PEAK_TFLOPS, "bf16x3")):
+        p = rs.Program(st64, g6, precision=prec)
+        p.launch()
+        ms = min(p.time() for _ in range(5))
+        flops = 8 * 64 * (1 << n) * (3 if name == "bf16x3" else 1)
+        gbs = (1 << n) * 16 / (ms * 1e-3) / 1e9
+        tf = flops / (ms * 1e-3) / 1e12
+        log(f"k=6 group {name}: {ms:.3f} ms, {tf:.1f} TFLOP/s of {peak}, {gbs:.0f} GB/s")
+        if name == "f32":
+            mfma_roof = {"kernel": "k_fused_f32<6> (one 6-qubit group, 30q c64)",
+                         "ms_per_launch": round(ms, 4), "achieved_tflops": round(tf, 2),
+                         "peak_tflops": peak, "frac_mfma": round(tf / peak, 4),
+                         "hbm_gbs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+        else:
+            mfma_roof["bf16x3"] = {"ms_per_launch": round(ms, 4), "achieved_tflops": round(tf, 2),
+                                   "peak_tflops": peak, "hbm_gbs": round(gbs, 1),
+                                   "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+        del p
+    st64.free()
     st.free()
 
-    # ---------------------------------------------------------------- layered 28q plain
-    lay = rs.layered(n_layered, 20, 0xC0FFEE)
+    # ------------------------------------------------------------ layered 28q
+    lay = rs.layered(n_layered, 20, seed)
     st = b.alloc(n_layered, rs.C128)
-    b.init_basis(st, 0)
+    resetl = lambda: b.init_basis(st, 0)  # noqa: E731
     prog = rs.Program(st, lay)
-    lay_s = timed_steps(b, st, prog, 1, 1)
-    lay_bytes = prog.num_kernels() * (1 << n_layered) * 32
-    log(f"layered({n_layered}, depth 20) plain c128: {lay_s:.4f} s "
-        f"({lay_bytes / lay_s / 1e9:.1f} GB/s)")
+    lay_s = timed_steps(b, st, prog, 1, 1, resetl)
+    lay_passes = prog.num_kernels()
     del prog
+    pt = rs.Program(st, lay, tile_qubits=args.tile)
+    lay_t = timed_steps(b, st, pt, 1, 1, resetl)
+    lay_tp = pt.num_kernels()
+    del pt
+    log(f"layered({n_layered}, 20) c128: plain {lay_s:.4f} s ({lay_passes} passes), "
+        f"tiled {lay_t:.4f} s ({lay_tp} passes)")
     st.free()
 
-    # ---------------------------------------------------------------- GHZ(20)
+    # ------------------------------------------------------------ GHZ(20)
     st = b.alloc(20, rs.C128)
-    ghz = rs.ghz_chain(20)
     b.init_basis(st, 0)
-    prog = rs.Program(st, ghz, use_graph=True)
+    prog = rs.Program(st, rs.ghz_chain(20), use_graph=True)
     prog.launch()
     b.synchronize(st)
     ghz_ms = min(prog.time() for _ in range(20))
-    log(f"GHZ(20) graph replay: {ghz_ms * 1e3:.1f} us")
     del prog
     st.free()
+    log(f"GHZ(20) graph replay: {ghz_ms * 1e3:.1f} us")
 
-    # ---------------------------------------------------------------- CPU baseline
+    # ------------------------------------------------------------ CPU baseline
     cpu = None
     if not args.no_cpu:
         gbs, dt, cores = cpu_baseline(args.cpu_gates, n)
         cpu = {"value": round(gbs, 3), "unit": "GB/s", "cores": cores, "kind": "port",
-               "sample": f"first {args.cpu_gates} of {passes} QFT({n}) gates, c128, "
-                         f"CpuBackend::parallel restatement ({dt:.1f} s)",
+               "sample": f"first {args.cpu_gates} of {passes} QFT({n}) gates, complex128, "
+                         f"CpuBackend::parallel restatement, {dt:.1f} s",
                "circuit_s_extrapolated": round(dt * passes / args.cpu_gates, 2)}
         log(f"CPU baseline: {gbs:.2f} GB/s on {cores} threads ({dt:.1f} s)")
 
@@ -163,11 +237,15 @@ def main():
                    "circuit_s": round(step_s, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_diag (controlled phase, 30q c128)",
+                     "traffic": None, "kernel": f"k_diag (controlled phase, {n}q c128)",
                      "bytes_per_launch": per_launch, "ms_per_launch": round(ms_cp, 4)},
         "copy_kernel_gbs": round(copy_gbs, 1),
         "h_gate_gbs": {str(q): round(per_launch / (t * 1e-3) / 1e9, 1) for q, t in ms_h.items()},
-        "layered28_plain_s": round(lay_s, 4),
+        "tiled": tiled,
+        "fused": fused_out,
+        "mfma_roofline": mfma_roof,
+        "layered28": {"plain_s": round(lay_s, 4), "plain_passes": lay_passes,
+                      "tiled_s": round(lay_t, 4), "tiled_passes": lay_tp},
         "ghz20_us": round(ghz_ms * 1e3, 2),
         "cpu_baseline": cpu,
     }

@@ -79,6 +79,15 @@ const char* qsvh_last_error(const qsvh_ctx* ctx);
 int qsvh_device_info(qsvh_ctx* ctx, char* name, size_t name_cap, int* compute_units,
                      uint64_t* hbm_bytes);
 int qsvh_abi_version(void);
+/* Launch-shape knobs of the streaming (1q, diagonal, copy) kernels, per context:
+ * "max_blocks" (0: one thread per work item; n > 0: grid-stride loop over n blocks),
+ * "unroll" (1, 2 or 4 independent 16-byte items per thread per iteration) and
+ * "nontemporal" (0/1: streaming load/store hints); "mfma_blocks" (workgroups of the
+ * persistent matrix-core group kernels).  No reference counterpart: the
+ * reference's kernels have a fixed 256-thread, one-item-per-thread shape
+ * (crates/qsv-cuda/src/lib.rs:32,80-87). */
+int qsvh_tune_set(qsvh_ctx* ctx, const char* key, int64_t value);
+int qsvh_tune_get(qsvh_ctx* ctx, const char* key, int64_t* value);
 
 /* ------------------------------------------------------------------ gates */
 /* The gate library (crates/qsv-core/src/gate.rs:96-180): name is one of h x y z s t sx
@@ -200,6 +209,9 @@ int qsvh_program_launch(qsvh_program* p);
 int qsvh_program_time(qsvh_program* p, double* ms);
 /* Number of device kernels one launch issues (passes over the state). */
 int qsvh_program_num_kernels(const qsvh_program* p, uint64_t* out);
+/* The circuit's op indices in the order the program applies them (tile passes move
+ * gates past gates on disjoint qubits).  Writes at most cap entries; *n = total. */
+int qsvh_program_order(const qsvh_program* p, uint64_t* order, size_t cap, size_t* n);
 void qsvh_program_destroy(qsvh_program* p);
 
 /* Apply one gate `reps` times back to back and return the mean device time per launch

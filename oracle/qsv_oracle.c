@@ -522,7 +522,8 @@ int or_ref_apply(uint32_t n, double* re, double* im, uint32_t m, const uint32_t*
       uint64_t half = 1ull << qs[0], block = half << 1;                                     \
       int64_t nblocks = (int64_t)(dim / block);                                             \
       if (parallel && (dim >> 1) >= MIN_PAIRS_LIGHT) {                                      \
-        _Pragma("omp parallel for schedule(dynamic, 1)")                                    \
+        /* rayon splits the chunk list into contiguous ranges: a static schedule */        \
+        _Pragma("omp parallel for schedule(static)")                                        \
         for (int64_t b = 0; b < nblocks; b++) {                                             \
           uint64_t o = (uint64_t)b * block;                                                 \
           one_qubit_block_##SUF(re + o, im + o, re + o + half, im + o + half, half, g);     \

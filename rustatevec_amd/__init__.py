@@ -66,6 +66,8 @@ def lib() -> C.CDLL:
         "qsvh_ctx_create": (i32, [i32, PP]),
         "qsvh_ctx_destroy": (None, [P]),
         "qsvh_last_error": (C.c_char_p, [P]),
+        "qsvh_tune_set": (i32, [P, C.c_char_p, C.c_int64]),
+        "qsvh_tune_get": (i32, [P, C.c_char_p, C.POINTER(C.c_int64)]),
         "qsvh_device_info": (i32, [P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int),
                                    C.POINTER(C.c_uint64)]),
         "qsvh_gate_matrix": (i32, [C.c_char_p, P, u32, P, P, C.POINTER(C.c_uint32)]),
@@ -106,6 +108,7 @@ def lib() -> C.CDLL:
         "qsvh_program_launch": (i32, [P]),
         "qsvh_program_time": (i32, [P, C.POINTER(C.c_double)]),
         "qsvh_program_num_kernels": (i32, [P, C.POINTER(C.c_uint64)]),
+        "qsvh_program_order": (i32, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "qsvh_program_destroy": (None, [P]),
         "qsvh_time_gate": (i32, [P, u32, P, P, P, u32, C.POINTER(C.c_double)]),
         "qsvh_time_copy": (i32, [P, u32, C.POINTER(C.c_double)]),
@@ -353,6 +356,17 @@ class HipBackend:
 
     def last_error(self) -> str:
         return (lib().qsvh_last_error(self.ctx) or b"").decode()
+
+    def tune(self, **knobs):
+        """Set launch-shape knobs (max_blocks, unroll, nontemporal); returns all values."""
+        for k, v in knobs.items():
+            _check(lib().qsvh_tune_set(self.ctx, k.encode(), int(v)), self.ctx)
+        out = {}
+        for k in ("max_blocks", "unroll", "nontemporal", "mfma_blocks"):
+            v = C.c_int64(0)
+            _check(lib().qsvh_tune_get(self.ctx, k.encode(), C.byref(v)), self.ctx)
+            out[k] = v.value
+        return out
 
     def device_info(self):
         name = C.create_string_buffer(256)

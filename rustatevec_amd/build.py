@@ -30,6 +30,8 @@ SOURCES = {
     "kernels_plain.hip": ["-ffp-contract=off"],
     "kernels_compose.hip": ["-ffp-contract=off"],
     "kernels_util.hip": [],
+    "kernels_mfma.hip": [],
+    "kernels_tile.hip": ["-ffp-contract=off"],
 }
 
 

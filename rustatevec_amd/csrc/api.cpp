@@ -88,9 +88,9 @@ Launch classify(const Op& op, int dtype) {
   return l;
 }
 
-hipError_t issue(const State& st, const Launch& l, const void* arena, const Tuning& t,
-                 hipStream_t s) {
-  const char* base = static_cast<const char*>(arena);
+hipError_t issue(const Program& p, const Launch& l, const Tuning& t, hipStream_t s) {
+  const State& st = *p.st;
+  const char* base = static_cast<const char*>(p.arena);
   const void* dmat = base ? base + l.arena_off * st.elem() : nullptr;
   switch (l.kind) {
     case Launch::k1q:
@@ -99,6 +99,12 @@ hipError_t issue(const State& st, const Launch& l, const void* arena, const Tuni
       return launch_diag(st, l.m, l.q, l.mat.data(), t, s);
     case Launch::kMq:
       return launch_mq(st, l.m, l.q, l.mat.data(), dmat, t, s);
+    case Launch::kFusedMfma:
+      return launch_fused_mfma(st, l.m, l.q, static_cast<const char*>(p.mfma) + l.mfma_off,
+                               p.precision, t.mfma_blocks, s);
+    case Launch::kTile:
+      return launch_tile(st, p.tile_passes[l.arena_off], p.tile_ops, p.tile_diags, p.tile_pool,
+                         t.tile_blocks, s);
     default:
       return hipErrorInvalidValue;
   }
@@ -132,7 +138,7 @@ int build_arena(Program* p) {
 hipError_t issue_all(Program* p) {
   State& st = *p->st;
   for (const Launch& l : p->launches) {
-    const hipError_t e = issue(st, l, p->arena, st.ctx->tune, st.ctx->stream);
+    const hipError_t e = issue(*p, l, st.ctx->tune, st.ctx->stream);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -237,6 +243,36 @@ void qsvh_ctx_destroy(qsvh_ctx* h) {
 }
 
 const char* qsvh_last_error(const qsvh_ctx* h) { return h ? h->c.last_error.c_str() : ""; }
+
+int qsvh_tune_set(qsvh_ctx* h, const char* key, int64_t value) {
+  if (!h || !key) return kInvalidArg;
+  Tuning& t = h->c.tune;
+  const std::string k(key);
+  if (k == "max_blocks" && value >= 0 && value <= (1ll << 30)) {
+    t.stream_max_blocks = int(value);
+  } else if (k == "unroll" && (value == 1 || value == 2 || value == 4)) {
+    t.unroll = int(value);
+  } else if (k == "nontemporal" && (value == 0 || value == 1)) {
+    t.nontemporal = int(value);
+  } else if (k == "mfma_blocks" && value >= 1 && value <= (1ll << 20)) {
+    t.mfma_blocks = int(value);
+  } else {
+    return set_err(&h->c, kInvalidArg, "unknown tuning key or value: " + k);
+  }
+  return kOk;
+}
+
+int qsvh_tune_get(qsvh_ctx* h, const char* key, int64_t* value) {
+  if (!h || !key || !value) return kInvalidArg;
+  const Tuning& t = h->c.tune;
+  const std::string k(key);
+  if (k == "max_blocks") *value = t.stream_max_blocks;
+  else if (k == "unroll") *value = t.unroll;
+  else if (k == "nontemporal") *value = t.nontemporal;
+  else if (k == "mfma_blocks") *value = t.mfma_blocks;
+  else return set_err(&h->c, kInvalidArg, "unknown tuning key: " + k);
+  return kOk;
+}
 
 int qsvh_device_info(qsvh_ctx* h, char* name, size_t cap, int* cus, uint64_t* hbm) {
   if (!h) return kInvalidArg;
@@ -675,6 +711,13 @@ int qsvh_program_num_kernels(const qsvh_program* p, uint64_t* out) {
   return kOk;
 }
 
+int qsvh_program_order(const qsvh_program* p, uint64_t* order, size_t cap, size_t* n) {
+  if (!p || !n) return kInvalidArg;
+  *n = p->p.order.size();
+  for (size_t i = 0; i < p->p.order.size() && i < cap && order; ++i) order[i] = p->p.order[i];
+  return kOk;
+}
+
 void qsvh_program_destroy(qsvh_program* p) {
   if (!p) return;
   program_free(&p->p);
@@ -731,7 +774,7 @@ int qsvh_time_copy(qsvh_state* st, uint32_t reps, double* ms) {
     return hip_err(s.ctx, e, "copy alloc");
   }
   const int rc =
-      time_launches(s.ctx, reps, ms, [&]() { return launch_copy(s, dr, di, s.ctx->stream); });
+      time_launches(s.ctx, reps, ms, [&]() { return launch_copy(s, dr, di, s.ctx->tune, s.ctx->stream); });
   hipStreamSynchronize(s.ctx->stream);
   hipFree(dr);
   hipFree(di);

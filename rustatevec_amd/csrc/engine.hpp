@@ -16,7 +16,10 @@ namespace qsvh {
 struct Tuning {
   int stream_block = 256;      // threads per block for the streaming kernels
   int stream_max_blocks = 0;   // 0: one thread per work item; >0: grid-stride cap
-  int unroll = 2;              // independent vector items per thread per iteration
+  int unroll = 1;              // independent vector items per thread per iteration
+  int nontemporal = 0;         // 1: non-temporal (streaming) loads and stores
+  int mfma_blocks = 2048;      // workgroups of the persistent MFMA group kernels
+  int tile_blocks = 1024;      // workgroups of the persistent tile-pass kernel
 };
 
 struct Ctx {
@@ -53,12 +56,13 @@ struct MqIndex {
 
 // One kernel launch of the plain executor.
 struct Launch {
-  enum Kind { k1q, kDiag, kMq, kFusedMfma } kind;
+  enum Kind { k1q, kDiag, kMq, kFusedMfma, kTile } kind;
   uint32_t m = 0;
   uint32_t q[kMaxGateQubits] = {0, 0, 0, 0, 0, 0};
   // matrix in f64 (interleaved re, im), row-major; for diagonal launches the 2^m phases.
   std::vector<double> mat;
   uint64_t arena_off = 0;  // element offset of the matrix in the device arena
+  uint64_t mfma_off = 0;   // byte offset of the block matrix in the program's MFMA buffer
 };
 
 // ---- launchers (kernels_plain.hip) ---------------------------------------------------
@@ -76,7 +80,42 @@ hipError_t launch_probs(const State& st, void* out, hipStream_t s);
 // f64 sum of |psi_i|^2 * sign(i & mask) (mask 0: the norm) into *d_result.
 hipError_t launch_reduce(const State& st, uint64_t mask, double* d_partials, int max_partials,
                          double* d_result, hipStream_t s);
-hipError_t launch_copy(const State& src, void* dst_re, void* dst_im, hipStream_t s);
+hipError_t launch_copy(const State& src, void* dst_re, void* dst_im, const Tuning& t,
+                       hipStream_t s);
+
+// ---- kernels_mfma.hip ------------------------------------------------------------------
+// Fused k-qubit group (k = 4..6, qubits `q` ascending) on a complex64 state of >= 11
+// qubits; `mats` is the 2D x 2D real block matrix in f32 (precision 1) or bf16 hi, lo
+// (precision 2).
+hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, const void* mats,
+                             int precision, int max_blocks, hipStream_t s);
+
+// ---- kernels_tile.hip ------------------------------------------------------------------
+// One op of a tile pass.  kind 0: 2x2 gate on tile bit lq[0]; 1: run of n_diag diagonal
+// gates starting at diags[off]; 2, 3: dense gate on tile bits lq[0..kind).  The op runs
+// on a tile only when (tile base & cmask) == cval.  Matrices live at pool[off] (complex,
+// interleaved, state precision, row-major).
+struct TileOp {
+  uint32_t kind;
+  uint32_t m;
+  uint32_t lq[3];
+  uint32_t n_diag;
+  uint64_t cmask, cval;
+  uint64_t off;
+};
+// A diagonal gate inside a diag run: global qubits q[0..m), phases at pool[tab ..].
+struct TileDiag {
+  uint32_t m;
+  uint32_t q[kMaxGateQubits];
+  uint32_t tab;
+};
+struct TilePassArg {
+  uint32_t tb;      // tile bits
+  uint32_t ts[16];  // ascending state bits of the tile
+  uint32_t op_begin, n_ops;
+};
+hipError_t launch_tile(const State& st, const TilePassArg& pa, const TileOp* ops,
+                       const TileDiag* diags, const void* pool, int max_blocks, hipStream_t s);
 
 // ---- kernels_util.hip ------------------------------------------------------------------
 // Random normalised start state (counter-based SplitMix64 + Box-Muller); `d_norm` receives

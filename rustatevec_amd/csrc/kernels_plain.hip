@@ -10,6 +10,8 @@
 // CpuBackend (crates/qsv-core/src/backend/cpu.rs:71-114,221-247).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "engine.hpp"
 
 namespace qsvh {
@@ -22,26 +24,37 @@ struct Vec {
   using type = T __attribute__((ext_vector_type(W)));
 };
 
-template <typename T, int W>
+// 16-byte (or scalar) loads and stores; NT marks them non-temporal (streamed past the
+// caches), one of the launch-shape knobs.
+template <typename T, int W, bool NT = false>
 __device__ __forceinline__ void vload(const T* p, T (&x)[W]) {
   if constexpr (W == 1) {
-    x[0] = p[0];
+    x[0] = NT ? __builtin_nontemporal_load(p) : p[0];
   } else {
-    const typename Vec<T, W>::type v = *reinterpret_cast<const typename Vec<T, W>::type*>(p);
+    using V = typename Vec<T, W>::type;
+    const V* q = reinterpret_cast<const V*>(p);
+    const V v = NT ? __builtin_nontemporal_load(q) : *q;
 #pragma unroll
     for (int j = 0; j < W; ++j) x[j] = v[j];
   }
 }
 
-template <typename T, int W>
+template <typename T, int W, bool NT = false>
 __device__ __forceinline__ void vstore(T* p, const T (&x)[W]) {
   if constexpr (W == 1) {
-    p[0] = x[0];
+    if constexpr (NT)
+      __builtin_nontemporal_store(x[0], p);
+    else
+      p[0] = x[0];
   } else {
-    typename Vec<T, W>::type v;
+    using V = typename Vec<T, W>::type;
+    V v;
 #pragma unroll
     for (int j = 0; j < W; ++j) v[j] = x[j];
-    *reinterpret_cast<typename Vec<T, W>::type*>(p) = v;
+    if constexpr (NT)
+      __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
+    else
+      *reinterpret_cast<V*>(p) = v;
   }
 }
 
@@ -69,7 +82,7 @@ __device__ __forceinline__ void pair_update(T& x0r, T& x0i, T& x1r, T& x1i, cons
 // nested-block walk of cpu.rs:148-175 with the 2^(n-1) pairs enumerated as
 // lo = insert_zero_bit(x, q), hi = lo + 2^q (layout.rs:22-26).  Consecutive lanes read
 // consecutive 16-byte vectors of both halves: every access is coalesced for any q.
-template <typename T, int W, int U>
+template <typename T, int W, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_1q_wide(T* __restrict__ re, T* __restrict__ im,
                                                     uint64_t n_items, uint32_t q, G2<T> g) {
   const uint64_t lowmask = (1ull << q) - 1;
@@ -85,10 +98,10 @@ __global__ __launch_bounds__(kBlock) void k_1q_wide(T* __restrict__ re, T* __res
       const uint64_t x = v * W;
       lo[u] = ((x & ~lowmask) << 1) | (x & lowmask);
       if (v < n_items) {
-        vload<T, W>(re + lo[u], ar[u]);
-        vload<T, W>(im + lo[u], ai[u]);
-        vload<T, W>(re + lo[u] + half, br[u]);
-        vload<T, W>(im + lo[u] + half, bi[u]);
+        vload<T, W, NT>(re + lo[u], ar[u]);
+        vload<T, W, NT>(im + lo[u], ai[u]);
+        vload<T, W, NT>(re + lo[u] + half, br[u]);
+        vload<T, W, NT>(im + lo[u] + half, bi[u]);
       }
     }
 #pragma unroll
@@ -97,10 +110,10 @@ __global__ __launch_bounds__(kBlock) void k_1q_wide(T* __restrict__ re, T* __res
       if (v < n_items) {
 #pragma unroll
         for (int j = 0; j < W; ++j) pair_update(ar[u][j], ai[u][j], br[u][j], bi[u][j], g);
-        vstore<T, W>(re + lo[u], ar[u]);
-        vstore<T, W>(im + lo[u], ai[u]);
-        vstore<T, W>(re + lo[u] + half, br[u]);
-        vstore<T, W>(im + lo[u] + half, bi[u]);
+        vstore<T, W, NT>(re + lo[u], ar[u]);
+        vstore<T, W, NT>(im + lo[u], ai[u]);
+        vstore<T, W, NT>(re + lo[u] + half, br[u]);
+        vstore<T, W, NT>(im + lo[u] + half, bi[u]);
       }
     }
   }
@@ -109,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_1q_wide(T* __restrict__ re, T* __res
 // Target q < log2(W): both partners of every pair sit inside one 16-byte vector, so each
 // work item is one vector of W amplitudes holding W/2 complete pairs.  Q is the target
 // qubit (a compile-time constant so the pairing stays in registers).
-template <typename T, int W, int Q, int U>
+template <typename T, int W, int Q, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_1q_intra(T* __restrict__ re, T* __restrict__ im,
                                                      uint64_t n_items, G2<T> g) {
   constexpr int h = 1 << Q;
@@ -121,8 +134,8 @@ __global__ __launch_bounds__(kBlock) void k_1q_intra(T* __restrict__ re, T* __re
     for (int u = 0; u < U; ++u) {
       const uint64_t v = v0 + uint64_t(u) * nthr;
       if (v < n_items) {
-        vload<T, W>(re + v * W, xr[u]);
-        vload<T, W>(im + v * W, xi[u]);
+        vload<T, W, NT>(re + v * W, xr[u]);
+        vload<T, W, NT>(im + v * W, xi[u]);
       }
     }
 #pragma unroll
@@ -132,8 +145,8 @@ __global__ __launch_bounds__(kBlock) void k_1q_intra(T* __restrict__ re, T* __re
 #pragma unroll
         for (int j = 0; j < W; ++j)
           if (((j >> Q) & 1) == 0) pair_update(xr[u][j], xi[u][j], xr[u][j + h], xi[u][j + h], g);
-        vstore<T, W>(re + v * W, xr[u]);
-        vstore<T, W>(im + v * W, xi[u]);
+        vstore<T, W, NT>(re + v * W, xr[u]);
+        vstore<T, W, NT>(im + v * W, xi[u]);
       }
     }
   }
@@ -149,7 +162,7 @@ struct DiagArg {
   T dr[64], di[64];
 };
 
-template <typename T, int W, int U>
+template <typename T, int W, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ re, T* __restrict__ im,
                                                  uint64_t n_items, DiagArg<T> a) {
   __shared__ T sdr[64], sdi[64];
@@ -167,8 +180,8 @@ __global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ re, T* __restri
     for (int u = 0; u < U; ++u) {
       const uint64_t v = v0 + uint64_t(u) * nthr;
       if (v < n_items) {
-        vload<T, W>(re + v * W, xr[u]);
-        vload<T, W>(im + v * W, xi[u]);
+        vload<T, W, NT>(re + v * W, xr[u]);
+        vload<T, W, NT>(im + v * W, xi[u]);
       }
     }
 #pragma unroll
@@ -185,8 +198,8 @@ __global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ re, T* __restri
           xr[u][j] = dr * x_r - di * x_i;
           xi[u][j] = dr * x_i + di * x_r;
         }
-        vstore<T, W>(re + v * W, xr[u]);
-        vstore<T, W>(im + v * W, xi[u]);
+        vstore<T, W, NT>(re + v * W, xr[u]);
+        vstore<T, W, NT>(im + v * W, xi[u]);
       }
     }
   }
@@ -388,17 +401,30 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double* __restric
   }
 }
 
-template <typename T, int W>
+template <typename T, int W, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_copy(const T* __restrict__ sr,
                                                  const T* __restrict__ si, T* __restrict__ dr,
                                                  T* __restrict__ di, uint64_t n_items) {
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
-    T a[W], b[W];
-    vload<T, W>(sr + v * W, a);
-    vload<T, W>(si + v * W, b);
-    vstore<T, W>(dr + v * W, a);
-    vstore<T, W>(di + v * W, b);
+  for (uint64_t v0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v0 < n_items;
+       v0 += nthr * U) {
+    T a[U][W], b[U][W];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t v = v0 + uint64_t(u) * nthr;
+      if (v < n_items) {
+        vload<T, W, NT>(sr + v * W, a[u]);
+        vload<T, W, NT>(si + v * W, b[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t v = v0 + uint64_t(u) * nthr;
+      if (v < n_items) {
+        vstore<T, W, NT>(dr + v * W, a[u]);
+        vstore<T, W, NT>(di + v * W, b[u]);
+      }
+    }
   }
 }
 
@@ -410,6 +436,24 @@ unsigned grid_for(uint64_t items, int unroll, const Tuning& t) {
   if (g < 1) g = 1;
   if (g > 0x7fffffffull) g = 0x7fffffffull;
   return unsigned(g);
+}
+
+// Call f(unroll, nontemporal) with the tuning's values as compile-time constants.
+template <typename F>
+hipError_t with_shape(const Tuning& t, F&& f) {
+  using std::integral_constant;
+  const bool nt = t.nontemporal != 0;
+  switch (t.unroll) {
+    case 1:
+      return nt ? f(integral_constant<int, 1>{}, std::true_type{})
+                : f(integral_constant<int, 1>{}, std::false_type{});
+    case 4:
+      return nt ? f(integral_constant<int, 4>{}, std::true_type{})
+                : f(integral_constant<int, 4>{}, std::false_type{});
+    default:
+      return nt ? f(integral_constant<int, 2>{}, std::true_type{})
+                : f(integral_constant<int, 2>{}, std::false_type{});
+  }
 }
 
 template <typename T>
@@ -425,30 +469,25 @@ hipError_t launch_1q_t(const State& st, uint32_t q, const G2<T>& g, const Tuning
   const uint64_t dim = st.dim();
   T* re = static_cast<T*>(st.re);
   T* im = static_cast<T*>(st.im);
-  if (int(q) >= ilog2(W)) {
-    const uint64_t items = (dim >> 1) / W;
-    switch (t.unroll) {
-      case 1:
-        k_1q_wide<T, W, 1><<<grid_for(items, 1, t), kBlock, 0, s>>>(re, im, items, q, g);
-        break;
-      case 4:
-        k_1q_wide<T, W, 4><<<grid_for(items, 4, t), kBlock, 0, s>>>(re, im, items, q, g);
-        break;
-      default:
-        k_1q_wide<T, W, 2><<<grid_for(items, 2, t), kBlock, 0, s>>>(re, im, items, q, g);
+  return with_shape(t, [&](auto uc, auto ntc) {
+    constexpr int U = decltype(uc)::value;
+    constexpr bool NT = decltype(ntc)::value;
+    if (int(q) >= ilog2(W)) {
+      const uint64_t items = (dim >> 1) / W;
+      k_1q_wide<T, W, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, q, g);
+    } else {
+      const uint64_t items = dim / W;
+      if constexpr (W >= 2) {
+        if (q == 0)
+          k_1q_intra<T, W, 0, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, g);
+      }
+      if constexpr (W >= 4) {
+        if (q == 1)
+          k_1q_intra<T, W, 1, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, g);
+      }
     }
-  } else {
-    const uint64_t items = dim / W;
-    if constexpr (W >= 2) {
-      if (q == 0)
-        k_1q_intra<T, W, 0, 2><<<grid_for(items, 2, t), kBlock, 0, s>>>(re, im, items, g);
-    }
-    if constexpr (W >= 4) {
-      if (q == 1)
-        k_1q_intra<T, W, 1, 2><<<grid_for(items, 2, t), kBlock, 0, s>>>(re, im, items, g);
-    }
-  }
-  return hipGetLastError();
+    return hipGetLastError();
+  });
 }
 
 template <typename T>
@@ -477,14 +516,18 @@ hipError_t launch_diag_any(const State& st, uint32_t m, const uint32_t* q, const
   T* re = static_cast<T*>(st.re);
   T* im = static_cast<T*>(st.im);
   constexpr int W = vec_width<T>();
-  if (st.dim() >= uint64_t(W)) {
-    const uint64_t items = st.dim() / W;
-    k_diag<T, W, 2><<<grid_for(items, 2, t), kBlock, 0, s>>>(re, im, items, a);
-  } else {
-    const uint64_t items = st.dim();
-    k_diag<T, 1, 2><<<grid_for(items, 2, t), kBlock, 0, s>>>(re, im, items, a);
-  }
-  return hipGetLastError();
+  return with_shape(t, [&](auto uc, auto ntc) {
+    constexpr int U = decltype(uc)::value;
+    constexpr bool NT = decltype(ntc)::value;
+    if (st.dim() >= uint64_t(W)) {
+      const uint64_t items = st.dim() / W;
+      k_diag<T, W, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, a);
+    } else {
+      const uint64_t items = st.dim();
+      k_diag<T, 1, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, a);
+    }
+    return hipGetLastError();
+  });
 }
 
 template <int M>
@@ -606,20 +649,23 @@ hipError_t launch_reduce_t(const State& st, uint64_t mask, double* part, int max
 }
 
 template <typename T>
-hipError_t launch_copy_t(const State& st, void* dr, void* di, hipStream_t s) {
+hipError_t launch_copy_t(const State& st, void* dr, void* di, const Tuning& t, hipStream_t s) {
   constexpr int W = vec_width<T>();
-  Tuning t;
-  if (st.dim() >= uint64_t(W)) {
-    const uint64_t items = st.dim() / W;
-    k_copy<T, W><<<grid_for(items, 1, t), kBlock, 0, s>>>(
-        static_cast<const T*>(st.re), static_cast<const T*>(st.im), static_cast<T*>(dr),
-        static_cast<T*>(di), items);
-  } else {
-    k_copy<T, 1><<<1, kBlock, 0, s>>>(static_cast<const T*>(st.re),
-                                      static_cast<const T*>(st.im), static_cast<T*>(dr),
-                                      static_cast<T*>(di), st.dim());
-  }
-  return hipGetLastError();
+  const T* sr = static_cast<const T*>(st.re);
+  const T* si = static_cast<const T*>(st.im);
+  return with_shape(t, [&](auto uc, auto ntc) {
+    constexpr int U = decltype(uc)::value;
+    constexpr bool NT = decltype(ntc)::value;
+    if (st.dim() >= uint64_t(W)) {
+      const uint64_t items = st.dim() / W;
+      k_copy<T, W, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(
+          sr, si, static_cast<T*>(dr), static_cast<T*>(di), items);
+    } else {
+      k_copy<T, 1, U, NT><<<1, kBlock, 0, s>>>(sr, si, static_cast<T*>(dr),
+                                               static_cast<T*>(di), st.dim());
+    }
+    return hipGetLastError();
+  });
 }
 
 }  // namespace
@@ -656,8 +702,9 @@ hipError_t launch_reduce(const State& st, uint64_t mask, double* part, int max_p
                        : launch_reduce_t<float>(st, mask, part, max_part, result, s);
 }
 
-hipError_t launch_copy(const State& st, void* dr, void* di, hipStream_t s) {
-  return st.dtype == 0 ? launch_copy_t<double>(st, dr, di, s) : launch_copy_t<float>(st, dr, di, s);
+hipError_t launch_copy(const State& st, void* dr, void* di, const Tuning& t, hipStream_t s) {
+  return st.dtype == 0 ? launch_copy_t<double>(st, dr, di, t, s)
+                       : launch_copy_t<float>(st, dr, di, t, s);
 }
 
 }  // namespace qsvh

@@ -1,25 +1,343 @@
-// plan.cpp -- launch-list passes of a program.
-#include "program.hpp"
+// plan.cpp -- launch-list passes of a program: routing wide dense gates to the matrix
+// cores (MFMA precisions on complex64 states) and the LDS tile passes.
+#include <algorithm>
+#include <cstring>
+#include <vector>
 
 #include "../../include/qsvh.h"
+#include "program.hpp"
 
 namespace qsvh {
 
+namespace {
+
+constexpr uint32_t kMfmaMinQubits = 4;   // 2D = 32 block rows: one 32x32 MFMA tile
+constexpr uint32_t kMfmaTileBits = 11;   // the MFMA kernels' tile (2048 amplitudes)
+
+// Round-to-nearest-even f32 -> bf16 bits (finite inputs).
+uint16_t bf16_bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+float bf16_value(uint16_t b) {
+  const uint32_t u = uint32_t(b) << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+
+// The 2D x 2D real block matrix [[Ur, -Ui], [Ui, Ur]] of a gate whose internal index is
+// re-expressed over its qubits in ascending order.
+std::vector<float> block_matrix(const Launch& l, uint32_t* sorted) {
+  const uint32_t m = l.m, D = 1u << m;
+  std::copy(l.q, l.q + m, sorted);
+  std::sort(sorted, sorted + m);
+  // bit j of the original index is qubit l.q[j]; it moves to position rank(l.q[j]).
+  uint32_t rank[kMaxGateQubits];
+  for (uint32_t j = 0; j < m; ++j)
+    rank[j] = uint32_t(std::find(sorted, sorted + m, l.q[j]) - sorted);
+  auto remap = [&](uint32_t s) {
+    uint32_t t = 0;
+    for (uint32_t j = 0; j < m; ++j) t |= ((s >> j) & 1u) << rank[j];
+    return t;
+  };
+  std::vector<float> a(size_t(4) * D * D);
+  const uint32_t R = 2 * D;
+  for (uint32_t r = 0; r < D; ++r)
+    for (uint32_t c = 0; c < D; ++c) {
+      const float ur = float(l.mat[2 * (size_t(r) * D + c)]);
+      const float ui = float(l.mat[2 * (size_t(r) * D + c) + 1]);
+      const uint32_t i = remap(r), j = remap(c);
+      a[size_t(i) * R + j] = ur;
+      a[size_t(i) * R + D + j] = -ui;
+      a[size_t(D + i) * R + j] = ui;
+      a[size_t(D + i) * R + D + j] = ur;
+    }
+  return a;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------- tile passes
+namespace {
+
+constexpr uint32_t kTileLowBits = 4;  // bits 0..3 always in a tile: 16-amplitude runs
+
+uint32_t popcount64(uint64_t x) { return uint32_t(__builtin_popcountll(x)); }
+
+// Qubits a gate actually acts on: those w.r.t. which its matrix is not block-diagonal
+// (a control qubit c has U[r][c'] == 0 whenever bit c of r and c' differ).  A diagonal
+// gate acts on none; CX(c, t) acts on t only.
+uint64_t action_mask(const Launch& l) {
+  if (l.kind == Launch::kDiag) return 0;
+  const uint32_t D = 1u << l.m;
+  uint64_t mask = 0;
+  for (uint32_t j = 0; j < l.m; ++j) {
+    bool control = true;
+    for (uint32_t r = 0; r < D && control; ++r)
+      for (uint32_t c = 0; c < D; ++c)
+        if (((r ^ c) >> j) & 1u) {
+          const size_t e = size_t(r) * D + c;
+          if (l.mat[2 * e] != 0.0 || l.mat[2 * e + 1] != 0.0) {
+            control = false;
+            break;
+          }
+        }
+    if (!control) mask |= 1ull << l.q[j];
+  }
+  return mask;
+}
+
+uint64_t qubit_mask(const Launch& l) {
+  uint64_t m = 0;
+  for (uint32_t j = 0; j < l.m; ++j) m |= 1ull << l.q[j];
+  return m;
+}
+
+struct TileBuilder {
+  const Program* p;
+  size_t pass_begin = 0;  // first op of the pass being built
+  std::vector<TileOp> ops;
+  std::vector<TileDiag> diags;
+  std::vector<double> pool;  // complex entries, interleaved
+
+  uint64_t add_matrix(const double* g, size_t n_complex) {
+    const uint64_t off = pool.size() / 2;
+    pool.insert(pool.end(), g, g + 2 * n_complex);
+    return off;
+  }
+
+  // Lower launch l into ops for a pass whose tile bits are `tmask` (pos[q] = index of
+  // state bit q among the tile bits).
+  void lower(const Launch& l, uint64_t tmask, const uint32_t* pos) {
+    if (l.kind == Launch::kDiag) {
+      if (ops.size() == pass_begin || ops.back().kind != 1 ||
+          ops.back().off + ops.back().n_diag != diags.size()) {
+        TileOp op{};
+        op.kind = 1;
+        op.off = diags.size();
+        ops.push_back(op);
+      }
+      TileDiag d{};
+      d.m = l.m;
+      for (uint32_t j = 0; j < l.m; ++j) d.q[j] = l.q[j];
+      d.tab = uint32_t(add_matrix(l.mat.data(), size_t(1) << l.m));
+      diags.push_back(d);
+      ops.back().n_diag++;
+      return;
+    }
+    const uint32_t D = 1u << l.m;
+    uint32_t in_j[3], out_j[3];
+    uint32_t nin = 0, nout = 0;
+    for (uint32_t j = 0; j < l.m; ++j) {
+      if ((tmask >> l.q[j]) & 1u)
+        in_j[nin++] = j;
+      else
+        out_j[nout++] = j;
+    }
+    uint64_t cmask = 0;
+    for (uint32_t k = 0; k < nout; ++k) cmask |= 1ull << l.q[out_j[k]];
+    const uint32_t Din = 1u << nin;
+    for (uint32_t v = 0; v < (1u << nout); ++v) {
+      // sub-block for outside value v over the inside bits (in gate order)
+      auto full = [&](uint32_t s) {
+        uint32_t r = 0;
+        for (uint32_t k = 0; k < nin; ++k) r |= ((s >> k) & 1u) << in_j[k];
+        for (uint32_t k = 0; k < nout; ++k) r |= ((v >> k) & 1u) << out_j[k];
+        return r;
+      };
+      std::vector<double> sub(2 * size_t(Din) * Din);
+      bool identity = true;
+      for (uint32_t r = 0; r < Din; ++r)
+        for (uint32_t c = 0; c < Din; ++c) {
+          const size_t e = size_t(full(r)) * D + full(c);
+          sub[2 * (size_t(r) * Din + c)] = l.mat[2 * e];
+          sub[2 * (size_t(r) * Din + c) + 1] = l.mat[2 * e + 1];
+          if (l.mat[2 * e] != (r == c ? 1.0 : 0.0) || l.mat[2 * e + 1] != 0.0) identity = false;
+        }
+      if (identity && nout > 0) continue;
+      TileOp op{};
+      op.kind = nin == 1 ? 0 : nin;
+      op.m = nin;
+      for (uint32_t k = 0; k < nin; ++k) op.lq[k] = pos[l.q[in_j[k]]];
+      op.cmask = cmask;
+      uint64_t cval = 0;
+      for (uint32_t k = 0; k < nout; ++k) cval |= uint64_t((v >> k) & 1u) << l.q[out_j[k]];
+      op.cval = cval;
+      op.off = add_matrix(sub.data(), size_t(Din) * Din);
+      ops.push_back(op);
+    }
+  }
+};
+
+}  // namespace
+
 int plan_tiles(Program* p, int tile_qubits) {
-  if (tile_qubits > 0)
-    return set_err(p->st->ctx, kInvalidArg, "tile passes are not available in this build");
-  return kOk;
+  State& st = *p->st;
+  p->order.clear();
+  if (tile_qubits <= 0) {
+    for (uint64_t i = 0; i < p->launches.size(); ++i) p->order.push_back(i);
+    return kOk;
+  }
+  const uint32_t lo = st.dtype == 0 ? 10 : 11, hi = st.dtype == 0 ? 12 : 13;
+  if (uint32_t(tile_qubits) < lo || uint32_t(tile_qubits) > hi)
+    return set_err(st.ctx, kInvalidArg,
+                   st.dtype == 0 ? "tile_qubits must be 10..12 for complex128"
+                                 : "tile_qubits must be 11..13 for complex64");
+  const uint32_t tb = uint32_t(tile_qubits);
+  if (st.n < tb) {  // the state is smaller than one tile: plain launches
+    for (uint64_t i = 0; i < p->launches.size(); ++i) p->order.push_back(i);
+    return kOk;
+  }
+  const std::vector<Launch> in = std::move(p->launches);
+  p->launches.clear();
+  std::vector<uint64_t> act(in.size()), qm(in.size());
+  std::vector<bool> tileable(in.size());
+  for (size_t i = 0; i < in.size(); ++i) {
+    qm[i] = qubit_mask(in[i]);
+    act[i] = action_mask(in[i]);
+    tileable[i] = in[i].kind == Launch::kDiag || (popcount64(act[i]) <= 3 && in[i].m <= 3);
+  }
+  TileBuilder tbld{p, 0, {}, {}, {}};
+  const uint64_t low = (1ull << kTileLowBits) - 1;
+
+  auto flush = [&](std::vector<uint64_t> rem) {
+    while (!rem.empty()) {
+      // choose the tile bits: the low bits, then the action qubits of the gates that
+      // come next, in order, skipping gates behind a blocked qubit
+      uint64_t tmask = low, blocked = 0;
+      for (uint64_t i : rem) {
+        if (qm[i] & blocked) {
+          blocked |= qm[i];
+          continue;
+        }
+        const uint64_t need = act[i] & ~tmask;
+        if (popcount64(tmask) + popcount64(need) <= tb)
+          tmask |= need;
+        else
+          blocked |= qm[i];
+        if (popcount64(tmask) == tb) break;
+      }
+      for (uint32_t b = 0; popcount64(tmask) < tb && b < st.n; ++b) tmask |= 1ull << b;
+      TilePassArg pa{};
+      pa.tb = tb;
+      uint32_t pos[64];
+      uint32_t k = 0;
+      for (uint32_t b = 0; b < 64; ++b)
+        if ((tmask >> b) & 1u) {
+          pa.ts[k] = b;
+          pos[b] = k++;
+        }
+      pa.op_begin = uint32_t(tbld.ops.size());
+      tbld.pass_begin = tbld.ops.size();
+      std::vector<uint64_t> rest;
+      blocked = 0;
+      for (uint64_t i : rem) {
+        if ((qm[i] & blocked) || (act[i] & ~tmask)) {
+          blocked |= qm[i];
+          rest.push_back(i);
+          continue;
+        }
+        tbld.lower(in[i], tmask, pos);
+        p->order.push_back(i);
+      }
+      pa.n_ops = uint32_t(tbld.ops.size()) - pa.op_begin;
+      Launch l;
+      l.kind = Launch::kTile;
+      l.arena_off = p->tile_passes.size();
+      p->tile_passes.push_back(pa);
+      p->launches.push_back(std::move(l));
+      rem.swap(rest);
+    }
+  };
+
+  std::vector<uint64_t> seg;
+  for (uint64_t i = 0; i < in.size(); ++i) {
+    if (tileable[i]) {
+      seg.push_back(i);
+      continue;
+    }
+    flush(seg);
+    seg.clear();
+    p->launches.push_back(in[i]);
+    p->order.push_back(i);
+  }
+  flush(seg);
+
+  if (tbld.ops.empty()) return kOk;
+  // upload ops, diagonal descriptors and the matrix pool (state precision)
+  auto upload = [&](const void* src, size_t bytes, void** dst) {
+    hipError_t e = hipMalloc(dst, bytes ? bytes : 16);
+    if (e != hipSuccess) return hip_err(st.ctx, e, "tile buffers alloc");
+    p->owned.push_back(*dst);
+    if (bytes) e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    return hip_err(st.ctx, e, "tile buffers upload");
+  };
+  int rc = upload(tbld.ops.data(), tbld.ops.size() * sizeof(TileOp),
+                  reinterpret_cast<void**>(&p->tile_ops));
+  if (rc != kOk) return rc;
+  rc = upload(tbld.diags.data(), tbld.diags.size() * sizeof(TileDiag),
+              reinterpret_cast<void**>(&p->tile_diags));
+  if (rc != kOk) return rc;
+  if (st.dtype == 0) {
+    rc = upload(tbld.pool.data(), tbld.pool.size() * sizeof(double), &p->tile_pool);
+  } else {
+    std::vector<float> f(tbld.pool.begin(), tbld.pool.end());
+    rc = upload(f.data(), f.size() * sizeof(float), &p->tile_pool);
+  }
+  return rc;
 }
 
 int plan_mfma(Program* p) {
-  if (p->precision != QSVH_PREC_EXACT)
-    return set_err(p->st->ctx, kInvalidArg, "MFMA precisions are not available in this build");
-  return kOk;
+  if (p->precision == QSVH_PREC_EXACT) return kOk;
+  if (p->precision != QSVH_PREC_MFMA_F32 && p->precision != QSVH_PREC_MFMA_BF16X3)
+    return set_err(p->st->ctx, kInvalidArg, "unknown precision");
+  State& st = *p->st;
+  if (st.n < kMfmaTileBits) return kOk;  // too small for one MFMA tile: VALU kernels
+  std::vector<uint8_t> blob;
+  for (Launch& l : p->launches) {
+    if (l.kind != Launch::kMq || l.m < kMfmaMinQubits) continue;
+    uint32_t sorted[kMaxGateQubits];
+    const std::vector<float> a = block_matrix(l, sorted);
+    blob.resize((blob.size() + 255) & ~size_t(255));
+    l.mfma_off = blob.size();
+    if (p->precision == QSVH_PREC_MFMA_F32) {
+      const size_t bytes = a.size() * sizeof(float);
+      blob.resize(blob.size() + bytes);
+      std::memcpy(blob.data() + l.mfma_off, a.data(), bytes);
+    } else {
+      std::vector<uint16_t> hi(a.size()), lo(a.size());
+      for (size_t e = 0; e < a.size(); ++e) {
+        hi[e] = bf16_bits(a[e]);
+        lo[e] = bf16_bits(a[e] - bf16_value(hi[e]));
+      }
+      const size_t bytes = a.size() * sizeof(uint16_t);
+      blob.resize(blob.size() + 2 * bytes);
+      std::memcpy(blob.data() + l.mfma_off, hi.data(), bytes);
+      std::memcpy(blob.data() + l.mfma_off + bytes, lo.data(), bytes);
+    }
+    std::copy(sorted, sorted + l.m, l.q);
+    l.kind = Launch::kFusedMfma;
+  }
+  if (blob.empty()) return kOk;
+  void* d = nullptr;
+  hipError_t e = hipMalloc(&d, blob.size());
+  if (e != hipSuccess) return hip_err(st.ctx, e, "mfma matrices alloc");
+  p->owned.push_back(d);
+  p->mfma = d;
+  e = hipMemcpy(d, blob.data(), blob.size(), hipMemcpyHostToDevice);
+  return hip_err(st.ctx, e, "mfma matrices upload");
 }
 
 void release_extra(Program* p) {
   for (void* b : p->owned) hipFree(b);
   p->owned.clear();
+  p->mfma = nullptr;
+  p->tile_ops = nullptr;
+  p->tile_diags = nullptr;
+  p->tile_pool = nullptr;
 }
 
 }  // namespace qsvh

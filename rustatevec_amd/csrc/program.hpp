@@ -19,7 +19,16 @@ struct Program {
   bool use_graph = false;
   hipGraphExec_t graph = nullptr;
   int precision = 0;
+  void* mfma = nullptr;      // block matrices of the MFMA group launches
   std::vector<void*> owned;  // further device buffers freed with the program
+  // Tile passes: Launch::kTile launches index `tile_passes` through arena_off.
+  std::vector<TilePassArg> tile_passes;
+  TileOp* tile_ops = nullptr;
+  TileDiag* tile_diags = nullptr;
+  void* tile_pool = nullptr;
+  // Original op index of each applied gate, in execution order (tile passes reorder
+  // gates that act on disjoint qubits).
+  std::vector<uint64_t> order;
 };
 
 int set_err(Ctx* ctx, int code, const std::string& msg);
