@@ -78,8 +78,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller sizes (debugging)")
-    ap.add_argument("--cpu-gates", type=int, default=30)
+    ap.add_argument("--cpu-gates", type=int, default=160)
     ap.add_argument("--tile", type=int, default=12)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only 5 launches each of the dominant kernel (controlled-phase "
+                         "diagonal, 30q c128) and of the copy kernel, for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -87,6 +90,16 @@ def main():
         sys.exit(2)
 
     import rustatevec_amd as rs
+
+    if args.kernels_only:
+        b = rs.HipBackend(0)
+        st = b.alloc(30, rs.C128)
+        b.init_random_normal(st, 1)
+        ms_cp = b.time_gate(st, rs.gate_matrix("cp", np.pi / 8), [29, 15], 4)
+        ms_copy = b.time_copy(st, 4)
+        print(json.dumps({"diag_cp_ms": ms_cp, "copy_ms": ms_copy,
+                          "bytes_per_launch": (1 << 30) * 32}), flush=True)
+        return
 
     n = 24 if args.quick else 30
     n_layered = 22 if args.quick else 28
@@ -198,6 +211,29 @@ def main():
         f"tiled {lay_t:.4f} s ({lay_tp} passes)")
     st.free()
 
+    # ------------------------------------------------------------ batched HEA sweep
+    count = 64 if args.quick else 512
+    circuits = rs.hea_batch(count, 8, seed)
+    batch = rs.Batch(b, circuits, rs.C64)
+    tb_best, bl_best, _ = batch.autotune()
+    batch.run()
+    b_ms = min(batch.time() for _ in range(3))
+    z0 = batch.expectation_z0()
+    rng = np.random.default_rng(seed)
+    sampled = sorted(int(i) for i in rng.choice(count, size=8, replace=False))
+    psums = [float(batch.probabilities(i).sum()) for i in sampled]
+    ns = [c.n_qubits for c in circuits]
+    equiv = sum(8 * (3 * q - 1) * (1 << q) * 16 for q in ns)  # plain-pass bytes, c64
+    batch_rec = {"circuits": count, "qubits_min": min(ns), "qubits_max": max(ns),
+                 "gates": sum(len(c) for c in circuits), "tile_qubits": tb_best,
+                 "blocks": bl_best, "launches": batch.info()["launches"],
+                 "batch_ms": round(b_ms, 3),
+                 "plain_pass_equiv_gbs": round(equiv / (b_ms * 1e-3) / 1e9, 1),
+                 "z0_mean": float(z0.mean()), "z0_first": [round(float(x), 6) for x in z0[:4]],
+                 "sampled_prob_sums": [round(x, 6) for x in psums]}
+    log(f"batched HEA x{count}: {batch_rec}")
+    del batch, circuits
+
     # ------------------------------------------------------------ GHZ(20)
     st = b.alloc(20, rs.C128)
     b.init_basis(st, 0)
@@ -246,6 +282,7 @@ def main():
         "mfma_roofline": mfma_roof,
         "layered28": {"plain_s": round(lay_s, 4), "plain_passes": lay_passes,
                       "tiled_s": round(lay_t, 4), "tiled_passes": lay_tp},
+        "batched_hea": batch_rec,
         "ghz20_us": round(ghz_ms * 1e3, 2),
         "cpu_baseline": cpu,
     }

@@ -214,6 +214,43 @@ int qsvh_program_num_kernels(const qsvh_program* p, uint64_t* out);
 int qsvh_program_order(const qsvh_program* p, uint64_t* order, size_t cap, size_t* n);
 void qsvh_program_destroy(qsvh_program* p);
 
+/* ------------------------------------------------------------------ batches */
+/* Many independent circuits (uneven qubit counts, 4..30 qubits each) run together: each
+ * circuit's gates are scheduled into LDS tile passes and launch j runs pass j of every
+ * circuit in one persistent kernel (work items handed out by an atomic counter); the
+ * launches are captured in one HIP graph.  A circuit that fits one tile never leaves LDS
+ * until its final state is written.  Every run starts each circuit from |0...0> and ends
+ * with its <Z_0>.  No reference counterpart: the reference executes one circuit per
+ * Backend::execute call (crates/qsv-core/src/backend/mod.rs:66-73).  Gates must act on
+ * at most 3 qubits (or be diagonal). */
+typedef struct qsvh_batch qsvh_batch;
+int qsvh_batch_create(qsvh_ctx* ctx, const qsvh_circuit* const* circuits, size_t count,
+                      int dtype, qsvh_batch** out);
+void qsvh_batch_destroy(qsvh_batch* b);
+/* Re-plan with a tile size (complex128: 10..12 bits, complex64: 11..13) and a persistent
+ * grid (0: two workgroups per compute unit). */
+int qsvh_batch_configure(qsvh_batch* b, int tile_qubits, int blocks);
+/* Time every (tile size, grid of 1, 2, 4, 8 workgroups per CU) candidate on this batch
+ * (best of three runs each), keep the fastest and report it. */
+int qsvh_batch_autotune(qsvh_batch* b, int* tile_qubits, int* blocks, double* best_ms);
+/* Run every circuit (graph replay; the graph is captured on first use). */
+int qsvh_batch_run(qsvh_batch* b);
+/* Run once and return the device time in milliseconds (HIP events on the stream). */
+int qsvh_batch_time(qsvh_batch* b, double* ms);
+int qsvh_batch_info(const qsvh_batch* b, size_t* count, int* tile_qubits, int* blocks,
+                    uint64_t* n_launches, uint64_t* total_amplitudes);
+/* <Z_0> of every circuit after the last run (out[count]). */
+int qsvh_batch_expectation_z0(qsvh_batch* b, double* out);
+/* Final state / probabilities of circuit i (host buffers of 2^n_i elements). */
+int qsvh_batch_download(qsvh_batch* b, size_t i, void* re, void* im);
+int qsvh_batch_probabilities(qsvh_batch* b, size_t i, void* host_out);
+/* Circuit i's op indices in the order the batch applies them. */
+int qsvh_batch_order(const qsvh_batch* b, size_t i, uint64_t* order, size_t cap, size_t* n);
+/* The variational-sweep workload: `count` hardware-efficient-ansatz circuits; with
+ * rng = SplitMix64(seed), circuit c has n_c = 10 + rng.below(13) qubits and is
+ * qsvh_circuit_hea(n_c, depth, rng.next_u64()).  Writes count handles to out. */
+int qsvh_circuit_hea_batch(size_t count, uint32_t depth, uint64_t seed, qsvh_circuit** out);
+
 /* Apply one gate `reps` times back to back and return the mean device time per launch
  * (HIP events on the context stream).  Measurement helper for single-kernel rooflines. */
 int qsvh_time_gate(qsvh_state* st, uint32_t m, const uint32_t* qubits, const double* re,

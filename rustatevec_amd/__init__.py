@@ -110,6 +110,21 @@ def lib() -> C.CDLL:
         "qsvh_program_num_kernels": (i32, [P, C.POINTER(C.c_uint64)]),
         "qsvh_program_order": (i32, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "qsvh_program_destroy": (None, [P]),
+        "qsvh_circuit_hea_batch": (i32, [C.c_size_t, u32, u64, P]),
+        "qsvh_batch_create": (i32, [P, P, C.c_size_t, i32, PP]),
+        "qsvh_batch_destroy": (None, [P]),
+        "qsvh_batch_configure": (i32, [P, i32, i32]),
+        "qsvh_batch_autotune": (i32, [P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_double)]),
+        "qsvh_batch_run": (i32, [P]),
+        "qsvh_batch_time": (i32, [P, C.POINTER(C.c_double)]),
+        "qsvh_batch_info": (i32, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int), C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_uint64)]),
+        "qsvh_batch_expectation_z0": (i32, [P, P]),
+        "qsvh_batch_download": (i32, [P, C.c_size_t, P, P]),
+        "qsvh_batch_probabilities": (i32, [P, C.c_size_t, P]),
+        "qsvh_batch_order": (i32, [P, C.c_size_t, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "qsvh_time_gate": (i32, [P, u32, P, P, P, u32, C.POINTER(C.c_double)]),
         "qsvh_time_copy": (i32, [P, u32, C.POINTER(C.c_double)]),
     }
@@ -486,6 +501,85 @@ class HipBackend:
         ms = C.c_double(0)
         _check(lib().qsvh_time_copy(state.handle, reps, C.byref(ms)), self.ctx)
         return ms.value
+
+
+def hea_batch(count: int, depth: int, seed: int):
+    """The variational-sweep workload: `count` HEA circuits, n_c = 10 + below(13)."""
+    arr = (C.c_void_p * count)()
+    _check(lib().qsvh_circuit_hea_batch(count, depth, seed, arr))
+    return [Circuit(_handle=C.c_void_p(arr[i])) for i in range(count)]
+
+
+class Batch:
+    """Many circuits run together (persistent tile-pass launches in one HIP graph)."""
+
+    def __init__(self, backend: "HipBackend", circuits, dtype: int = C64):
+        self.backend = backend
+        self.circuits = list(circuits)
+        self.dtype = dtype
+        arr = (C.c_void_p * len(self.circuits))(*[c.handle.value for c in self.circuits])
+        h = C.c_void_p()
+        _check(lib().qsvh_batch_create(backend.ctx, arr, len(self.circuits), dtype,
+                                       C.byref(h)), backend.ctx)
+        self._h = h
+
+    def configure(self, tile_qubits: int, blocks: int = 0):
+        _check(lib().qsvh_batch_configure(self._h, tile_qubits, blocks), self.backend.ctx)
+
+    def autotune(self):
+        t, bl, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        _check(lib().qsvh_batch_autotune(self._h, C.byref(t), C.byref(bl), C.byref(ms)),
+               self.backend.ctx)
+        return t.value, bl.value, ms.value
+
+    def run(self):
+        _check(lib().qsvh_batch_run(self._h), self.backend.ctx)
+
+    def time(self) -> float:
+        ms = C.c_double(0)
+        _check(lib().qsvh_batch_time(self._h, C.byref(ms)), self.backend.ctx)
+        return ms.value
+
+    def info(self):
+        n, t, bl = C.c_size_t(0), C.c_int(0), C.c_int(0)
+        nl, amps = C.c_uint64(0), C.c_uint64(0)
+        _check(lib().qsvh_batch_info(self._h, C.byref(n), C.byref(t), C.byref(bl),
+                                     C.byref(nl), C.byref(amps)))
+        return {"count": n.value, "tile_qubits": t.value, "blocks": bl.value,
+                "launches": nl.value, "amplitudes": amps.value}
+
+    def expectation_z0(self) -> np.ndarray:
+        out = np.zeros(len(self.circuits))
+        _check(lib().qsvh_batch_expectation_z0(self._h, _ptr(out)), self.backend.ctx)
+        return out
+
+    def _np(self):
+        return np.float64 if self.dtype == C128 else np.float32
+
+    def download(self, i: int) -> np.ndarray:
+        n = self.circuits[i].n_qubits
+        re = np.empty(1 << n, dtype=self._np())
+        im = np.empty(1 << n, dtype=self._np())
+        _check(lib().qsvh_batch_download(self._h, i, _ptr(re), _ptr(im)), self.backend.ctx)
+        return re + 1j * im
+
+    def probabilities(self, i: int) -> np.ndarray:
+        n = self.circuits[i].n_qubits
+        out = np.empty(1 << n, dtype=self._np())
+        _check(lib().qsvh_batch_probabilities(self._h, i, _ptr(out)), self.backend.ctx)
+        return out
+
+    def order(self, i: int):
+        n = C.c_size_t(0)
+        lib().qsvh_batch_order(self._h, i, None, 0, C.byref(n))
+        out = np.zeros(max(n.value, 1), dtype=np.uint64)
+        _check(lib().qsvh_batch_order(self._h, i, _ptr(out), n.value, C.byref(n)))
+        return [int(x) for x in out[: n.value]]
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.qsvh_batch_destroy(self._h)
+            self._h = None
 
 
 def fuse_groups(circuit: Circuit, max_qubits: int = 4):

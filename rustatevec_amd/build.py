@@ -32,6 +32,8 @@ SOURCES = {
     "kernels_util.hip": [],
     "kernels_mfma.hip": [],
     "kernels_tile.hip": ["-ffp-contract=off"],
+    "kernels_batch.hip": ["-ffp-contract=off"],
+    "batch.cpp": [],
 }
 
 

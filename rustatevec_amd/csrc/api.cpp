@@ -14,19 +14,8 @@
 #include "engine.hpp"
 #include "ir.hpp"
 #include "program.hpp"
+#include "handles.hpp"
 
-struct qsvh_ctx {
-  qsvh::Ctx c;
-};
-struct qsvh_state {
-  qsvh::State s;
-};
-struct qsvh_circuit {
-  qsvh::Circuit c;
-};
-struct qsvh_program {
-  qsvh::Program p;
-};
 
 namespace qsvh {
 
@@ -369,6 +358,21 @@ int qsvh_circuit_hea(uint32_t n, uint32_t depth, uint64_t seed, qsvh_circuit** o
   if (n < 1) return kInvalidArg;
   return wrap(gen_hea(n, depth, seed), out);
 }
+int qsvh_circuit_hea_batch(size_t count, uint32_t depth, uint64_t seed, qsvh_circuit** out) {
+  if (!out) return kInvalidArg;
+  SplitMix64 rng(seed);
+  for (size_t c = 0; c < count; ++c) {
+    const uint32_t n = 10 + uint32_t(rng.below(13));
+    const uint64_t s = rng.next_u64();
+    const int rc = wrap(gen_hea(n, depth, s), &out[c]);
+    if (rc != kOk) {
+      for (size_t k = 0; k < c; ++k) qsvh_circuit_destroy(out[k]);
+      return rc;
+    }
+  }
+  return kOk;
+}
+
 int qsvh_circuit_mix(uint32_t n, uint64_t n_gates, uint64_t seed, qsvh_circuit** out) {
   if (n < 3) return kInvalidArg;
   return wrap(gen_mix(n, n_gates, seed), out);

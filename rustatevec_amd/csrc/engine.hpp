@@ -117,6 +117,29 @@ struct TilePassArg {
 hipError_t launch_tile(const State& st, const TilePassArg& pa, const TileOp* ops,
                        const TileDiag* diags, const void* pool, int max_blocks, hipStream_t s);
 
+// ---- kernels_batch.hip -----------------------------------------------------------------
+constexpr uint32_t kBatchInit = 1;    // build |0...0> in LDS instead of loading the tile
+constexpr uint32_t kBatchExpZ0 = 2;   // write the tile's partial <Z_0> to partials
+// One pass of one circuit of a batch: its tiles are work items
+// [tile_begin, tile_begin + n_tiles) of the launch.
+struct BatchPass {
+  uint64_t state_off;  // element offset of the circuit's state in the pool
+  uint64_t tile_begin;
+  uint64_t n_tiles;
+  uint32_t n, tb;
+  uint32_t ts[16];
+  uint32_t op_begin, n_ops;
+  uint32_t flags;
+  uint32_t part_off;  // first partial slot (kBatchExpZ0)
+};
+hipError_t launch_batch_pass(int dtype, const BatchPass* passes, uint32_t n_passes,
+                             uint64_t total_work, unsigned long long* counter, void* re_pool,
+                             void* im_pool, const TileOp* ops, const TileDiag* diags,
+                             const void* pool, double* partials, int blocks, hipStream_t s);
+hipError_t launch_batch_reduce(const uint32_t* part_off, const uint32_t* n_part,
+                               uint32_t n_circ, const double* partials, double* out,
+                               hipStream_t s);
+
 // ---- kernels_util.hip ------------------------------------------------------------------
 // Random normalised start state (counter-based SplitMix64 + Box-Muller); `d_norm` receives
 // the pre-normalisation <psi|psi>.

@@ -1,0 +1,164 @@
+// kernels_batch.hip -- many independent circuits in one persistent launch per pass.
+//
+// Every circuit of a batch has its own state in a shared pool and its own tile-pass
+// schedule (tile_ops.hpp).  Launch j runs pass j of every circuit that has one: its work
+// items are (circuit pass, tile) pairs, handed out through an atomic counter so that
+// uneven circuits (2^10 .. 2^22 amplitudes) load-balance over the workgroups.  A circuit
+// whose state fits one tile (n <= tile bits) is a single pass: it is built from |0..0>
+// in LDS, run to the end and written once.  The last pass of every circuit also forms
+// per-tile partial sums of <Z_0> = sum_i |psi_i|^2 (-1)^(i & 1) (sample.rs:177-200), which
+// a final kernel adds up per circuit in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "engine.hpp"
+#include "tile_ops.hpp"
+
+namespace qsvh {
+namespace {
+
+constexpr int kThreads = 512;
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+template <typename T, int MAXTB>
+__global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict__ passes,
+                                                    uint32_t n_passes, uint64_t total_work,
+                                                    unsigned long long* __restrict__ counter,
+                                                    T* __restrict__ re_pool,
+                                                    T* __restrict__ im_pool,
+                                                    const TileOp* __restrict__ ops,
+                                                    const TileDiag* __restrict__ diags,
+                                                    const T* __restrict__ pool,
+                                                    double* __restrict__ partials) {
+  constexpr uint32_t NMAX = 1u << MAXTB;
+  constexpr int W = 16 / int(sizeof(T));
+  __shared__ T sr[NMAX], si[NMAX];
+  __shared__ unsigned long long w_sh;
+  __shared__ double red[kThreads / 64];
+  using V = T __attribute__((ext_vector_type(W)));
+
+  for (;;) {
+    if (threadIdx.x == 0) w_sh = atomicAdd(counter, 1ull);
+    __syncthreads();
+    const uint64_t w = w_sh;
+    __syncthreads();
+    if (w >= total_work) break;
+    // the pass owning work item w (passes sorted by tile_begin)
+    uint32_t lo = 0, hi = n_passes - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (passes[mid].tile_begin <= w)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const BatchPass* bp = passes + lo;
+    const uint32_t tb = bp->tb;
+    const uint32_t N = 1u << tb;
+    const uint32_t* ts = bp->ts;  // uniform: read through the scalar cache
+    const uint64_t t = w - bp->tile_begin;
+    const uint64_t base = tile::insert_zero_bits(t, ts, tb);
+    T* re = re_pool + bp->state_off;
+    T* im = im_pool + bp->state_off;
+    const uint32_t flags = bp->flags;
+    for (uint32_t e = threadIdx.x * W; e < N; e += kThreads * W) {
+      if (flags & kBatchInit) {
+        // |0...0>: amplitude 1 at state index 0, which is tile 0, local index 0
+#pragma unroll
+        for (int v = 0; v < W; ++v) {
+          sr[e + v] = (t == 0 && e + v == 0) ? T(1) : T(0);
+          si[e + v] = T(0);
+        }
+      } else {
+        const uint64_t a = base + tile::offset(e, ts, tb);
+        const V vr = *reinterpret_cast<const V*>(re + a);
+        const V vi = *reinterpret_cast<const V*>(im + a);
+#pragma unroll
+        for (int v = 0; v < W; ++v) {
+          sr[e + v] = vr[v];
+          si[e + v] = vi[v];
+        }
+      }
+    }
+    __syncthreads();
+    tile::run_ops<T, kThreads, int(NMAX / kThreads)>(sr, si, N, base, ts, tb,
+                                                     ops + bp->op_begin, bp->n_ops, diags, pool);
+    if (flags & kBatchExpZ0) {
+      double acc = 0.0;
+      for (uint32_t e = threadIdx.x; e < N; e += kThreads) {
+        const double a = double(sr[e]), b = double(si[e]);
+        const double p = a * a + b * b;
+        acc += ((base | tile::offset(e, ts, tb)) & 1u) ? -p : p;
+      }
+      acc = wave_sum(acc);
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < kThreads / 64; ++k) s += red[k];
+        partials[bp->part_off + t] = s;
+      }
+    }
+    for (uint32_t e = threadIdx.x * W; e < N; e += kThreads * W) {
+      const uint64_t a = base + tile::offset(e, ts, tb);
+      V vr, vi;
+#pragma unroll
+      for (int v = 0; v < W; ++v) {
+        vr[v] = sr[e + v];
+        vi[v] = si[e + v];
+      }
+      *reinterpret_cast<V*>(re + a) = vr;
+      *reinterpret_cast<V*>(im + a) = vi;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_reset_counter(unsigned long long* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0ull;
+}
+
+// out[c] = sum of the circuit's last-pass partials, in tile order.
+__global__ void k_batch_reduce(const uint32_t* __restrict__ part_off,
+                               const uint32_t* __restrict__ n_part, uint32_t n_circ,
+                               const double* __restrict__ partials, double* __restrict__ out) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_circ) return;
+  double s = 0.0;
+  for (uint32_t k = 0; k < n_part[c]; ++k) s += partials[part_off[c] + k];
+  out[c] = s;
+}
+
+}  // namespace
+
+hipError_t launch_batch_pass(int dtype, const BatchPass* passes, uint32_t n_passes,
+                             uint64_t total_work, unsigned long long* counter, void* re_pool,
+                             void* im_pool, const TileOp* ops, const TileDiag* diags,
+                             const void* pool, double* partials, int blocks, hipStream_t s) {
+  k_reset_counter<<<1, 64, 0, s>>>(counter);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (dtype == 0) {
+    k_batch<double, 12><<<blocks, kThreads, 0, s>>>(
+        passes, n_passes, total_work, counter, static_cast<double*>(re_pool),
+        static_cast<double*>(im_pool), ops, diags, static_cast<const double*>(pool), partials);
+  } else {
+    k_batch<float, 13><<<blocks, kThreads, 0, s>>>(
+        passes, n_passes, total_work, counter, static_cast<float*>(re_pool),
+        static_cast<float*>(im_pool), ops, diags, static_cast<const float*>(pool), partials);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_reduce(const uint32_t* part_off, const uint32_t* n_part,
+                               uint32_t n_circ, const double* partials, double* out,
+                               hipStream_t s) {
+  k_batch_reduce<<<(n_circ + 255) / 256, 256, 0, s>>>(part_off, n_part, n_circ, partials, out);
+  return hipGetLastError();
+}
+
+}  // namespace qsvh

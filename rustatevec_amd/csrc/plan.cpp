@@ -96,22 +96,92 @@ uint64_t qubit_mask(const Launch& l) {
   return m;
 }
 
-struct TileBuilder {
-  const Program* p;
-  size_t pass_begin = 0;  // first op of the pass being built
-  std::vector<TileOp> ops;
-  std::vector<TileDiag> diags;
-  std::vector<double> pool;  // complex entries, interleaved
 
-  uint64_t add_matrix(const double* g, size_t n_complex) {
+}  // namespace
+
+void schedule_tiles(const std::vector<Launch>& in, uint32_t n, uint32_t tb, TileBuilder& bld,
+                    std::vector<TilePassArg>& passes, std::vector<Launch>& out,
+                    std::vector<uint64_t>& order) {
+  std::vector<uint64_t> act(in.size()), qm(in.size());
+  std::vector<bool> tileable(in.size());
+  for (size_t i = 0; i < in.size(); ++i) {
+    qm[i] = qubit_mask(in[i]);
+    act[i] = action_mask(in[i]);
+    tileable[i] = in[i].kind == Launch::kDiag || (popcount64(act[i]) <= 3 && in[i].m <= 3);
+  }
+  const uint64_t low = (1ull << std::min(kTileLowBits, tb)) - 1;
+
+  auto flush = [&](std::vector<uint64_t> rem) {
+    while (!rem.empty()) {
+      // choose the tile bits: the low bits, then the action qubits of the gates that
+      // come next, in order, skipping gates behind a blocked qubit
+      uint64_t tmask = low, blocked = 0;
+      for (uint64_t i : rem) {
+        if (qm[i] & blocked) {
+          blocked |= qm[i];
+          continue;
+        }
+        const uint64_t need = act[i] & ~tmask;
+        if (popcount64(tmask) + popcount64(need) <= tb)
+          tmask |= need;
+        else
+          blocked |= qm[i];
+        if (popcount64(tmask) == tb) break;
+      }
+      for (uint32_t b = 0; popcount64(tmask) < tb && b < n; ++b) tmask |= 1ull << b;
+      TilePassArg pa{};
+      pa.tb = tb;
+      uint32_t pos[64] = {0};
+      uint32_t k = 0;
+      for (uint32_t b = 0; b < 64; ++b)
+        if ((tmask >> b) & 1u) {
+          pa.ts[k] = b;
+          pos[b] = k++;
+        }
+      pa.op_begin = uint32_t(bld.ops.size());
+      bld.pass_begin = bld.ops.size();
+      std::vector<uint64_t> rest;
+      blocked = 0;
+      for (uint64_t i : rem) {
+        if ((qm[i] & blocked) || (act[i] & ~tmask)) {
+          blocked |= qm[i];
+          rest.push_back(i);
+          continue;
+        }
+        bld.lower(in[i], tmask, pos);
+        order.push_back(i);
+      }
+      pa.n_ops = uint32_t(bld.ops.size()) - pa.op_begin;
+      Launch l;
+      l.kind = Launch::kTile;
+      l.arena_off = passes.size();
+      passes.push_back(pa);
+      out.push_back(std::move(l));
+      rem.swap(rest);
+    }
+  };
+
+  std::vector<uint64_t> seg;
+  for (uint64_t i = 0; i < in.size(); ++i) {
+    if (tileable[i]) {
+      seg.push_back(i);
+      continue;
+    }
+    flush(seg);
+    seg.clear();
+    out.push_back(in[i]);
+    order.push_back(i);
+  }
+  flush(seg);
+}
+
+uint64_t TileBuilder::add_matrix(const double* g, size_t n_complex) {
     const uint64_t off = pool.size() / 2;
     pool.insert(pool.end(), g, g + 2 * n_complex);
     return off;
   }
 
-  // Lower launch l into ops for a pass whose tile bits are `tmask` (pos[q] = index of
-  // state bit q among the tile bits).
-  void lower(const Launch& l, uint64_t tmask, const uint32_t* pos) {
+void TileBuilder::lower(const Launch& l, uint64_t tmask, const uint32_t* pos) {
     if (l.kind == Launch::kDiag) {
       if (ops.size() == pass_begin || ops.back().kind != 1 ||
           ops.back().off + ops.back().n_diag != diags.size()) {
@@ -170,9 +240,6 @@ struct TileBuilder {
       ops.push_back(op);
     }
   }
-};
-
-}  // namespace
 
 int plan_tiles(Program* p, int tile_qubits) {
   State& st = *p->st;
@@ -193,79 +260,8 @@ int plan_tiles(Program* p, int tile_qubits) {
   }
   const std::vector<Launch> in = std::move(p->launches);
   p->launches.clear();
-  std::vector<uint64_t> act(in.size()), qm(in.size());
-  std::vector<bool> tileable(in.size());
-  for (size_t i = 0; i < in.size(); ++i) {
-    qm[i] = qubit_mask(in[i]);
-    act[i] = action_mask(in[i]);
-    tileable[i] = in[i].kind == Launch::kDiag || (popcount64(act[i]) <= 3 && in[i].m <= 3);
-  }
-  TileBuilder tbld{p, 0, {}, {}, {}};
-  const uint64_t low = (1ull << kTileLowBits) - 1;
-
-  auto flush = [&](std::vector<uint64_t> rem) {
-    while (!rem.empty()) {
-      // choose the tile bits: the low bits, then the action qubits of the gates that
-      // come next, in order, skipping gates behind a blocked qubit
-      uint64_t tmask = low, blocked = 0;
-      for (uint64_t i : rem) {
-        if (qm[i] & blocked) {
-          blocked |= qm[i];
-          continue;
-        }
-        const uint64_t need = act[i] & ~tmask;
-        if (popcount64(tmask) + popcount64(need) <= tb)
-          tmask |= need;
-        else
-          blocked |= qm[i];
-        if (popcount64(tmask) == tb) break;
-      }
-      for (uint32_t b = 0; popcount64(tmask) < tb && b < st.n; ++b) tmask |= 1ull << b;
-      TilePassArg pa{};
-      pa.tb = tb;
-      uint32_t pos[64];
-      uint32_t k = 0;
-      for (uint32_t b = 0; b < 64; ++b)
-        if ((tmask >> b) & 1u) {
-          pa.ts[k] = b;
-          pos[b] = k++;
-        }
-      pa.op_begin = uint32_t(tbld.ops.size());
-      tbld.pass_begin = tbld.ops.size();
-      std::vector<uint64_t> rest;
-      blocked = 0;
-      for (uint64_t i : rem) {
-        if ((qm[i] & blocked) || (act[i] & ~tmask)) {
-          blocked |= qm[i];
-          rest.push_back(i);
-          continue;
-        }
-        tbld.lower(in[i], tmask, pos);
-        p->order.push_back(i);
-      }
-      pa.n_ops = uint32_t(tbld.ops.size()) - pa.op_begin;
-      Launch l;
-      l.kind = Launch::kTile;
-      l.arena_off = p->tile_passes.size();
-      p->tile_passes.push_back(pa);
-      p->launches.push_back(std::move(l));
-      rem.swap(rest);
-    }
-  };
-
-  std::vector<uint64_t> seg;
-  for (uint64_t i = 0; i < in.size(); ++i) {
-    if (tileable[i]) {
-      seg.push_back(i);
-      continue;
-    }
-    flush(seg);
-    seg.clear();
-    p->launches.push_back(in[i]);
-    p->order.push_back(i);
-  }
-  flush(seg);
-
+  TileBuilder tbld;
+  schedule_tiles(in, st.n, tb, tbld, p->tile_passes, p->launches, p->order);
   if (tbld.ops.empty()) return kOk;
   // upload ops, diagonal descriptors and the matrix pool (state precision)
   auto upload = [&](const void* src, size_t bytes, void** dst) {

@@ -34,6 +34,27 @@ struct Program {
 int set_err(Ctx* ctx, int code, const std::string& msg);
 int hip_err(Ctx* ctx, hipError_t e, const char* where);
 
+// Lowered tile-pass ops of one or more circuits, with their matrices (f64, interleaved).
+struct TileBuilder {
+  size_t pass_begin = 0;  // first op of the pass being built
+  std::vector<TileOp> ops;
+  std::vector<TileDiag> diags;
+  std::vector<double> pool;  // complex entries, interleaved
+  uint64_t add_matrix(const double* g, size_t n_complex);
+  // Lower launch l into ops for a pass whose tile bits are `tmask` (pos[q] = index of
+  // state bit q among the tile bits).
+  void lower(const Launch& l, uint64_t tmask, const uint32_t* pos);
+};
+
+// Tile schedule of one circuit's classified launches on n qubits with tb tile bits
+// (tb <= n): appends lowered ops to `bld`, passes to `passes`, the resulting launch list
+// to `out` (kTile launches index `passes` through arena_off; launches that cannot be
+// tiled -- dense gates on more than 3 qubits -- are kept as they are) and the applied
+// order of the input launches to `order`.
+void schedule_tiles(const std::vector<Launch>& in, uint32_t n, uint32_t tb, TileBuilder& bld,
+                    std::vector<TilePassArg>& passes, std::vector<Launch>& out,
+                    std::vector<uint64_t>& order);
+
 // Passes that rewrite the plain launch list (plan.cpp).
 int plan_tiles(Program* p, int tile_qubits);
 int plan_mfma(Program* p);

@@ -91,6 +91,16 @@ def test_generators_match_oracle():
     _same_ops(rs.mix(9, 120, 6), oracle.gen_mix(9, 120, 6))
 
 
+def test_hea_batch_workload_matches_oracle():
+    circuits = rs.hea_batch(6, 8, 0xC0FFEE)
+    rng = oracle.SplitMix64(0xC0FFEE)
+    for c in circuits:
+        n = 10 + rng.below(13)
+        seed = rng.next_u64()
+        assert c.n_qubits == n
+        _same_ops(c, oracle.gen_hea(n, 8, seed))
+
+
 def test_golden_circuits_build_through_the_abi(golden):
     _, meta = golden
     for case in meta["cases"]:
