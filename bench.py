@@ -234,6 +234,51 @@ def main():
     log(f"batched HEA x{count}: {batch_rec}")
     del batch, circuits
 
+    # ------------------------------------------------------------ learned planner
+    from rustatevec_amd import planner
+
+    t0 = time.perf_counter()
+    X, y = planner.collect_timings(b, sizes=(22, 24) if args.quick else (24, 26),
+                                   per_size=150, seed=seed)
+    model, train_loss = planner.train_cost_model(b, X, y, epochs=2000)
+    t_model = time.perf_counter() - t0
+    nq = 22 if args.quick else 28
+    mixc = rs.mix(nq, 40 * nq, seed)
+    ref = b.alloc(nq, rs.C128)
+    resetr = lambda: b.init_basis(ref, 0)  # noqa: E731
+    p = rs.Program(ref, mixc)
+    mix_plain_s = timed_steps(b, ref, p, 1, 1, resetr)
+    resetr()
+    p.launch()
+    del p
+    st64 = b.alloc(nq, rs.C64)
+    reset64 = lambda: b.init_basis(st64, 0)  # noqa: E731
+    legs = {}
+    t0 = time.perf_counter()
+    planned, pred_ms, chosen = planner.plan(b, model, mixc, 6)
+    t_plan = time.perf_counter() - t0
+    cands = [("greedy_k%d" % k, rs.fuse(b, mixc, k)) for k in (4, 5, 6)] + [("learned", planned)]
+    for name, circ in cands:
+        p = rs.Program(st64, circ, precision=rs.PREC_MFMA_BF16X3)
+        s = timed_steps(b, st64, p, 2, 1, reset64)
+        reset64()
+        p.launch()
+        err, fid = b.compare(ref, st64)
+        legs[name] = {"passes": p.num_kernels(), "circuit_s": round(s, 5),
+                      "speedup_vs_plain_c128": round(mix_plain_s / s, 2),
+                      "max_abs_err": err, "one_minus_fidelity": 1 - fid}
+        del p
+    learned_rec = {"qubits": nq, "gates": len(mixc), "timings": int(len(X)),
+                   "train_mse_std": round(train_loss, 4), "model_s": round(t_model, 2),
+                   "plan_s": round(t_plan, 3), "predicted_ms": round(pred_ms, 3),
+                   "chosen": "dp" if chosen == 0 else f"greedy_k{chosen}",
+                   "plain_c128_s": round(mix_plain_s, 4), "legs": legs,
+                   "gain_vs_greedy_k4": round(legs["greedy_k4"]["circuit_s"]
+                                              / legs["learned"]["circuit_s"], 3)}
+    log(f"learned planner: {learned_rec}")
+    st64.free()
+    ref.free()
+
     # ------------------------------------------------------------ GHZ(20)
     st = b.alloc(20, rs.C128)
     b.init_basis(st, 0)
@@ -283,6 +328,7 @@ def main():
         "layered28": {"plain_s": round(lay_s, 4), "plain_passes": lay_passes,
                       "tiled_s": round(lay_t, 4), "tiled_passes": lay_tp},
         "batched_hea": batch_rec,
+        "learned_planner": learned_rec,
         "ghz20_us": round(ghz_ms * 1e3, 2),
         "cpu_baseline": cpu,
     }

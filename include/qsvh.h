@@ -71,7 +71,10 @@ typedef struct qsvh_program qsvh_program; /* a circuit bound to a state, ready t
 /* ------------------------------------------------------------------ context */
 /* CudaBackend::new(ordinal) (crates/qsv-cuda/src/lib.rs:57-77). */
 int qsvh_ctx_create(int device, qsvh_ctx** out);
+/* Every state, program, batch and model created on ctx must be destroyed first. */
 void qsvh_ctx_destroy(qsvh_ctx* ctx);
+/* Wait until all work queued on the context's stream has finished. */
+int qsvh_synchronize_ctx(qsvh_ctx* ctx);
 /* Text of the last error recorded on ctx ("" if none).  No reference counterpart:
  * the reference panics with the message instead. */
 const char* qsvh_last_error(const qsvh_ctx* ctx);
@@ -250,6 +253,47 @@ int qsvh_batch_order(const qsvh_batch* b, size_t i, uint64_t* order, size_t cap,
  * rng = SplitMix64(seed), circuit c has n_c = 10 + rng.below(13) qubits and is
  * qsvh_circuit_hea(n_c, depth, rng.next_u64()).  Writes count handles to out. */
 int qsvh_circuit_hea_batch(size_t count, uint32_t depth, uint64_t seed, qsvh_circuit** out);
+
+/* ------------------------------------------------------------------ learned planner */
+/* Cost model: a fully connected network n_features -> hidden -> hidden -> 1 with ReLU,
+ * FP32, trained on the device by the library's own forward, backward and Adam kernels.
+ * No reference counterpart: the reference fuses greedily with a fixed max_qubits
+ * (crates/qsv-core/src/fusion.rs:24-77). */
+typedef struct qsvh_mlp qsvh_mlp;
+int qsvh_mlp_create(qsvh_ctx* ctx, uint32_t n_features, uint32_t hidden, uint64_t seed,
+                    qsvh_mlp** out);
+void qsvh_mlp_destroy(qsvh_mlp* m);
+int qsvh_mlp_num_params(const qsvh_mlp* m, size_t* n);
+/* Flat parameters: W1[F x H], b1[H], W2[H x H], b2[H], W3[H x 1], b3 (row-major). */
+int qsvh_mlp_get_params(qsvh_mlp* m, float* out);
+int qsvh_mlp_set_params(qsvh_mlp* m, const float* in);
+/* Raw network outputs for rows x n_features inputs. */
+int qsvh_mlp_forward(qsvh_mlp* m, const float* x, size_t rows, float* out);
+/* Mean squared error of the raw outputs against y and its gradient w.r.t. every parameter
+ * (same layout as the parameters). */
+int qsvh_mlp_gradients(qsvh_mlp* m, const float* x, const float* y, size_t rows, float* grads,
+                       float* loss);
+/* Full-batch Adam on the MSE; targets are standardised internally (mean/stddev kept for
+ * qsvh_mlp_predict).  *final_loss: MSE on the standardised targets after training. */
+int qsvh_mlp_train(qsvh_mlp* m, const float* x, const float* y, size_t rows, uint32_t epochs,
+                   float lr, float* final_loss);
+/* Outputs in target units (de-standardised). */
+int qsvh_mlp_predict(qsvh_mlp* m, const float* x, size_t rows, float* out);
+int qsvh_mlp_target_scale(const qsvh_mlp* m, float* mean, float* stddev);
+/* The 12 cost-model features of a fused group on an n-qubit state: k/6, n/32, diagonal
+ * flag, density (nonzeros / 4^k), (q_i + 1)/n for the ascending qubits (6 slots, 0 when
+ * absent), min(lowest qubit, 8)/8, matrix-core flag (non-diagonal, k >= 4). */
+int qsvh_group_features(uint32_t n, uint32_t k, const uint32_t* qubits, int diagonal,
+                        double density, float* out);
+/* Plan the fusion of `in` with the cost model (targets: log2 of device milliseconds per
+ * group at 2^30 amplitudes): dynamic programming over contiguous groups of <= max_qubits
+ * qubits and <= window gates (0: 48), scored against the greedy plans of fusion::fuse at
+ * every max_qubits (fusion.rs:39-77), the cheapest predicted plan kept.  Groups are
+ * composed in f64 on the device.  *chosen: 0 for the dynamic programme, m for greedy at
+ * max_qubits m. */
+int qsvh_plan_fusion(qsvh_ctx* ctx, qsvh_mlp* model, const qsvh_circuit* in,
+                     uint32_t max_qubits, uint32_t window, qsvh_circuit** out,
+                     double* predicted_ms, int* chosen);
 
 /* Apply one gate `reps` times back to back and return the mean device time per launch
  * (HIP events on the context stream).  Measurement helper for single-kernel rooflines. */

@@ -65,6 +65,7 @@ def lib() -> C.CDLL:
         "qsvh_abi_version": (i32, []),
         "qsvh_ctx_create": (i32, [i32, PP]),
         "qsvh_ctx_destroy": (None, [P]),
+        "qsvh_synchronize_ctx": (i32, [P]),
         "qsvh_last_error": (C.c_char_p, [P]),
         "qsvh_tune_set": (i32, [P, C.c_char_p, C.c_int64]),
         "qsvh_tune_get": (i32, [P, C.c_char_p, C.POINTER(C.c_int64)]),
@@ -365,8 +366,15 @@ class HipBackend:
         self.ctx = h
 
     def close(self):
+        """Wait for queued work.  The context itself is destroyed when the last object
+        that uses it (states, programs, batches, models -- each keeps a reference to this
+        backend) is gone: the C ABI requires every such object to be destroyed first."""
         if self.ctx is not None:
-            lib().qsvh_ctx_destroy(self.ctx)
+            _check(lib().qsvh_synchronize_ctx(self.ctx), self.ctx)
+
+    def __del__(self):
+        if getattr(self, "ctx", None) is not None and _lib is not None:
+            _lib.qsvh_ctx_destroy(self.ctx)
             self.ctx = None
 
     def last_error(self) -> str:

@@ -34,6 +34,9 @@ SOURCES = {
     "kernels_tile.hip": ["-ffp-contract=off"],
     "kernels_batch.hip": ["-ffp-contract=off"],
     "batch.cpp": [],
+    "kernels_mlp.hip": [],
+    "mlp.cpp": [],
+    "planner.cpp": ["-ffp-contract=off"],
 }
 
 

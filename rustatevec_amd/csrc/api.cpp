@@ -233,6 +233,11 @@ void qsvh_ctx_destroy(qsvh_ctx* h) {
 
 const char* qsvh_last_error(const qsvh_ctx* h) { return h ? h->c.last_error.c_str() : ""; }
 
+int qsvh_synchronize_ctx(qsvh_ctx* h) {
+  if (!h) return kInvalidArg;
+  return hip_err(&h->c, hipStreamSynchronize(h->c.stream), "synchronize");
+}
+
 int qsvh_tune_set(qsvh_ctx* h, const char* key, int64_t value) {
   if (!h || !key) return kInvalidArg;
   Tuning& t = h->c.tune;

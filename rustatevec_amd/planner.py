@@ -1,0 +1,183 @@
+"""Learned fusion planning: timing collection, cost-model training and planning.
+
+The greedy ``fusion::fuse`` (nez0b/rustatevec crates/qsv-core/src/fusion.rs:39-77) groups
+consecutive gates up to a fixed ``max_qubits``.  Here a small network predicts the
+device time of a fused group from its features (include/qsvh.h,
+``qsvh_group_features``); it is trained on the device (``qsvh_mlp_train``) on timings
+that the engine itself produces for random groups, and ``qsvh_plan_fusion`` picks the
+group boundaries and widths that minimise the predicted total -- keeping a greedy plan
+when that predicts lower.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import (C64, PREC_MFMA_BF16X3, Circuit, HipBackend, Program, QsvhError, _check, _ptr,
+               lib)
+
+N_FEATURES = 12
+HIDDEN = 64
+
+
+def _bind():
+    L = lib()
+    P, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    PP = C.POINTER(C.c_void_p)
+    sig = {
+        "qsvh_mlp_create": (i32, [P, u32, u32, u64, PP]),
+        "qsvh_mlp_destroy": (None, [P]),
+        "qsvh_mlp_num_params": (i32, [P, C.POINTER(C.c_size_t)]),
+        "qsvh_mlp_get_params": (i32, [P, P]),
+        "qsvh_mlp_set_params": (i32, [P, P]),
+        "qsvh_mlp_forward": (i32, [P, P, C.c_size_t, P]),
+        "qsvh_mlp_gradients": (i32, [P, P, P, C.c_size_t, P, C.POINTER(C.c_float)]),
+        "qsvh_mlp_train": (i32, [P, P, P, C.c_size_t, u32, C.c_float,
+                                 C.POINTER(C.c_float)]),
+        "qsvh_mlp_predict": (i32, [P, P, C.c_size_t, P]),
+        "qsvh_mlp_target_scale": (i32, [P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "qsvh_group_features": (i32, [u32, u32, P, i32, C.c_double, P]),
+        "qsvh_plan_fusion": (i32, [P, P, P, u32, u32, PP, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+def group_features(n: int, qubits, diagonal: bool, density: float) -> np.ndarray:
+    q = np.asarray(sorted(qubits), dtype=np.uint32)
+    out = np.zeros(N_FEATURES, dtype=np.float32)
+    _check(_bind().qsvh_group_features(n, len(q), _ptr(q), int(bool(diagonal)), density,
+                                       _ptr(out)))
+    return out
+
+
+class Mlp:
+    """The cost-model network (device parameters, device training)."""
+
+    def __init__(self, backend: HipBackend, n_features=N_FEATURES, hidden=HIDDEN, seed=1):
+        self.backend = backend
+        self.F, self.H = n_features, hidden
+        h = C.c_void_p()
+        _check(_bind().qsvh_mlp_create(backend.ctx, n_features, hidden, seed, C.byref(h)),
+               backend.ctx)
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            _bind().qsvh_mlp_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def num_params(self) -> int:
+        n = C.c_size_t(0)
+        _check(_bind().qsvh_mlp_num_params(self._h, C.byref(n)))
+        return n.value
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), dtype=np.float32)
+        _check(_bind().qsvh_mlp_get_params(self._h, _ptr(out)), self.backend.ctx)
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params()
+        _check(_bind().qsvh_mlp_set_params(self._h, _ptr(p)), self.backend.ctx)
+
+    def forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros(len(X), dtype=np.float32)
+        _check(_bind().qsvh_mlp_forward(self._h, _ptr(X), len(X), _ptr(out)), self.backend.ctx)
+        return out
+
+    def predict(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros(len(X), dtype=np.float32)
+        _check(_bind().qsvh_mlp_predict(self._h, _ptr(X), len(X), _ptr(out)), self.backend.ctx)
+        return out
+
+    def gradients(self, X, y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(self.num_params(), dtype=np.float32)
+        loss = C.c_float(0)
+        _check(_bind().qsvh_mlp_gradients(self._h, _ptr(X), _ptr(y), len(X), _ptr(g),
+                                          C.byref(loss)), self.backend.ctx)
+        return float(loss.value), g
+
+    def train(self, X, y, epochs=1500, lr=3e-3) -> float:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = C.c_float(0)
+        _check(_bind().qsvh_mlp_train(self._h, _ptr(X), _ptr(y), len(X), epochs, lr,
+                                      C.byref(loss)), self.backend.ctx)
+        return float(loss.value)
+
+
+def _random_group(rng, n, k, kind):
+    qs = sorted(int(x) for x in rng.choice(n, size=k, replace=False))
+    d = 1 << k
+    if kind == "diag":
+        U = np.diag(np.exp(1j * rng.uniform(0, 2 * np.pi, size=d)))
+    elif kind == "perm":
+        U = np.eye(d)[rng.permutation(d)]
+        if k == 1:
+            U = np.array([[0, 1], [1, 0]], dtype=complex)
+    else:
+        z = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+        U, _ = np.linalg.qr(z)
+    density = float(np.count_nonzero(U)) / (d * d)
+    return qs, U, kind == "diag", density
+
+
+def collect_timings(backend: HipBackend, sizes=(22, 24, 26), per_size=200, seed=7,
+                    precision=PREC_MFMA_BF16X3, reps=3):
+    """Time random fused groups (k = 1..6; diagonal, permutation or dense) as the engine
+    runs them on complex64 states; returns features X and targets y = log2 of the device
+    milliseconds scaled to 2^30 amplitudes."""
+    rng = np.random.default_rng(seed)
+    X, y = [], []
+    for n in sizes:
+        st = backend.alloc(n, C64)
+        backend.init_random_normal(st, seed)
+        for _ in range(per_size):
+            k = int(rng.integers(1, 7))
+            kind = rng.choice(["diag", "perm", "dense"], p=[0.3, 0.2, 0.5])
+            qs, U, diag, dens = _random_group(rng, n, k, kind)
+            p = Program(st, Circuit(n).push(U, qs), precision=precision)
+            p.launch()
+            ms = min(p.time() for _ in range(reps))
+            X.append(group_features(n, qs, diag, dens))
+            y.append(np.log2(ms * 2.0 ** (30 - n)))
+            del p
+        st.free()
+    return np.array(X, dtype=np.float32), np.array(y, dtype=np.float32)
+
+
+def train_cost_model(backend: HipBackend, X, y, epochs=1500, lr=3e-3, seed=1):
+    m = Mlp(backend, seed=seed)
+    loss = m.train(X, y, epochs=epochs, lr=lr)
+    return m, loss
+
+
+def plan(backend: HipBackend, model: Mlp, circuit: Circuit, max_qubits=6, window=48):
+    """(planned circuit, predicted ms, chosen): chosen 0 = dynamic programme, m > 0 =
+    greedy fuse at max_qubits m."""
+    h = C.c_void_p()
+    ms = C.c_double(0)
+    chosen = C.c_int(0)
+    _check(_bind().qsvh_plan_fusion(backend.ctx, model.handle, circuit.handle, max_qubits,
+                                    window, C.byref(h), C.byref(ms), C.byref(chosen)),
+           backend.ctx)
+    return Circuit(_handle=h), ms.value, chosen.value
+
+
+__all__ = ["Mlp", "group_features", "collect_timings", "train_cost_model", "plan",
+           "QsvhError"]

@@ -1,0 +1,101 @@
+"""The learned planner: the cost model's device forward/backward against the float64
+numpy restatement (oracle/mlp.py), device training, and plans that are correct (judged
+against the complex128 plain run) and no slower than greedy fusion.
+
+Tolerances: FP32 network vs float64 oracle -- forward 1e-4 relative, gradients 2e-3
+relative to the largest gradient component."""
+import numpy as np
+import pytest
+
+import oracle
+import rustatevec_amd as rs
+from oracle import mlp as omlp
+from rustatevec_amd import planner
+
+pytestmark = pytest.mark.gpu
+
+
+def test_forward_and_gradients_match_oracle(backend):
+    m = planner.Mlp(backend, seed=3)
+    F, H = m.F, m.H
+    rng = np.random.default_rng(4)
+    X = rng.uniform(size=(300, F)).astype(np.float32)
+    y = rng.normal(size=300).astype(np.float32)
+    p = m.params()
+    want_out, _ = omlp.forward(p, X, F, H)
+    got_out = m.forward(X)
+    assert np.abs(got_out - want_out).max() < 1e-4 * max(1.0, np.abs(want_out).max())
+    loss, g = m.gradients(X, y)
+    want_loss, want_g = omlp.loss_and_grad(p, X, y, F, H)
+    assert abs(loss - want_loss) < 1e-4 * max(1.0, want_loss)
+    assert np.abs(g - want_g).max() < 2e-3 * np.abs(want_g).max()
+    # and after setting arbitrary parameters
+    p2 = rng.normal(scale=0.3, size=p.size).astype(np.float32)
+    m.set_params(p2)
+    loss, g = m.gradients(X, y)
+    want_loss, want_g = omlp.loss_and_grad(p2, X, y, F, H)
+    assert np.abs(g - want_g).max() < 2e-3 * np.abs(want_g).max()
+
+
+def test_training_fits_a_smooth_target(backend):
+    rng = np.random.default_rng(5)
+    X = rng.uniform(size=(512, 12)).astype(np.float32)
+    y = (np.sin(3 * X[:, 0]) + X[:, 1] * X[:, 2] - 0.5 * X[:, 3]).astype(np.float32)
+    m = planner.Mlp(backend, seed=6)
+    loss0, _ = m.gradients(X, (y - y.mean()) / y.std())
+    final = m.train(X, y, epochs=1500, lr=3e-3)
+    assert final < 0.05 * loss0, (loss0, final)
+    pred = m.predict(X)
+    assert np.abs(pred - y).mean() < 0.1 * np.abs(y - y.mean()).mean() + 0.05
+
+
+@pytest.fixture(scope="module")
+def cost_model(backend):
+    X, y = planner.collect_timings(backend, sizes=(20, 22), per_size=120, seed=11)
+    m, loss = planner.train_cost_model(backend, X, y, epochs=1500)
+    print(f"cost model: {len(X)} timings, final standardised MSE {loss:.4f}")
+    return m
+
+
+def test_planned_circuits_are_correct(backend, cost_model):
+    for n, seed in [(12, 1), (14, 2), (16, 3)]:
+        circ = rs.mix(n, 40 * n // 4, seed)
+        planned, pred_ms, chosen = planner.plan(backend, cost_model, circ, 6)
+        assert pred_ms > 0 and len(planned) <= len(circ)
+        ref = backend.execute(circ, rs.C128)
+        st = backend.alloc(n, rs.C64)
+        backend.init_basis(st, 0)
+        p = rs.Program(st, planned, precision=rs.PREC_MFMA_BF16X3)
+        p.launch()
+        err, fid = backend.compare(ref, st)
+        assert err < 1e-5 and 1 - fid < 1e-6, (n, err, fid)
+        # exact execution of the plan equals the oracle on the original circuit
+        exact = backend.execute(planned, rs.C128)
+        want = oracle.execute(oracle.CPU_PARALLEL, n, circ.ops())
+        assert np.abs(backend.download(exact) - want).max() < 1e-12
+        del p
+        for s in (st, ref, exact):
+            s.free()
+
+
+def test_planner_gain_over_greedy(backend, cost_model):
+    n = 22
+    circ = rs.mix(n, 40 * n, 21)
+    st = backend.alloc(n, rs.C64)
+
+    def run(c):
+        p = rs.Program(st, c, precision=rs.PREC_MFMA_BF16X3)
+        backend.init_basis(st, 0)
+        p.launch()
+        t = min(p.time() for _ in range(3))
+        return t
+
+    greedy = {k: run(rs.fuse(backend, circ, k)) for k in (2, 3, 4, 5, 6)}
+    planned, pred, chosen = planner.plan(backend, cost_model, circ, 6)
+    t_plan = run(planned)
+    best_k = min(greedy, key=greedy.get)
+    print(f"greedy ms {greedy}; planned {t_plan:.3f} ms (predicted {pred:.3f}, "
+          f"chosen {chosen}); gain vs greedy k=4: {greedy[4] / t_plan:.2f}x, "
+          f"vs best greedy k={best_k}: {greedy[best_k] / t_plan:.2f}x")
+    assert t_plan <= 1.15 * greedy[best_k]
+    st.free()
