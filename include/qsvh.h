@@ -85,8 +85,11 @@ int qsvh_abi_version(void);
 /* Launch-shape knobs of the streaming (1q, diagonal, copy) kernels, per context:
  * "max_blocks" (0: one thread per work item; n > 0: grid-stride loop over n blocks),
  * "unroll" (1, 2 or 4 independent 16-byte items per thread per iteration) and
- * "nontemporal" (0/1: streaming load/store hints); "mfma_blocks" (workgroups of the
- * persistent matrix-core group kernels).  No reference counterpart: the
+ * "nontemporal" (0/1: streaming load/store hints), "contig" (0: a thread's unrolled items
+ * are a grid apart; 1: adjacent); "mfma_blocks" and "tile_blocks" (workgroups of the
+ * persistent matrix-core group and tile-pass kernels); "im_pad" (bytes, a multiple of 256
+ * up to 64 MiB, placed between the re and im halves of states allocated afterwards).
+ * No reference counterpart: the
  * reference's kernels have a fixed 256-thread, one-item-per-thread shape
  * (crates/qsv-cuda/src/lib.rs:32,80-87). */
 int qsvh_tune_set(qsvh_ctx* ctx, const char* key, int64_t value);

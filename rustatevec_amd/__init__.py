@@ -357,6 +357,10 @@ class Program:
             self._h = None
 
 
+TUNE_KEYS = ("max_blocks", "unroll", "nontemporal", "mfma_blocks", "tile_blocks", "contig",
+             "im_pad")
+
+
 class HipBackend:
     """``Backend<R>`` (crates/qsv-core/src/backend/mod.rs:31-74) on one MI355X."""
 
@@ -381,11 +385,13 @@ class HipBackend:
         return (lib().qsvh_last_error(self.ctx) or b"").decode()
 
     def tune(self, **knobs):
-        """Set launch-shape knobs (max_blocks, unroll, nontemporal); returns all values."""
+        """Set launch-shape knobs (see TUNE_KEYS); returns all values.
+
+        im_pad applies to states allocated afterwards."""
         for k, v in knobs.items():
             _check(lib().qsvh_tune_set(self.ctx, k.encode(), int(v)), self.ctx)
         out = {}
-        for k in ("max_blocks", "unroll", "nontemporal", "mfma_blocks"):
+        for k in TUNE_KEYS:
             v = C.c_int64(0)
             _check(lib().qsvh_tune_get(self.ctx, k.encode(), C.byref(v)), self.ctx)
             out[k] = v.value

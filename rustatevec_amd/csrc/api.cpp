@@ -250,6 +250,12 @@ int qsvh_tune_set(qsvh_ctx* h, const char* key, int64_t value) {
     t.nontemporal = int(value);
   } else if (k == "mfma_blocks" && value >= 1 && value <= (1ll << 20)) {
     t.mfma_blocks = int(value);
+  } else if (k == "tile_blocks" && value >= 1 && value <= (1ll << 20)) {
+    t.tile_blocks = int(value);
+  } else if (k == "contig" && (value == 0 || value == 1)) {
+    t.contig = int(value);
+  } else if (k == "im_pad" && value >= 0 && value <= (64ll << 20) && value % 256 == 0) {
+    t.im_pad = value;
   } else {
     return set_err(&h->c, kInvalidArg, "unknown tuning key or value: " + k);
   }
@@ -264,6 +270,9 @@ int qsvh_tune_get(qsvh_ctx* h, const char* key, int64_t* value) {
   else if (k == "unroll") *value = t.unroll;
   else if (k == "nontemporal") *value = t.nontemporal;
   else if (k == "mfma_blocks") *value = t.mfma_blocks;
+  else if (k == "tile_blocks") *value = t.tile_blocks;
+  else if (k == "contig") *value = t.contig;
+  else if (k == "im_pad") *value = t.im_pad;
   else return set_err(&h->c, kInvalidArg, "unknown tuning key: " + k);
   return kOk;
 }
@@ -473,16 +482,18 @@ int qsvh_state_alloc(qsvh_ctx* h, uint32_t n, int dtype, qsvh_state** out) {
   s.ctx = &h->c;
   s.n = n;
   s.dtype = dtype;
+  // One allocation: re, then `im_pad` bytes, then im (the pad moves im[i] off the HBM
+  // channel/bank that re[i] maps to when the halves are a power of two apart).
   const size_t bytes = size_t(s.dim()) * s.elem();
+  const size_t pad = size_t(h->c.tune.im_pad);
   hipError_t e = hipSetDevice(h->c.device);
-  if (e == hipSuccess) e = hipMalloc(&s.re, bytes);
-  if (e == hipSuccess) e = hipMalloc(&s.im, bytes);
+  if (e == hipSuccess) e = hipMalloc(&s.block, 2 * bytes + pad);
   if (e != hipSuccess) {
-    hipFree(s.re);
-    hipFree(s.im);
     delete st;
     return hip_err(&h->c, e, "state alloc");
   }
+  s.re = s.block;
+  s.im = static_cast<char*>(s.block) + bytes + pad;
   *out = st;
   return kOk;
 }
@@ -490,8 +501,7 @@ int qsvh_state_alloc(qsvh_ctx* h, uint32_t n, int dtype, qsvh_state** out) {
 void qsvh_state_free(qsvh_state* st) {
   if (!st) return;
   hipStreamSynchronize(st->s.ctx->stream);
-  hipFree(st->s.re);
-  hipFree(st->s.im);
+  hipFree(st->s.block);
   delete st;
 }
 
@@ -774,19 +784,17 @@ int qsvh_time_copy(qsvh_state* st, uint32_t reps, double* ms) {
   if (!st || !ms || reps == 0) return kInvalidArg;
   State& s = st->s;
   const size_t bytes = size_t(s.dim()) * s.elem();
-  void *dr = nullptr, *di = nullptr;
-  hipError_t e = hipMalloc(&dr, bytes);
-  if (e == hipSuccess) e = hipMalloc(&di, bytes);
-  if (e != hipSuccess) {
-    hipFree(dr);
-    hipFree(di);
-    return hip_err(s.ctx, e, "copy alloc");
-  }
+  // The destination uses the same layout as a state allocation (re, pad, im).
+  const size_t pad = size_t(s.ctx->tune.im_pad);
+  void* blk = nullptr;
+  hipError_t e = hipMalloc(&blk, 2 * bytes + pad);
+  if (e != hipSuccess) return hip_err(s.ctx, e, "copy alloc");
+  void* dr = blk;
+  void* di = static_cast<char*>(blk) + bytes + pad;
   const int rc =
       time_launches(s.ctx, reps, ms, [&]() { return launch_copy(s, dr, di, s.ctx->tune, s.ctx->stream); });
   hipStreamSynchronize(s.ctx->stream);
-  hipFree(dr);
-  hipFree(di);
+  hipFree(blk);
   return rc;
 }
 

@@ -20,6 +20,11 @@ struct Tuning {
   int nontemporal = 0;         // 1: non-temporal (streaming) loads and stores
   int mfma_blocks = 2048;      // workgroups of the persistent MFMA group kernels
   int tile_blocks = 1024;      // workgroups of the persistent tile-pass kernel
+  int contig = 0;              // 1: a thread's unrolled items are adjacent (else strided)
+  // Bytes between the re and im halves of a state allocation.  Halves exactly a power of
+  // two apart map their same-index lines onto the same HBM channels; a 4 KiB shift
+  // measured +8% (c64) and up to +4% (c128) on 30-qubit passes.
+  int64_t im_pad = 4096;
 };
 
 struct Ctx {
@@ -41,6 +46,7 @@ struct State {
   int dtype = 0;  // 0 = c128 (f64 SoA), 1 = c64 (f32 SoA)
   void* re = nullptr;
   void* im = nullptr;
+  void* block = nullptr;  // the single allocation holding re, pad, im
   uint64_t dim() const { return 1ull << n; }
   size_t elem() const { return dtype == 0 ? sizeof(double) : sizeof(float); }
 };

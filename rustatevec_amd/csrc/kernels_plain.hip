@@ -77,6 +77,14 @@ __device__ __forceinline__ void pair_update(T& x0r, T& x0i, T& x1r, T& x1i, cons
   x1i = ci + di;
 }
 
+// Work item of unroll slot u in the grid-stride iteration whose first item for this thread
+// is v0: strided (slot u is u * nthr items further, so a wave's loads stay 1 KiB contiguous
+// per slot) or, with `contig`, the thread's U items adjacent.
+__device__ __forceinline__ uint64_t item_of(uint64_t v0, uint64_t tid, int u, int U,
+                                            uint64_t nthr, int contig) {
+  return contig ? (v0 - tid) + tid * uint64_t(U) + uint64_t(u) : v0 + uint64_t(u) * nthr;
+}
+
 // ---------------------------------------------------------------------------- 1q
 // Target q >= log2(W): each work item is W consecutive "lower" amplitudes, i.e. the
 // nested-block walk of cpu.rs:148-175 with the 2^(n-1) pairs enumerated as
@@ -84,17 +92,18 @@ __device__ __forceinline__ void pair_update(T& x0r, T& x0i, T& x1r, T& x1i, cons
 // consecutive 16-byte vectors of both halves: every access is coalesced for any q.
 template <typename T, int W, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_1q_wide(T* __restrict__ re, T* __restrict__ im,
-                                                    uint64_t n_items, uint32_t q, G2<T> g) {
+                                                    uint64_t n_items, uint32_t q, G2<T> g,
+                                                    int contig) {
   const uint64_t lowmask = (1ull << q) - 1;
   const uint64_t half = 1ull << q;
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t v0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v0 < n_items;
-       v0 += nthr * U) {
+  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (uint64_t v0 = tid; v0 < n_items; v0 += nthr * U) {
     T ar[U][W], ai[U][W], br[U][W], bi[U][W];
     uint64_t lo[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t v = v0 + uint64_t(u) * nthr;
+      const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
       const uint64_t x = v * W;
       lo[u] = ((x & ~lowmask) << 1) | (x & lowmask);
       if (v < n_items) {
@@ -106,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_1q_wide(T* __restrict__ re, T* __res
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t v = v0 + uint64_t(u) * nthr;
+      const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
       if (v < n_items) {
 #pragma unroll
         for (int j = 0; j < W; ++j) pair_update(ar[u][j], ai[u][j], br[u][j], bi[u][j], g);
@@ -124,15 +133,15 @@ __global__ __launch_bounds__(kBlock) void k_1q_wide(T* __restrict__ re, T* __res
 // qubit (a compile-time constant so the pairing stays in registers).
 template <typename T, int W, int Q, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_1q_intra(T* __restrict__ re, T* __restrict__ im,
-                                                     uint64_t n_items, G2<T> g) {
+                                                     uint64_t n_items, G2<T> g, int contig) {
   constexpr int h = 1 << Q;
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t v0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v0 < n_items;
-       v0 += nthr * U) {
+  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (uint64_t v0 = tid; v0 < n_items; v0 += nthr * U) {
     T xr[U][W], xi[U][W];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t v = v0 + uint64_t(u) * nthr;
+      const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
       if (v < n_items) {
         vload<T, W, NT>(re + v * W, xr[u]);
         vload<T, W, NT>(im + v * W, xi[u]);
@@ -140,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_1q_intra(T* __restrict__ re, T* __re
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t v = v0 + uint64_t(u) * nthr;
+      const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
       if (v < n_items) {
 #pragma unroll
         for (int j = 0; j < W; ++j)
@@ -164,7 +173,8 @@ struct DiagArg {
 
 template <typename T, int W, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ re, T* __restrict__ im,
-                                                 uint64_t n_items, DiagArg<T> a) {
+                                                 uint64_t n_items, DiagArg<T> a,
+                                                 int contig) {
   __shared__ T sdr[64], sdi[64];
   const uint32_t sub = 1u << a.m;
   if (threadIdx.x < sub) {
@@ -173,12 +183,12 @@ __global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ re, T* __restri
   }
   __syncthreads();
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t v0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v0 < n_items;
-       v0 += nthr * U) {
+  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (uint64_t v0 = tid; v0 < n_items; v0 += nthr * U) {
     T xr[U][W], xi[U][W];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t v = v0 + uint64_t(u) * nthr;
+      const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
       if (v < n_items) {
         vload<T, W, NT>(re + v * W, xr[u]);
         vload<T, W, NT>(im + v * W, xi[u]);
@@ -186,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ re, T* __restri
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t v = v0 + uint64_t(u) * nthr;
+      const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
       if (v < n_items) {
 #pragma unroll
         for (int j = 0; j < W; ++j) {
@@ -404,14 +414,15 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double* __restric
 template <typename T, int W, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_copy(const T* __restrict__ sr,
                                                  const T* __restrict__ si, T* __restrict__ dr,
-                                                 T* __restrict__ di, uint64_t n_items) {
+                                                 T* __restrict__ di, uint64_t n_items,
+                                                 int contig) {
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t v0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v0 < n_items;
-       v0 += nthr * U) {
+  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (uint64_t v0 = tid; v0 < n_items; v0 += nthr * U) {
     T a[U][W], b[U][W];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t v = v0 + uint64_t(u) * nthr;
+      const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
       if (v < n_items) {
         vload<T, W, NT>(sr + v * W, a[u]);
         vload<T, W, NT>(si + v * W, b[u]);
@@ -419,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_copy(const T* __restrict__ sr,
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t v = v0 + uint64_t(u) * nthr;
+      const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
       if (v < n_items) {
         vstore<T, W, NT>(dr + v * W, a[u]);
         vstore<T, W, NT>(di + v * W, b[u]);
@@ -474,16 +485,19 @@ hipError_t launch_1q_t(const State& st, uint32_t q, const G2<T>& g, const Tuning
     constexpr bool NT = decltype(ntc)::value;
     if (int(q) >= ilog2(W)) {
       const uint64_t items = (dim >> 1) / W;
-      k_1q_wide<T, W, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, q, g);
+      k_1q_wide<T, W, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, q, g,
+                                                                   t.contig);
     } else {
       const uint64_t items = dim / W;
       if constexpr (W >= 2) {
         if (q == 0)
-          k_1q_intra<T, W, 0, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, g);
+          k_1q_intra<T, W, 0, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, g,
+                                                                          t.contig);
       }
       if constexpr (W >= 4) {
         if (q == 1)
-          k_1q_intra<T, W, 1, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, g);
+          k_1q_intra<T, W, 1, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, g,
+                                                                          t.contig);
       }
     }
     return hipGetLastError();
@@ -521,10 +535,10 @@ hipError_t launch_diag_any(const State& st, uint32_t m, const uint32_t* q, const
     constexpr bool NT = decltype(ntc)::value;
     if (st.dim() >= uint64_t(W)) {
       const uint64_t items = st.dim() / W;
-      k_diag<T, W, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, a);
+      k_diag<T, W, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, a, t.contig);
     } else {
       const uint64_t items = st.dim();
-      k_diag<T, 1, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, a);
+      k_diag<T, 1, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, a, t.contig);
     }
     return hipGetLastError();
   });
@@ -659,10 +673,10 @@ hipError_t launch_copy_t(const State& st, void* dr, void* di, const Tuning& t, h
     if (st.dim() >= uint64_t(W)) {
       const uint64_t items = st.dim() / W;
       k_copy<T, W, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(
-          sr, si, static_cast<T*>(dr), static_cast<T*>(di), items);
+          sr, si, static_cast<T*>(dr), static_cast<T*>(di), items, t.contig);
     } else {
       k_copy<T, 1, U, NT><<<1, kBlock, 0, s>>>(sr, si, static_cast<T*>(dr),
-                                               static_cast<T*>(di), st.dim());
+                                               static_cast<T*>(di), st.dim(), 0);
     }
     return hipGetLastError();
   });
