@@ -109,11 +109,28 @@ struct TileOp {
   uint64_t cmask, cval;
   uint64_t off;
 };
-// A diagonal gate inside a diag run: global qubits q[0..m), phases at pool[tab ..].
+// Tile kernels run 2^kTileLaneBits threads; tile-local index e = tid + k * 2^kTileLaneBits,
+// so tile-local bits below kTileLaneBits are lane bits and the others register bits.
+constexpr uint32_t kTileLaneBits = 9;
+constexpr uint32_t kOutside = 0xffffffffu;
+// A diagonal gate inside a diag run, lowered for one pass.
+//  m <= 2: the gate's bits are sorted into outside bits (state bits oq0, oq1 of the tile
+//  base: so = bit oq0 | bit oq1 << 1, nout of them) and up to two "index bits" forming
+//  c (0..3), each absent, a lane bit (tid bit lp0 / lp1) or a register bit (k bit p);
+//  variant = kind0 | kind1 << 4 with kind 0 absent, 1 lane, 2 + p register bit p.  The
+//  phase of an amplitude is pool[tab + 4 so + c]; bit 4 so + c of idw: it is exactly 1 + 0i.
+//  m > 2: phases pool[tab + s], s = gather of bits q[j] (tile-local bit loc[j] or, for
+//  kOutside, the base bit).
 struct TileDiag {
   uint32_t m;
-  uint32_t q[kMaxGateQubits];
+  uint32_t variant;
+  uint32_t lp0, lp1;
+  uint32_t oq0, oq1;
+  uint32_t nout;
   uint32_t tab;
+  uint32_t idw;
+  uint32_t q[kMaxGateQubits];
+  uint32_t loc[kMaxGateQubits];
 };
 struct TilePassArg {
   uint32_t tb;      // tile bits

@@ -16,7 +16,7 @@
 namespace qsvh {
 namespace {
 
-constexpr int kThreads = 512;
+constexpr int kThreads = 1 << kTileLaneBits;
 
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -85,8 +85,8 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict_
       }
     }
     __syncthreads();
-    tile::run_ops<T, kThreads, int(NMAX / kThreads)>(sr, si, N, base, ts, tb,
-                                                     ops + bp->op_begin, bp->n_ops, diags, pool);
+    tile::run_ops<T, kThreads, int(NMAX / kThreads)>(sr, si, N, base, ops + bp->op_begin,
+                                                     bp->n_ops, diags, pool);
     if (flags & kBatchExpZ0) {
       double acc = 0.0;
       for (uint32_t e = threadIdx.x; e < N; e += kThreads) {

@@ -23,7 +23,7 @@
 namespace qsvh {
 namespace {
 
-constexpr int kThreads = 512;
+constexpr int kThreads = 1 << kTileLaneBits;
 
 template <typename T, int TB>
 __global__ __launch_bounds__(kThreads) void k_tile(T* __restrict__ re, T* __restrict__ im,
@@ -54,8 +54,8 @@ __global__ __launch_bounds__(kThreads) void k_tile(T* __restrict__ re, T* __rest
       }
     }
     __syncthreads();
-    tile::run_ops<T, kThreads, int(N / kThreads)>(sr, si, N, base, pa.ts, TB, ops + pa.op_begin,
-                                                  pa.n_ops, diags, pool);
+    tile::run_ops<T, kThreads, int(N / kThreads)>(sr, si, N, base, ops + pa.op_begin, pa.n_ops,
+                                                  diags, pool);
 #pragma unroll
     for (uint32_t c = 0; c < PER; ++c) {
       const uint32_t e = (threadIdx.x + kThreads * c) * W;

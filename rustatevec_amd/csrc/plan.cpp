@@ -192,8 +192,48 @@ void TileBuilder::lower(const Launch& l, uint64_t tmask, const uint32_t* pos) {
       }
       TileDiag d{};
       d.m = l.m;
-      for (uint32_t j = 0; j < l.m; ++j) d.q[j] = l.q[j];
-      d.tab = uint32_t(add_matrix(l.mat.data(), size_t(1) << l.m));
+      for (uint32_t j = 0; j < l.m; ++j) {
+        d.q[j] = l.q[j];
+        d.loc[j] = ((tmask >> l.q[j]) & 1u) ? pos[l.q[j]] : kOutside;
+      }
+      if (l.m > 2) {
+        d.tab = uint32_t(add_matrix(l.mat.data(), size_t(1) << l.m));
+      } else {
+        // index bits: lane bits first, then register bits by ascending k bit
+        struct Bit { uint32_t j, kind, lp; };
+        Bit ib[2];
+        uint32_t ni = 0, oj[2], no = 0;
+        for (uint32_t j = 0; j < l.m; ++j) {
+          if (d.loc[j] == kOutside) {
+            oj[no++] = j;
+          } else if (d.loc[j] < kTileLaneBits) {
+            ib[ni++] = Bit{j, 1, d.loc[j]};
+          } else {
+            ib[ni++] = Bit{j, 2 + d.loc[j] - kTileLaneBits, 0};
+          }
+        }
+        if (ni == 2 && (ib[0].kind > ib[1].kind)) std::swap(ib[0], ib[1]);
+        d.variant = (ni > 0 ? ib[0].kind : 0) | ((ni > 1 ? ib[1].kind : 0) << 4);
+        d.lp0 = ni > 0 && ib[0].kind == 1 ? ib[0].lp : 0;
+        d.lp1 = ni > 1 && ib[1].kind == 1 ? ib[1].lp : 0;
+        d.nout = no;
+        d.oq0 = no > 0 ? l.q[oj[0]] : 0;
+        d.oq1 = no > 1 ? l.q[oj[1]] : 0;
+        // 4 phases per outside value so, indexed by c; unreachable c hold zeros
+        std::vector<double> tab(2 * 4 * (size_t(1) << no), 0.0);
+        for (uint32_t so = 0; so < (1u << no); ++so)
+          for (uint32_t c = 0; c < 4; ++c) {
+            if ((c >> ni) != 0) continue;
+            uint32_t s = 0;
+            for (uint32_t i = 0; i < ni; ++i) s |= ((c >> i) & 1u) << ib[i].j;
+            for (uint32_t o = 0; o < no; ++o) s |= ((so >> o) & 1u) << oj[o];
+            const size_t at = 2 * (4 * size_t(so) + c);
+            tab[at] = l.mat[2 * s];
+            tab[at + 1] = l.mat[2 * s + 1];
+            if (tab[at] == 1.0 && tab[at + 1] == 0.0) d.idw |= 1u << (4 * so + c);
+          }
+        d.tab = uint32_t(add_matrix(tab.data(), tab.size() / 2));
+      }
       diags.push_back(d);
       ops.back().n_diag++;
       return;

@@ -90,73 +90,194 @@ __device__ __forceinline__ void apply_mq(T* sr, T* si, uint32_t N, const TileOp&
   }
 }
 
-// A run of diagonal gates.  Each lane keeps its KMAX amplitudes (e = tid + k NT) and their
-// state indices in registers for the whole run; a gate's descriptor and (for m <= 2) its
-// phases are uniform, so per amplitude the work is a bit extraction, a select and one
-// complex multiply -- the same multiply, in the same order, as diag_chunk.
+constexpr uint32_t ilog2c(uint32_t x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
+
+// f(integral_constant<int, k>) for k = K0 .. K1-1.  Register arrays indexed only through
+// it are indexed by constants from the start, so they become plain registers (a loop
+// unrolled later leaves them as one vector value that control flow then copies whole).
+template <int K0, int K1, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (K0 < K1) {
+    f(std::integral_constant<int, K0>{});
+    static_for<K0 + 1, K1>(f);
+  }
+}
+
+// Entry i (0..3) of a 4-entry table held in registers.  Selects on values, not on
+// element references: a select of references becomes a select of addresses and puts the
+// table in private memory.
+template <typename T>
+__device__ __forceinline__ T sel4(const T (&t)[4], uint32_t i) {
+  const T t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
+  const T lo = (i & 1u) ? t1 : t0;
+  const T hi = (i & 1u) ? t3 : t2;
+  return (i & 2u) ? hi : lo;
+}
+
+// One diagonal gate of m <= 2 qubits on the lane's KMAX amplitudes (tile-local index
+// e = tid + k NT), lowered by the host (TileDiag): its outside bits pick one 4-entry phase
+// table per tile, and the phase index c is made of at most two bits, each a lane bit (fixed
+// for the lane) or a register bit (a bit of k).  Register-bit positions become compile-time
+// constants (a switch over the few possible positions), so per amplitude the work is one
+// complex multiply whose phase operand is known to the instruction.  An amplitude whose
+// phase is exactly 1 + 0i is left as it is: the multiply would return it unchanged (up to
+// the sign of a zero).
+// Call f with integral_constant<int, p> for a runtime p < LK.
+template <int LK, typename F>
+__device__ __forceinline__ void with_kpos(uint32_t p, F&& f) {
+  using std::integral_constant;
+  if constexpr (LK > 0) if (p == 0) return f(integral_constant<int, 0>{});
+  if constexpr (LK > 1) if (p == 1) return f(integral_constant<int, 1>{});
+  if constexpr (LK > 2) if (p == 2) return f(integral_constant<int, 2>{});
+  if constexpr (LK > 3) if (p == 3) return f(integral_constant<int, 3>{});
+  if constexpr (LK > 4) if (p == 4) return f(integral_constant<int, 4>{});
+}
+
+// Phases uniform over the workgroup: pr/pi[c] is the phase of an amplitude whose register
+// bits at positions KA, KB (-1: none) read c = bit KA | bit KB << 1; bit c of `id` set:
+// that phase is the identity.
+template <typename T, int NT, int KMAX, int KA, int KB>
+__device__ __forceinline__ void diag_uniform(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N,
+                                             const T (&pr)[4], const T (&pi)[4], uint32_t id) {
+  static_for<0, KMAX>([&](auto kc) {
+    constexpr int k = kc.value;
+    constexpr int c = (KA >= 0 ? (k >> (KA < 0 ? 0 : KA)) & 1 : 0) |
+                      (KB >= 0 ? ((k >> (KB < 0 ? 0 : KB)) & 1) << 1 : 0);
+    if (!((id >> c) & 1u) && threadIdx.x + uint32_t(k) * NT < N) {
+      asm volatile("");  // keeps the uniform skip a branch (no speculated multiply + select)
+      cmul(xr[k], xi[k], pr[c], pi[c]);
+    }
+  });
+}
+
+// Phases per lane: p0 for register bit KA = 0 (or KA = -1: every k), p1 for bit KA = 1.
+template <typename T, int NT, int KMAX, int KA>
+__device__ __forceinline__ void diag_lane(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N, T p0r,
+                                          T p0i, T p1r, T p1i, bool skip0, bool skip1) {
+  static_for<0, KMAX>([&](auto kc) {
+    constexpr int k = kc.value;
+    constexpr bool hi = KA >= 0 && ((k >> (KA < 0 ? 0 : KA)) & 1);
+    if (!(hi ? skip1 : skip0) && threadIdx.x + uint32_t(k) * NT < N) {
+      asm volatile("");
+      cmul(xr[k], xi[k], hi ? p1r : p0r, hi ? p1i : p0i);
+    }
+  });
+}
+
+template <typename T, int NT, int KMAX>
+__device__ __forceinline__ void diag_gate_small(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N,
+                                                uint64_t base, const TileDiag* __restrict__ dg,
+                                                const T* __restrict__ pool) {
+  constexpr int LK = int(ilog2c(KMAX));
+  const uint32_t nout = dg->nout;
+  uint32_t so = 0;
+  if (nout > 0) so = uint32_t((base >> dg->oq0) & 1u);
+  if (nout > 1) so |= uint32_t((base >> dg->oq1) & 1u) << 1;
+  const T* __restrict__ tab = pool + 2 * (dg->tab + 4 * so);
+  T er[4], ei[4];
+  static_for<0, 4>([&](auto c) {
+    er[c.value] = tab[2 * c.value];
+    ei[c.value] = tab[2 * c.value + 1];
+  });
+  const uint32_t id = (dg->idw >> (4 * so)) & 15u;
+  const uint32_t v = dg->variant;
+  const uint32_t kind0 = v & 15u, kind1 = v >> 4;
+  if (kind0 == 0) {  // one phase for the whole tile
+    diag_uniform<T, NT, KMAX, -1, -1>(xr, xi, N, er, ei, id);
+  } else if (kind0 >= 2) {  // register bits only: phases uniform for each k
+    if (kind1 == 0) {
+      with_kpos<LK>(kind0 - 2, [&](auto A) {
+        diag_uniform<T, NT, KMAX, A.value, -1>(xr, xi, N, er, ei, id);
+      });
+    } else {
+      with_kpos<LK>(kind0 - 2, [&](auto A) {
+        with_kpos<LK>(kind1 - 2, [&](auto B) {
+          if constexpr (A.value < B.value)
+            diag_uniform<T, NT, KMAX, A.value, B.value>(xr, xi, N, er, ei, id);
+        });
+      });
+    }
+  } else {  // index bit 0 is a lane bit
+    const bool l0 = (threadIdx.x >> dg->lp0) & 1u;
+    if (kind1 == 0) {
+      if ((id & 3u) != 3u) {
+        const T pr = l0 ? er[1] : er[0], pi = l0 ? ei[1] : ei[0];
+        diag_lane<T, NT, KMAX, -1>(xr, xi, N, pr, pi, pr, pi, false, false);
+      }
+    } else if (kind1 == 1) {
+      if (id != 15u) {
+        const uint32_t c = uint32_t(l0) | (((threadIdx.x >> dg->lp1) & 1u) << 1);
+        const T pr = sel4(er, c), pi = sel4(ei, c);
+        diag_lane<T, NT, KMAX, -1>(xr, xi, N, pr, pi, pr, pi, false, false);
+      }
+    } else {
+      const T p0r = l0 ? er[1] : er[0], p0i = l0 ? ei[1] : ei[0];
+      const T p1r = l0 ? er[3] : er[2], p1i = l0 ? ei[3] : ei[2];
+      const bool skip0 = (id & 3u) == 3u, skip1 = (id >> 2) == 3u;
+      with_kpos<LK>(kind1 - 2, [&](auto A) {
+        diag_lane<T, NT, KMAX, A.value>(xr, xi, N, p0r, p0i, p1r, p1i, skip0, skip1);
+      });
+    }
+  }
+}
+
+// A diagonal gate on m > 2 qubits: sub-index from the tile-local index bit by bit.
+template <typename T, int NT, int KMAX>
+__device__ __forceinline__ void diag_gate_wide(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N,
+                                               uint64_t base, const TileDiag* __restrict__ dg,
+                                               const T* __restrict__ pool) {
+  const uint32_t m = dg->m;
+  const T* __restrict__ tab = pool + 2 * dg->tab;
+  static_for<0, KMAX>([&](auto kc) {
+    constexpr int k = kc.value;
+    const uint32_t e = threadIdx.x + uint32_t(k) * NT;
+    if (e < N) {
+      uint32_t s = 0;
+      for (uint32_t j = 0; j < m; ++j) {
+        const uint32_t loc = dg->loc[j];
+        const uint32_t bit = loc == kOutside ? uint32_t((base >> dg->q[j]) & 1u) : (e >> loc) & 1u;
+        s |= bit << j;
+      }
+      cmul(xr[k], xi[k], tab[2 * s], tab[2 * s + 1]);
+    }
+  });
+}
+
+// A run of diagonal gates.  Each lane keeps its KMAX amplitudes (e = tid + k NT) in
+// registers for the whole run and multiplies them by every gate's phase in order -- the
+// multiply of diag_chunk, in the same order.
 template <typename T, int NT, int KMAX>
 __device__ __forceinline__ void apply_diag_run(T* sr, T* si, uint32_t N, uint64_t base,
-                                               const uint32_t* ts, uint32_t tb,
                                                const TileOp& op,
                                                const TileDiag* __restrict__ diags,
                                                const T* __restrict__ pool) {
   T xr[KMAX], xi[KMAX];
-  uint64_t f[KMAX];
-#pragma unroll
-  for (int k = 0; k < KMAX; ++k) {
-    const uint32_t e = threadIdx.x + uint32_t(k) * NT;
+  static_for<0, KMAX>([&](auto kc) {
+    const uint32_t e = threadIdx.x + uint32_t(kc.value) * NT;
     if (e < N) {
-      xr[k] = sr[e];
-      xi[k] = si[e];
-      f[k] = base | offset(e, ts, tb);
+      xr[kc.value] = sr[e];
+      xi[kc.value] = si[e];
     }
-  }
+  });
   for (uint32_t d = 0; d < op.n_diag; ++d) {
-    // fields read one by one (no local copy of the descriptor: its qubit list would be
-    // indexed dynamically and land in private memory)
     const TileDiag* dg = diags + op.off + d;
-    const uint32_t m = dg->m;
-    const T* tab = pool + 2 * dg->tab;
-    if (m <= 2) {
-      const uint32_t q0 = dg->q[0], q1 = m == 2 ? dg->q[1] : 0;
-      const T t0r = tab[0], t0i = tab[1], t1r = tab[2], t1i = tab[3];
-      const T t2r = m == 2 ? tab[4] : T(0), t2i = m == 2 ? tab[5] : T(0);
-      const T t3r = m == 2 ? tab[6] : T(0), t3i = m == 2 ? tab[7] : T(0);
-#pragma unroll
-      for (int k = 0; k < KMAX; ++k) {
-        if (threadIdx.x + uint32_t(k) * NT < N) {
-          const bool b0 = (f[k] >> q0) & 1u;
-          const bool b1 = m == 2 && ((f[k] >> q1) & 1u);
-          const T dr = b1 ? (b0 ? t3r : t2r) : (b0 ? t1r : t0r);
-          const T di = b1 ? (b0 ? t3i : t2i) : (b0 ? t1i : t0i);
-          cmul(xr[k], xi[k], dr, di);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < KMAX; ++k) {
-        if (threadIdx.x + uint32_t(k) * NT < N) {
-          uint32_t s = 0;
-          for (uint32_t j = 0; j < m; ++j) s |= uint32_t((f[k] >> dg->q[j]) & 1u) << j;
-          cmul(xr[k], xi[k], tab[2 * s], tab[2 * s + 1]);
-        }
-      }
-    }
+    if (dg->m <= 2)
+      diag_gate_small<T, NT, KMAX>(xr, xi, N, base, dg, pool);
+    else
+      diag_gate_wide<T, NT, KMAX>(xr, xi, N, base, dg, pool);
   }
-#pragma unroll
-  for (int k = 0; k < KMAX; ++k) {
-    const uint32_t e = threadIdx.x + uint32_t(k) * NT;
+  static_for<0, KMAX>([&](auto kc) {
+    const uint32_t e = threadIdx.x + uint32_t(kc.value) * NT;
     if (e < N) {
-      sr[e] = xr[k];
-      si[e] = xi[k];
+      sr[e] = xr[kc.value];
+      si[e] = xi[kc.value];
     }
-  }
+  });
 }
 
 // Interpret ops[0..n_ops) on the tile (LDS); ends with a workgroup barrier.
 template <typename T, int NT, int KMAX>
 __device__ __forceinline__ void run_ops(T* sr, T* si, uint32_t N, uint64_t base,
-                                        const uint32_t* ts, uint32_t tb,
                                         const TileOp* __restrict__ ops, uint32_t n_ops,
                                         const TileDiag* __restrict__ diags,
                                         const T* __restrict__ pool) {
@@ -181,7 +302,7 @@ __device__ __forceinline__ void run_ops(T* sr, T* si, uint32_t N, uint64_t base,
         si[i1] = ci + di;
       }
     } else if (op.kind == 1) {
-      apply_diag_run<T, NT, KMAX>(sr, si, N, base, ts, tb, op, diags, pool);
+      apply_diag_run<T, NT, KMAX>(sr, si, N, base, op, diags, pool);
     } else if (op.kind == 2) {
       apply_mq<T, 2, NT>(sr, si, N, op, pool);
     } else {
