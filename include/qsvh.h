@@ -9,9 +9,11 @@
  *
  * Conventions (crates/qsv-core/src/gate.rs:8-12): a gate acting on the ordered qubit
  * list qs[0..m) is a row-major 2^m x 2^m complex matrix whose internal index has bit j
- * equal to global qubit qs[j]; controlled gates list the control first.  Amplitudes are
- * Structure-of-Arrays (crates/qsv-core/src/state/mod.rs:25-29): re[2^n], im[2^n] of f64
- * (QSVH_C128) or f32 (QSVH_C64).  Matrices always cross the ABI in f64 and are rounded to
+ * equal to global qubit qs[j]; controlled gates list the control first.  Amplitudes cross
+ * the ABI (download, upload) as Structure-of-Arrays like the reference's StateVector
+ * (crates/qsv-core/src/state/mod.rs:25-29): re[2^n], im[2^n] of f64 (QSVH_C128) or f32
+ * (QSVH_C64); in HBM the engine keeps them interleaved (re, im pairs), which halves the
+ * address streams of an in-place gate.  Matrices always cross the ABI in f64 and are rounded to
  * the state's precision on the device side, as `DenseGate<R>` is built with
  * `Cplx::from_f64` (crates/qsv-core/src/complex.rs:43-50).
  *
@@ -86,9 +88,9 @@ int qsvh_abi_version(void);
  * "max_blocks" (0: one thread per work item; n > 0: grid-stride loop over n blocks),
  * "unroll" (1, 2 or 4 independent 16-byte items per thread per iteration) and
  * "nontemporal" (0/1: streaming load/store hints), "contig" (0: a thread's unrolled items
- * are a grid apart; 1: adjacent); "mfma_blocks" and "tile_blocks" (workgroups of the
- * persistent matrix-core group and tile-pass kernels); "im_pad" (bytes, a multiple of 256
- * up to 64 MiB, placed between the re and im halves of states allocated afterwards).
+ * are a grid apart; 1: adjacent), "xcd" (1: consecutive blocks of a streaming kernel take
+ * neighbouring address ranges within one XCD; 0: launch order); "mfma_blocks" and
+ * "tile_blocks" (workgroups of the persistent matrix-core group and tile-pass kernels).
  * No reference counterpart: the
  * reference's kernels have a fixed 256-thread, one-item-per-thread shape
  * (crates/qsv-cuda/src/lib.rs:32,80-87). */

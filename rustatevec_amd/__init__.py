@@ -358,7 +358,7 @@ class Program:
 
 
 TUNE_KEYS = ("max_blocks", "unroll", "nontemporal", "mfma_blocks", "tile_blocks", "contig",
-             "im_pad")
+             "xcd")
 
 
 class HipBackend:
@@ -385,9 +385,7 @@ class HipBackend:
         return (lib().qsvh_last_error(self.ctx) or b"").decode()
 
     def tune(self, **knobs):
-        """Set launch-shape knobs (see TUNE_KEYS); returns all values.
-
-        im_pad applies to states allocated afterwards."""
+        """Set launch-shape knobs (see TUNE_KEYS); returns all values."""
         for k, v in knobs.items():
             _check(lib().qsvh_tune_set(self.ctx, k.encode(), int(v)), self.ctx)
         out = {}

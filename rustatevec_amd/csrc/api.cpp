@@ -184,6 +184,56 @@ void program_free(Program* p) {
   p->arena = nullptr;
 }
 
+
+// Host SoA <-> device interleaved, through a host staging buffer of at most 2^22
+// amplitudes per round.
+template <typename T>
+static hipError_t split_t(const State& s, T* re, T* im) {
+  const uint64_t dim = s.dim();
+  const uint64_t chunk = std::min<uint64_t>(dim, uint64_t(1) << 22);
+  std::vector<T> buf(2 * chunk);
+  for (uint64_t a = 0; a < dim; a += chunk) {
+    const uint64_t c = std::min(chunk, dim - a);
+    const hipError_t e = hipMemcpy(buf.data(), static_cast<const T*>(s.amp) + 2 * a,
+                                   2 * c * sizeof(T), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (uint64_t i = 0; i < c; ++i) {
+      re[a + i] = buf[2 * i];
+      im[a + i] = buf[2 * i + 1];
+    }
+  }
+  return hipSuccess;
+}
+
+template <typename T>
+static hipError_t join_t(const State& s, const T* re, const T* im) {
+  const uint64_t dim = s.dim();
+  const uint64_t chunk = std::min<uint64_t>(dim, uint64_t(1) << 22);
+  std::vector<T> buf(2 * chunk);
+  for (uint64_t a = 0; a < dim; a += chunk) {
+    const uint64_t c = std::min(chunk, dim - a);
+    for (uint64_t i = 0; i < c; ++i) {
+      buf[2 * i] = re[a + i];
+      buf[2 * i + 1] = im[a + i];
+    }
+    const hipError_t e = hipMemcpy(static_cast<T*>(s.amp) + 2 * a, buf.data(),
+                                   2 * c * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t download_split(const State& s, void* re, void* im) {
+  return s.dtype == 0 ? split_t(s, static_cast<double*>(re), static_cast<double*>(im))
+                      : split_t(s, static_cast<float*>(re), static_cast<float*>(im));
+}
+
+hipError_t upload_join(const State& s, const void* re, const void* im) {
+  return s.dtype == 0
+             ? join_t(s, static_cast<const double*>(re), static_cast<const double*>(im))
+             : join_t(s, static_cast<const float*>(re), static_cast<const float*>(im));
+}
+
 }  // namespace qsvh
 
 using namespace qsvh;
@@ -254,8 +304,8 @@ int qsvh_tune_set(qsvh_ctx* h, const char* key, int64_t value) {
     t.tile_blocks = int(value);
   } else if (k == "contig" && (value == 0 || value == 1)) {
     t.contig = int(value);
-  } else if (k == "im_pad" && value >= 0 && value <= (64ll << 20) && value % 256 == 0) {
-    t.im_pad = value;
+  } else if (k == "xcd" && (value == 0 || value == 1)) {
+    t.xcd = int(value);
   } else {
     return set_err(&h->c, kInvalidArg, "unknown tuning key or value: " + k);
   }
@@ -272,7 +322,7 @@ int qsvh_tune_get(qsvh_ctx* h, const char* key, int64_t* value) {
   else if (k == "mfma_blocks") *value = t.mfma_blocks;
   else if (k == "tile_blocks") *value = t.tile_blocks;
   else if (k == "contig") *value = t.contig;
-  else if (k == "im_pad") *value = t.im_pad;
+  else if (k == "xcd") *value = t.xcd;
   else return set_err(&h->c, kInvalidArg, "unknown tuning key: " + k);
   return kOk;
 }
@@ -482,18 +532,12 @@ int qsvh_state_alloc(qsvh_ctx* h, uint32_t n, int dtype, qsvh_state** out) {
   s.ctx = &h->c;
   s.n = n;
   s.dtype = dtype;
-  // One allocation: re, then `im_pad` bytes, then im (the pad moves im[i] off the HBM
-  // channel/bank that re[i] maps to when the halves are a power of two apart).
-  const size_t bytes = size_t(s.dim()) * s.elem();
-  const size_t pad = size_t(h->c.tune.im_pad);
   hipError_t e = hipSetDevice(h->c.device);
-  if (e == hipSuccess) e = hipMalloc(&s.block, 2 * bytes + pad);
+  if (e == hipSuccess) e = hipMalloc(&s.amp, s.bytes());
   if (e != hipSuccess) {
     delete st;
     return hip_err(&h->c, e, "state alloc");
   }
-  s.re = s.block;
-  s.im = static_cast<char*>(s.block) + bytes + pad;
   *out = st;
   return kOk;
 }
@@ -501,7 +545,7 @@ int qsvh_state_alloc(qsvh_ctx* h, uint32_t n, int dtype, qsvh_state** out) {
 void qsvh_state_free(qsvh_state* st) {
   if (!st) return;
   hipStreamSynchronize(st->s.ctx->stream);
-  hipFree(st->s.block);
+  hipFree(st->s.amp);
   delete st;
 }
 
@@ -539,23 +583,20 @@ int qsvh_amplitude(qsvh_state* st, uint64_t index, double* re, double* im) {
   State& s = st->s;
   if (index >= s.dim()) return set_err(s.ctx, kInvalidArg, "amplitude index out of range");
   const size_t es = s.elem();
-  alignas(8) unsigned char br[8], bi[8];
+  alignas(16) unsigned char b[16];
   hipStream_t str = s.ctx->stream;
-  hipError_t e = hipMemcpyAsync(br, static_cast<char*>(s.re) + index * es, es,
+  hipError_t e = hipMemcpyAsync(b, static_cast<char*>(s.amp) + 2 * index * es, 2 * es,
                                 hipMemcpyDeviceToHost, str);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(bi, static_cast<char*>(s.im) + index * es, es, hipMemcpyDeviceToHost, str);
   if (e == hipSuccess) e = hipStreamSynchronize(str);
   if (e != hipSuccess) return hip_err(s.ctx, e, "amplitude");
   if (s.dtype == 0) {
-    std::memcpy(re, br, 8);
-    std::memcpy(im, bi, 8);
+    std::memcpy(re, b, 8);
+    std::memcpy(im, b + 8, 8);
   } else {
-    float fr, fi;
-    std::memcpy(&fr, br, 4);
-    std::memcpy(&fi, bi, 4);
-    *re = fr;
-    *im = fi;
+    float f[2];
+    std::memcpy(f, b, 8);
+    *re = f[0];
+    *im = f[1];
   }
   return kOk;
 }
@@ -603,20 +644,16 @@ int qsvh_expectation_z(qsvh_state* st, uint64_t mask, double* out) {
 int qsvh_download(qsvh_state* st, void* re, void* im) {
   if (!st || !re || !im) return kInvalidArg;
   State& s = st->s;
-  const size_t bytes = size_t(s.dim()) * s.elem();
   hipError_t e = hipStreamSynchronize(s.ctx->stream);
-  if (e == hipSuccess) e = hipMemcpy(re, s.re, bytes, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(im, s.im, bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = download_split(s, re, im);
   return hip_err(s.ctx, e, "download");
 }
 
 int qsvh_upload(qsvh_state* st, const void* re, const void* im) {
   if (!st || !re || !im) return kInvalidArg;
   State& s = st->s;
-  const size_t bytes = size_t(s.dim()) * s.elem();
   hipError_t e = hipStreamSynchronize(s.ctx->stream);
-  if (e == hipSuccess) e = hipMemcpy(s.re, re, bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(s.im, im, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = upload_join(s, re, im);
   return hip_err(s.ctx, e, "upload");
 }
 
@@ -783,18 +820,13 @@ int qsvh_time_gate(qsvh_state* st, uint32_t m, const uint32_t* qubits, const dou
 int qsvh_time_copy(qsvh_state* st, uint32_t reps, double* ms) {
   if (!st || !ms || reps == 0) return kInvalidArg;
   State& s = st->s;
-  const size_t bytes = size_t(s.dim()) * s.elem();
-  // The destination uses the same layout as a state allocation (re, pad, im).
-  const size_t pad = size_t(s.ctx->tune.im_pad);
-  void* blk = nullptr;
-  hipError_t e = hipMalloc(&blk, 2 * bytes + pad);
+  void* dst = nullptr;
+  hipError_t e = hipMalloc(&dst, s.bytes());
   if (e != hipSuccess) return hip_err(s.ctx, e, "copy alloc");
-  void* dr = blk;
-  void* di = static_cast<char*>(blk) + bytes + pad;
-  const int rc =
-      time_launches(s.ctx, reps, ms, [&]() { return launch_copy(s, dr, di, s.ctx->tune, s.ctx->stream); });
+  const int rc = time_launches(s.ctx, reps, ms,
+                               [&]() { return launch_copy(s, dst, s.ctx->tune, s.ctx->stream); });
   hipStreamSynchronize(s.ctx->stream);
-  hipFree(blk);
+  hipFree(dst);
   return rc;
 }
 

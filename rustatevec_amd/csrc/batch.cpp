@@ -24,10 +24,9 @@ struct Batch {
   Ctx* ctx = nullptr;
   int dtype = 1;
   std::vector<Circuit> circuits;
-  std::vector<uint64_t> off;  // state offsets (elements)
+  std::vector<uint64_t> off;  // state offsets (amplitudes)
   uint64_t total = 0;
-  void* re = nullptr;
-  void* im = nullptr;
+  void* amp = nullptr;        // every circuit's interleaved state, back to back
   // plan
   int tb = 0;
   int blocks = 0;
@@ -52,8 +51,7 @@ struct Batch {
     s.ctx = ctx;
     s.n = circuits[i].n;
     s.dtype = dtype;
-    s.re = static_cast<char*>(re) + off[i] * elem();
-    s.im = static_cast<char*>(im) + off[i] * elem();
+    s.amp = static_cast<char*>(amp) + off[i] * 2 * elem();
     return s;
   }
   void free_plan() {
@@ -187,8 +185,8 @@ hipError_t issue(Batch* b) {
     uint64_t work = 0;
     for (const BatchPass& p : L) work += p.n_tiles;
     const hipError_t e = launch_batch_pass(b->dtype, b->d_passes + b->launch_first[j],
-                                           uint32_t(L.size()), work, b->d_counter, b->re,
-                                           b->im, b->d_ops, b->d_diags, b->d_pool,
+                                           uint32_t(L.size()), work, b->d_counter, b->amp,
+                                           b->d_ops, b->d_diags, b->d_pool,
                                            b->d_partials, b->blocks, s);
     if (e != hipSuccess) return e;
   }
@@ -265,13 +263,11 @@ int qsvh_batch_create(qsvh_ctx* h, const qsvh_circuit* const* circuits, size_t c
     b.off.push_back(b.total);
     b.total += uint64_t(1) << circuits[i]->c.n;
   }
-  hipError_t e = hipMalloc(&b.re, b.total * b.elem());
-  if (e == hipSuccess) e = hipMalloc(&b.im, b.total * b.elem());
+  hipError_t e = hipMalloc(&b.amp, b.total * 2 * b.elem());
   if (e == hipSuccess) e = hipMalloc(&b.d_out, count * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b.d_counter), 64);
   if (e != hipSuccess) {
-    hipFree(b.re);
-    hipFree(b.im);
+    hipFree(b.amp);
     hipFree(b.d_out);
     hipFree(b.d_counter);
     delete bh;
@@ -291,8 +287,7 @@ void qsvh_batch_destroy(qsvh_batch* bh) {
   Batch& b = bh->b;
   hipStreamSynchronize(b.ctx->stream);
   b.free_plan();
-  hipFree(b.re);
-  hipFree(b.im);
+  hipFree(b.amp);
   hipFree(b.d_out);
   hipFree(b.d_counter);
   delete bh;
@@ -385,10 +380,8 @@ int qsvh_batch_download(qsvh_batch* bh, size_t i, void* re, void* im) {
   if (!bh || !re || !im || i >= bh->b.circuits.size()) return kInvalidArg;
   Batch& b = bh->b;
   const State s = b.view(i);
-  const size_t bytes = size_t(s.dim()) * s.elem();
   hipError_t e = hipStreamSynchronize(b.ctx->stream);
-  if (e == hipSuccess) e = hipMemcpy(re, s.re, bytes, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(im, s.im, bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = download_split(s, re, im);
   return hip_err(b.ctx, e, "batch download");
 }
 

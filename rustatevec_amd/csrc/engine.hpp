@@ -21,10 +21,9 @@ struct Tuning {
   int mfma_blocks = 2048;      // workgroups of the persistent MFMA group kernels
   int tile_blocks = 1024;      // workgroups of the persistent tile-pass kernel
   int contig = 0;              // 1: a thread's unrolled items are adjacent (else strided)
-  // Bytes between the re and im halves of a state allocation.  Halves exactly a power of
-  // two apart map their same-index lines onto the same HBM channels; a 4 KiB shift
-  // measured +8% (c64) and up to +4% (c128) on 30-qubit passes.
-  int64_t im_pad = 4096;
+  // 1: blocks b and b + 1 of a streaming launch take neighbouring address ranges of one
+  // XCD (the hardware deals blocks to the 8 XCDs round robin, b -> XCD b % 8).
+  int xcd = 1;
 };
 
 struct Ctx {
@@ -40,15 +39,16 @@ struct Ctx {
   static constexpr int kMaxPartials = 4096;
 };
 
+// A state vector in HBM: 2^n complex amplitudes, interleaved (re, im) pairs of T --
+// amplitude i at amp[2 i], amp[2 i + 1].
 struct State {
   Ctx* ctx = nullptr;
   uint32_t n = 0;
-  int dtype = 0;  // 0 = c128 (f64 SoA), 1 = c64 (f32 SoA)
-  void* re = nullptr;
-  void* im = nullptr;
-  void* block = nullptr;  // the single allocation holding re, pad, im
+  int dtype = 0;  // 0 = c128 (f64 pairs), 1 = c64 (f32 pairs)
+  void* amp = nullptr;
   uint64_t dim() const { return 1ull << n; }
-  size_t elem() const { return dtype == 0 ? sizeof(double) : sizeof(float); }
+  size_t elem() const { return dtype == 0 ? sizeof(double) : sizeof(float); }  // one part
+  size_t bytes() const { return size_t(dim()) * 2 * elem(); }
 };
 
 // ---- gate descriptors passed by value to the kernels -------------------------------
@@ -86,8 +86,11 @@ hipError_t launch_probs(const State& st, void* out, hipStream_t s);
 // f64 sum of |psi_i|^2 * sign(i & mask) (mask 0: the norm) into *d_result.
 hipError_t launch_reduce(const State& st, uint64_t mask, double* d_partials, int max_partials,
                          double* d_result, hipStream_t s);
-hipError_t launch_copy(const State& src, void* dst_re, void* dst_im, const Tuning& t,
-                       hipStream_t s);
+hipError_t launch_copy(const State& src, void* dst, const Tuning& t, hipStream_t s);
+// Synchronous copies between a state and host SoA arrays re[dim], im[dim] of the state's
+// precision (api.cpp).
+hipError_t download_split(const State& s, void* re, void* im);
+hipError_t upload_join(const State& s, const void* re, const void* im);
 
 // ---- kernels_mfma.hip ------------------------------------------------------------------
 // Fused k-qubit group (k = 4..6, qubits `q` ascending) on a complex64 state of >= 11
@@ -146,7 +149,7 @@ constexpr uint32_t kBatchExpZ0 = 2;   // write the tile's partial <Z_0> to parti
 // One pass of one circuit of a batch: its tiles are work items
 // [tile_begin, tile_begin + n_tiles) of the launch.
 struct BatchPass {
-  uint64_t state_off;  // element offset of the circuit's state in the pool
+  uint64_t state_off;  // amplitude offset of the circuit's state in the pool
   uint64_t tile_begin;
   uint64_t n_tiles;
   uint32_t n, tb;
@@ -156,8 +159,8 @@ struct BatchPass {
   uint32_t part_off;  // first partial slot (kBatchExpZ0)
 };
 hipError_t launch_batch_pass(int dtype, const BatchPass* passes, uint32_t n_passes,
-                             uint64_t total_work, unsigned long long* counter, void* re_pool,
-                             void* im_pool, const TileOp* ops, const TileDiag* diags,
+                             uint64_t total_work, unsigned long long* counter, void* amp_pool,
+                             const TileOp* ops, const TileDiag* diags,
                              const void* pool, double* partials, int blocks, hipStream_t s);
 hipError_t launch_batch_reduce(const uint32_t* part_off, const uint32_t* n_part,
                                uint32_t n_circ, const double* partials, double* out,

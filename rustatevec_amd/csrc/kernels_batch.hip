@@ -10,6 +10,7 @@
 // a final kernel adds up per circuit in a fixed order.
 #include <hip/hip_runtime.h>
 
+#include "amp.hpp"
 #include "engine.hpp"
 #include "tile_ops.hpp"
 
@@ -28,18 +29,16 @@ template <typename T, int MAXTB>
 __global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict__ passes,
                                                     uint32_t n_passes, uint64_t total_work,
                                                     unsigned long long* __restrict__ counter,
-                                                    T* __restrict__ re_pool,
-                                                    T* __restrict__ im_pool,
+                                                    T* __restrict__ amp_pool,
                                                     const TileOp* __restrict__ ops,
                                                     const TileDiag* __restrict__ diags,
                                                     const T* __restrict__ pool,
                                                     double* __restrict__ partials) {
   constexpr uint32_t NMAX = 1u << MAXTB;
-  constexpr int W = 16 / int(sizeof(T));
+  constexpr int A = amps_per_chunk<T>();  // batch circuits have >= 4 qubits: a chunk fits
   __shared__ T sr[NMAX], si[NMAX];
   __shared__ unsigned long long w_sh;
   __shared__ double red[kThreads / 64];
-  using V = T __attribute__((ext_vector_type(W)));
 
   for (;;) {
     if (threadIdx.x == 0) w_sh = atomicAdd(counter, 1ull);
@@ -62,25 +61,23 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict_
     const uint32_t* ts = bp->ts;  // uniform: read through the scalar cache
     const uint64_t t = w - bp->tile_begin;
     const uint64_t base = tile::insert_zero_bits(t, ts, tb);
-    T* re = re_pool + bp->state_off;
-    T* im = im_pool + bp->state_off;
+    T* amp = amp_pool + 2 * bp->state_off;
     const uint32_t flags = bp->flags;
-    for (uint32_t e = threadIdx.x * W; e < N; e += kThreads * W) {
+    for (uint32_t e = threadIdx.x * A; e < N; e += kThreads * A) {
       if (flags & kBatchInit) {
         // |0...0>: amplitude 1 at state index 0, which is tile 0, local index 0
 #pragma unroll
-        for (int v = 0; v < W; ++v) {
+        for (int v = 0; v < A; ++v) {
           sr[e + v] = (t == 0 && e + v == 0) ? T(1) : T(0);
           si[e + v] = T(0);
         }
       } else {
-        const uint64_t a = base + tile::offset(e, ts, tb);
-        const V vr = *reinterpret_cast<const V*>(re + a);
-        const V vi = *reinterpret_cast<const V*>(im + a);
+        T r[A], m[A];
+        cload<T, A>(amp, base + tile::offset(e, ts, tb), r, m);
 #pragma unroll
-        for (int v = 0; v < W; ++v) {
-          sr[e + v] = vr[v];
-          si[e + v] = vi[v];
+        for (int v = 0; v < A; ++v) {
+          sr[e + v] = r[v];
+          si[e + v] = m[v];
         }
       }
     }
@@ -103,16 +100,14 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict_
         partials[bp->part_off + t] = s;
       }
     }
-    for (uint32_t e = threadIdx.x * W; e < N; e += kThreads * W) {
-      const uint64_t a = base + tile::offset(e, ts, tb);
-      V vr, vi;
+    for (uint32_t e = threadIdx.x * A; e < N; e += kThreads * A) {
+      T r[A], m[A];
 #pragma unroll
-      for (int v = 0; v < W; ++v) {
-        vr[v] = sr[e + v];
-        vi[v] = si[e + v];
+      for (int v = 0; v < A; ++v) {
+        r[v] = sr[e + v];
+        m[v] = si[e + v];
       }
-      *reinterpret_cast<V*>(re + a) = vr;
-      *reinterpret_cast<V*>(im + a) = vi;
+      cstore<T, A>(amp, base + tile::offset(e, ts, tb), r, m);
     }
     __syncthreads();
   }
@@ -136,20 +131,20 @@ __global__ void k_batch_reduce(const uint32_t* __restrict__ part_off,
 }  // namespace
 
 hipError_t launch_batch_pass(int dtype, const BatchPass* passes, uint32_t n_passes,
-                             uint64_t total_work, unsigned long long* counter, void* re_pool,
-                             void* im_pool, const TileOp* ops, const TileDiag* diags,
+                             uint64_t total_work, unsigned long long* counter, void* amp_pool,
+                             const TileOp* ops, const TileDiag* diags,
                              const void* pool, double* partials, int blocks, hipStream_t s) {
   k_reset_counter<<<1, 64, 0, s>>>(counter);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (dtype == 0) {
     k_batch<double, 12><<<blocks, kThreads, 0, s>>>(
-        passes, n_passes, total_work, counter, static_cast<double*>(re_pool),
-        static_cast<double*>(im_pool), ops, diags, static_cast<const double*>(pool), partials);
+        passes, n_passes, total_work, counter, static_cast<double*>(amp_pool), ops, diags,
+        static_cast<const double*>(pool), partials);
   } else {
     k_batch<float, 13><<<blocks, kThreads, 0, s>>>(
-        passes, n_passes, total_work, counter, static_cast<float*>(re_pool),
-        static_cast<float*>(im_pool), ops, diags, static_cast<const float*>(pool), partials);
+        passes, n_passes, total_work, counter, static_cast<float*>(amp_pool), ops, diags,
+        static_cast<const float*>(pool), partials);
   }
   return hipGetLastError();
 }

@@ -73,8 +73,9 @@ __device__ __forceinline__ void tile_rc(uint32_t tl, const TileMap& m, uint32_t&
   }
 }
 
-// Each lane moves two float4 chunks of re and of im per tile (2048 / 4 / 256 = 2).
-constexpr int kChunks = (1 << kTileBits) / 4 / kBlock;
+// Each lane moves four 16-byte chunks of two interleaved amplitudes per tile
+// (2048 / 2 / 256 = 4); tile-local bit 0 is state bit 0, so a chunk is consecutive.
+constexpr int kChunks = (1 << kTileBits) / 2 / kBlock;
 
 // ------------------------------------------------------------------------- FP32
 // LDS layout [row][col], row stride chosen = 32 (mod 64) floats so that the two half-waves
@@ -88,8 +89,7 @@ struct F32Shape {
 };
 
 template <int K>
-__global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ re,
-                                                      float* __restrict__ im,
+__global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
                                                       uint64_t n_tiles, TileMap map,
                                                       const float* __restrict__ ablock) {
   using S = F32Shape<K>;
@@ -108,27 +108,26 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ re,
 
   // Per-lane chunk offsets and LDS positions (the same for every tile).
   uint64_t off[kChunks];
-  uint32_t pos[kChunks][4];
+  uint32_t pos[kChunks][2];
 #pragma unroll
   for (int c = 0; c < kChunks; ++c) {
-    const uint32_t tl = 4u * (threadIdx.x + kBlock * c);
+    const uint32_t tl = 2u * (threadIdx.x + kBlock * c);
     off[c] = tile_offset(tl, map);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < 2; ++e) {
       uint32_t s, n;
       tile_rc(tl + e, map, s, n);
       pos[c][e] = s * STRIDE + n;
     }
   }
 
-  floatx4 vr[kChunks], vi[kChunks];
+  floatx4 v[kChunks];
   uint64_t t = blockIdx.x;
   if (t < n_tiles) {
     const uint64_t base = tile_base(t, map);
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
-      vr[c] = *reinterpret_cast<const floatx4*>(re + base + off[c]);
-      vi[c] = *reinterpret_cast<const floatx4*>(im + base + off[c]);
+      v[c] = *reinterpret_cast<const floatx4*>(amp + 2 * (base + off[c]));
     }
   }
   for (; t < n_tiles; t += gridDim.x) {
@@ -137,9 +136,9 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ re,
 #pragma unroll
     for (int c = 0; c < kChunks; ++c)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        xr[pos[c][e]] = vr[c][e];
-        xi[pos[c][e]] = vi[c][e];
+      for (int e = 0; e < 2; ++e) {
+        xr[pos[c][e]] = v[c][2 * e];
+        xi[pos[c][e]] = v[c][2 * e + 1];
       }
     __syncthreads();
     const uint64_t tn = t + gridDim.x;
@@ -147,8 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ re,
       const uint64_t nb = tile_base(tn, map);
 #pragma unroll
       for (int c = 0; c < kChunks; ++c) {
-        vr[c] = *reinterpret_cast<const floatx4*>(re + nb + off[c]);
-        vi[c] = *reinterpret_cast<const floatx4*>(im + nb + off[c]);
+        v[c] = *reinterpret_cast<const floatx4*>(amp + 2 * (nb + off[c]));
       }
     }
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
@@ -172,14 +170,13 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ re,
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
-      floatx4 yr, yi;
+      floatx4 y;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        yr[e] = xr[pos[c][e]];
-        yi[e] = xi[pos[c][e]];
+      for (int e = 0; e < 2; ++e) {
+        y[2 * e] = xr[pos[c][e]];
+        y[2 * e + 1] = xi[pos[c][e]];
       }
-      *reinterpret_cast<floatx4*>(re + base + off[c]) = yr;
-      *reinterpret_cast<floatx4*>(im + base + off[c]) = yi;
+      *reinterpret_cast<floatx4*>(amp + 2 * (base + off[c])) = y;
     }
   }
 }
@@ -203,8 +200,7 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
 }
 
 template <int K>
-__global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ re,
-                                                         float* __restrict__ im,
+__global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp,
                                                          uint64_t n_tiles, TileMap map,
                                                          const __bf16* __restrict__ ahi,
                                                          const __bf16* __restrict__ alo) {
@@ -228,27 +224,26 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ re,
   }
 
   uint64_t off[kChunks];
-  uint32_t pos[kChunks][4];
+  uint32_t pos[kChunks][2];
 #pragma unroll
   for (int c = 0; c < kChunks; ++c) {
-    const uint32_t tl = 4u * (threadIdx.x + kBlock * c);
+    const uint32_t tl = 2u * (threadIdx.x + kBlock * c);
     off[c] = tile_offset(tl, map);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < 2; ++e) {
       uint32_t s, n;
       tile_rc(tl + e, map, s, n);
       pos[c][e] = n * CS + s;  // re at +0, im at +D
     }
   }
 
-  floatx4 vr[kChunks], vi[kChunks];
+  floatx4 v[kChunks];
   uint64_t t = blockIdx.x;
   if (t < n_tiles) {
     const uint64_t base = tile_base(t, map);
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
-      vr[c] = *reinterpret_cast<const floatx4*>(re + base + off[c]);
-      vi[c] = *reinterpret_cast<const floatx4*>(im + base + off[c]);
+      v[c] = *reinterpret_cast<const floatx4*>(amp + 2 * (base + off[c]));
     }
   }
   for (; t < n_tiles; t += gridDim.x) {
@@ -257,9 +252,9 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ re,
 #pragma unroll
     for (int c = 0; c < kChunks; ++c)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        xb[pos[c][e]] = vr[c][e];
-        xb[pos[c][e] + D] = vi[c][e];
+      for (int e = 0; e < 2; ++e) {
+        xb[pos[c][e]] = v[c][2 * e];
+        xb[pos[c][e] + D] = v[c][2 * e + 1];
       }
     __syncthreads();
     const uint64_t tn = t + gridDim.x;
@@ -267,8 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ re,
       const uint64_t nb = tile_base(tn, map);
 #pragma unroll
       for (int c = 0; c < kChunks; ++c) {
-        vr[c] = *reinterpret_cast<const floatx4*>(re + nb + off[c]);
-        vi[c] = *reinterpret_cast<const floatx4*>(im + nb + off[c]);
+        v[c] = *reinterpret_cast<const floatx4*>(amp + 2 * (nb + off[c]));
       }
     }
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
@@ -300,14 +294,13 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ re,
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
-      floatx4 yr, yi;
+      floatx4 y;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        yr[e] = xb[pos[c][e]];
-        yi[e] = xb[pos[c][e] + D];
+      for (int e = 0; e < 2; ++e) {
+        y[2 * e] = xb[pos[c][e]];
+        y[2 * e + 1] = xb[pos[c][e] + D];
       }
-      *reinterpret_cast<floatx4*>(re + base + off[c]) = yr;
-      *reinterpret_cast<floatx4*>(im + base + off[c]) = yi;
+      *reinterpret_cast<floatx4*>(amp + 2 * (base + off[c])) = y;
     }
   }
 }
@@ -318,15 +311,14 @@ hipError_t launch_k(const State& st, const TileMap& map, const void* mats, int p
   const uint64_t n_tiles = st.dim() >> kTileBits;
   uint64_t g = n_tiles;
   if (g > uint64_t(max_blocks)) g = max_blocks;
-  float* re = static_cast<float*>(st.re);
-  float* im = static_cast<float*>(st.im);
+  float* amp = static_cast<float*>(st.amp);
   if (precision == 1) {
-    k_fused_f32<K><<<unsigned(g), kBlock, 0, s>>>(re, im, n_tiles, map,
+    k_fused_f32<K><<<unsigned(g), kBlock, 0, s>>>(amp, n_tiles, map,
                                                   static_cast<const float*>(mats));
   } else {
     constexpr size_t n = size_t(4) << (2 * K);  // (2D)^2 entries
     const __bf16* hi = static_cast<const __bf16*>(mats);
-    k_fused_bf16x3<K><<<unsigned(g), kBlock, 0, s>>>(re, im, n_tiles, map, hi, hi + n);
+    k_fused_bf16x3<K><<<unsigned(g), kBlock, 0, s>>>(amp, n_tiles, map, hi, hi + n);
   }
   return hipGetLastError();
 }

@@ -1,8 +1,9 @@
 // kernels_plain.hip -- the plain (unfused, exact-precision) gate kernels for gfx950.
 //
-// Amplitudes are SoA: re[2^n], im[2^n] of T (double for complex128, float for complex64).
-// Every streaming kernel moves 16-byte vectors (W = 2 doubles or 4 floats per access),
-// so a wave64 load instruction covers 1 KiB of contiguous HBM for each component array.
+// Amplitudes are interleaved complex pairs of T (double for complex128, float for
+// complex64; amp.hpp).  Every streaming kernel moves 16-byte chunks (A = 1 complex128 or
+// 2 complex64 amplitudes per access), so a wave64 load instruction covers 1 KiB of
+// contiguous HBM, and orders its blocks XCD-contiguously (Tuning::xcd).
 //
 // Arithmetic is the reference's, operation for operation and without contraction (this
 // file is compiled with -ffp-contract=off): complex products as (ac - bd, ad + bc), sums
@@ -12,51 +13,13 @@
 
 #include <type_traits>
 
+#include "amp.hpp"
 #include "engine.hpp"
 
 namespace qsvh {
 namespace {
 
 constexpr int kBlock = 256;
-
-template <typename T, int W>
-struct Vec {
-  using type = T __attribute__((ext_vector_type(W)));
-};
-
-// 16-byte (or scalar) loads and stores; NT marks them non-temporal (streamed past the
-// caches), one of the launch-shape knobs.
-template <typename T, int W, bool NT = false>
-__device__ __forceinline__ void vload(const T* p, T (&x)[W]) {
-  if constexpr (W == 1) {
-    x[0] = NT ? __builtin_nontemporal_load(p) : p[0];
-  } else {
-    using V = typename Vec<T, W>::type;
-    const V* q = reinterpret_cast<const V*>(p);
-    const V v = NT ? __builtin_nontemporal_load(q) : *q;
-#pragma unroll
-    for (int j = 0; j < W; ++j) x[j] = v[j];
-  }
-}
-
-template <typename T, int W, bool NT = false>
-__device__ __forceinline__ void vstore(T* p, const T (&x)[W]) {
-  if constexpr (W == 1) {
-    if constexpr (NT)
-      __builtin_nontemporal_store(x[0], p);
-    else
-      p[0] = x[0];
-  } else {
-    using V = typename Vec<T, W>::type;
-    V v;
-#pragma unroll
-    for (int j = 0; j < W; ++j) v[j] = x[j];
-    if constexpr (NT)
-      __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
-    else
-      *reinterpret_cast<V*>(p) = v;
-  }
-}
 
 // 2x2 gate, row-major entries g00 g01 g10 g11.
 template <typename T>
@@ -86,31 +49,28 @@ __device__ __forceinline__ uint64_t item_of(uint64_t v0, uint64_t tid, int u, in
 }
 
 // ---------------------------------------------------------------------------- 1q
-// Target q >= log2(W): each work item is W consecutive "lower" amplitudes, i.e. the
+// Target q >= log2(A): each work item is A consecutive "lower" amplitudes, i.e. the
 // nested-block walk of cpu.rs:148-175 with the 2^(n-1) pairs enumerated as
 // lo = insert_zero_bit(x, q), hi = lo + 2^q (layout.rs:22-26).  Consecutive lanes read
-// consecutive 16-byte vectors of both halves: every access is coalesced for any q.
-template <typename T, int W, int U, bool NT>
-__global__ __launch_bounds__(kBlock) void k_1q_wide(T* __restrict__ re, T* __restrict__ im,
-                                                    uint64_t n_items, uint32_t q, G2<T> g,
-                                                    int contig) {
+// consecutive 16-byte chunks at lo and at hi: every access is coalesced for any q.
+template <typename T, int A, int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_1q_wide(T* __restrict__ amp, uint64_t n_items,
+                                                    uint32_t q, G2<T> g, int contig, int xcd) {
   const uint64_t lowmask = (1ull << q) - 1;
   const uint64_t half = 1ull << q;
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t tid = xcd_block(xcd) * blockDim.x + threadIdx.x;
   for (uint64_t v0 = tid; v0 < n_items; v0 += nthr * U) {
-    T ar[U][W], ai[U][W], br[U][W], bi[U][W];
+    T ar[U][A], ai[U][A], br[U][A], bi[U][A];
     uint64_t lo[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
-      const uint64_t x = v * W;
+      const uint64_t x = v * A;
       lo[u] = ((x & ~lowmask) << 1) | (x & lowmask);
       if (v < n_items) {
-        vload<T, W, NT>(re + lo[u], ar[u]);
-        vload<T, W, NT>(im + lo[u], ai[u]);
-        vload<T, W, NT>(re + lo[u] + half, br[u]);
-        vload<T, W, NT>(im + lo[u] + half, bi[u]);
+        cload<T, A, NT>(amp, lo[u], ar[u], ai[u]);
+        cload<T, A, NT>(amp, lo[u] + half, br[u], bi[u]);
       }
     }
 #pragma unroll
@@ -118,44 +78,38 @@ __global__ __launch_bounds__(kBlock) void k_1q_wide(T* __restrict__ re, T* __res
       const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
       if (v < n_items) {
 #pragma unroll
-        for (int j = 0; j < W; ++j) pair_update(ar[u][j], ai[u][j], br[u][j], bi[u][j], g);
-        vstore<T, W, NT>(re + lo[u], ar[u]);
-        vstore<T, W, NT>(im + lo[u], ai[u]);
-        vstore<T, W, NT>(re + lo[u] + half, br[u]);
-        vstore<T, W, NT>(im + lo[u] + half, bi[u]);
+        for (int j = 0; j < A; ++j) pair_update(ar[u][j], ai[u][j], br[u][j], bi[u][j], g);
+        cstore<T, A, NT>(amp, lo[u], ar[u], ai[u]);
+        cstore<T, A, NT>(amp, lo[u] + half, br[u], bi[u]);
       }
     }
   }
 }
 
-// Target q < log2(W): both partners of every pair sit inside one 16-byte vector, so each
-// work item is one vector of W amplitudes holding W/2 complete pairs.  Q is the target
-// qubit (a compile-time constant so the pairing stays in registers).
-template <typename T, int W, int Q, int U, bool NT>
-__global__ __launch_bounds__(kBlock) void k_1q_intra(T* __restrict__ re, T* __restrict__ im,
-                                                     uint64_t n_items, G2<T> g, int contig) {
+// Target q < log2(A) (complex64, q = 0): both partners of every pair sit inside one
+// 16-byte chunk, so each work item is one chunk of A amplitudes holding A/2 complete
+// pairs.  Q is the target qubit (a compile-time constant so the pairing stays in registers).
+template <typename T, int A, int Q, int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_1q_intra(T* __restrict__ amp, uint64_t n_items,
+                                                     G2<T> g, int contig, int xcd) {
   constexpr int h = 1 << Q;
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t tid = xcd_block(xcd) * blockDim.x + threadIdx.x;
   for (uint64_t v0 = tid; v0 < n_items; v0 += nthr * U) {
-    T xr[U][W], xi[U][W];
+    T xr[U][A], xi[U][A];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
-      if (v < n_items) {
-        vload<T, W, NT>(re + v * W, xr[u]);
-        vload<T, W, NT>(im + v * W, xi[u]);
-      }
+      if (v < n_items) cload<T, A, NT>(amp, v * A, xr[u], xi[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
       if (v < n_items) {
 #pragma unroll
-        for (int j = 0; j < W; ++j)
+        for (int j = 0; j < A; ++j)
           if (((j >> Q) & 1) == 0) pair_update(xr[u][j], xi[u][j], xr[u][j + h], xi[u][j + h], g);
-        vstore<T, W, NT>(re + v * W, xr[u]);
-        vstore<T, W, NT>(im + v * W, xi[u]);
+        cstore<T, A, NT>(amp, v * A, xr[u], xi[u]);
       }
     }
   }
@@ -171,10 +125,9 @@ struct DiagArg {
   T dr[64], di[64];
 };
 
-template <typename T, int W, int U, bool NT>
-__global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ re, T* __restrict__ im,
-                                                 uint64_t n_items, DiagArg<T> a,
-                                                 int contig) {
+template <typename T, int A, int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ amp, uint64_t n_items,
+                                                 DiagArg<T> a, int contig, int xcd) {
   __shared__ T sdr[64], sdi[64];
   const uint32_t sub = 1u << a.m;
   if (threadIdx.x < sub) {
@@ -183,24 +136,21 @@ __global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ re, T* __restri
   }
   __syncthreads();
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t tid = xcd_block(xcd) * blockDim.x + threadIdx.x;
   for (uint64_t v0 = tid; v0 < n_items; v0 += nthr * U) {
-    T xr[U][W], xi[U][W];
+    T xr[U][A], xi[U][A];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
-      if (v < n_items) {
-        vload<T, W, NT>(re + v * W, xr[u]);
-        vload<T, W, NT>(im + v * W, xi[u]);
-      }
+      if (v < n_items) cload<T, A, NT>(amp, v * A, xr[u], xi[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
       if (v < n_items) {
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-          const uint64_t i = v * W + j;
+        for (int j = 0; j < A; ++j) {
+          const uint64_t i = v * A + j;
           uint32_t s = 0;
           for (uint32_t b = 0; b < a.m; ++b) s |= uint32_t((i >> a.q[b]) & 1u) << b;
           const T dr = sdr[s], di = sdi[s];
@@ -208,8 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ re, T* __restri
           xr[u][j] = dr * x_r - di * x_i;
           xi[u][j] = dr * x_i + di * x_r;
         }
-        vstore<T, W, NT>(re + v * W, xr[u]);
-        vstore<T, W, NT>(im + v * W, xi[u]);
+        cstore<T, A, NT>(amp, v * A, xr[u], xi[u]);
       }
     }
   }
@@ -240,21 +189,17 @@ struct MatArg {
 
 // mq_block (cpu.rs:221-247) for m = 2, 3 with the matrix in kernel arguments (scalar
 // registers).  A work item is VW consecutive blocks o..o+VW-1; when the lowest gate qubit
-// is >= log2(VW) their amplitudes are consecutive, so every gather is a 16-byte vector.
+// is >= log2(VW) their amplitudes are consecutive, so every gather is one 16-byte chunk.
 template <typename T, int M, int VW>
-__global__ __launch_bounds__(kBlock) void k_mq_reg(T* __restrict__ re, T* __restrict__ im,
-                                                   uint64_t n_items, MqIdx<M> idx,
-                                                   MatArg<T, M> g) {
+__global__ __launch_bounds__(kBlock) void k_mq_reg(T* __restrict__ amp, uint64_t n_items,
+                                                   MqIdx<M> idx, MatArg<T, M> g, int xcd) {
   constexpr int D = 1 << M;
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+  for (uint64_t v = xcd_block(xcd) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
     const uint64_t base = insert_zero_bits<M>(v * VW, idx.sorted);
     T ar[D][VW], ai[D][VW];
 #pragma unroll
-    for (int s = 0; s < D; ++s) {
-      vload<T, VW>(re + base + idx.off[s], ar[s]);
-      vload<T, VW>(im + base + idx.off[s], ai[s]);
-    }
+    for (int s = 0; s < D; ++s) cload<T, VW>(amp, base + idx.off[s], ar[s], ai[s]);
 #pragma unroll
     for (int r = 0; r < D; ++r) {
       T yr[VW], yi[VW];
@@ -272,8 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_mq_reg(T* __restrict__ re, T* __rest
         yr[w] = accr;
         yi[w] = acci;
       }
-      vstore<T, VW>(re + base + idx.off[r], yr);
-      vstore<T, VW>(im + base + idx.off[r], yi);
+      cstore<T, VW>(amp, base + idx.off[r], yr, yi);
     }
   }
 }
@@ -283,9 +227,8 @@ __global__ __launch_bounds__(kBlock) void k_mq_reg(T* __restrict__ re, T* __rest
 // out[s][c] = sum_j U[s][j] X[j][c] in ascending j (the reference's order).  Loads run
 // along c, the direction in which the blocks' amplitudes are consecutive.
 template <typename T, int M>
-__global__ __launch_bounds__(kBlock) void k_mq_lds(T* __restrict__ re, T* __restrict__ im,
-                                                   uint64_t n_blocks, MqIdx<M> idx,
-                                                   const T* __restrict__ mat) {
+__global__ __launch_bounds__(kBlock) void k_mq_lds(T* __restrict__ amp, uint64_t n_blocks,
+                                                   MqIdx<M> idx, const T* __restrict__ mat) {
   constexpr int D = 1 << M;
   constexpr int C = 1024 / D;
   __shared__ T ur[D * D], ui[D * D];
@@ -302,8 +245,10 @@ __global__ __launch_bounds__(kBlock) void k_mq_lds(T* __restrict__ re, T* __rest
       const uint64_t o = t * C + c;
       if (o < n_blocks) {
         const uint64_t a = insert_zero_bits<M>(o, idx.sorted) + idx.off[s];
-        xr[e] = re[a];
-        xi[e] = im[a];
+        T r[1], m[1];
+        cload<T, 1>(amp, a, r, m);
+        xr[e] = r[0];
+        xi[e] = m[0];
       }
     }
     __syncthreads();
@@ -321,42 +266,41 @@ __global__ __launch_bounds__(kBlock) void k_mq_lds(T* __restrict__ re, T* __rest
         acci = acci + pi;
       }
       const uint64_t a = insert_zero_bits<M>(o, idx.sorted) + idx.off[s];
-      re[a] = accr;
-      im[a] = acci;
+      const T r[1] = {accr}, m[1] = {acci};
+      cstore<T, 1>(amp, a, r, m);
     }
   }
 }
 
 // ---------------------------------------------------------------------- init / reduce
-template <typename T, int W>
-__global__ __launch_bounds__(kBlock) void k_init_basis(T* __restrict__ re, T* __restrict__ im,
-                                                       uint64_t n_items, uint64_t basis) {
+template <typename T, int A>
+__global__ __launch_bounds__(kBlock) void k_init_basis(T* __restrict__ amp, uint64_t n_items,
+                                                       uint64_t basis, int xcd) {
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
-    T zr[W], zi[W];
+  for (uint64_t v = xcd_block(xcd) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+    T zr[A], zi[A];
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
-      zr[j] = (v * W + j == basis) ? T(1) : T(0);
+    for (int j = 0; j < A; ++j) {
+      zr[j] = (v * A + j == basis) ? T(1) : T(0);
       zi[j] = T(0);
     }
-    vstore<T, W>(re + v * W, zr);
-    vstore<T, W>(im + v * W, zi);
+    cstore<T, A>(amp, v * A, zr, zi);
   }
 }
 
 // StateVector::probabilities (state/mod.rs:130-134): p_i = re^2 + im^2 in T.
-template <typename T, int W>
-__global__ __launch_bounds__(kBlock) void k_probs(const T* __restrict__ re,
-                                                  const T* __restrict__ im,
-                                                  T* __restrict__ p, uint64_t n_items) {
+template <typename T, int A>
+__global__ __launch_bounds__(kBlock) void k_probs(const T* __restrict__ amp,
+                                                  T* __restrict__ p, uint64_t n_items,
+                                                  int xcd) {
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
-    T xr[W], xi[W], y[W];
-    vload<T, W>(re + v * W, xr);
-    vload<T, W>(im + v * W, xi);
+  for (uint64_t v = xcd_block(xcd) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+    T xr[A], xi[A];
+    cload<T, A>(amp, v * A, xr, xi);
+    VecT<T, A> y;
 #pragma unroll
-    for (int j = 0; j < W; ++j) y[j] = xr[j] * xr[j] + xi[j] * xi[j];
-    vstore<T, W>(p + v * W, y);
+    for (int j = 0; j < A; ++j) y[j] = xr[j] * xr[j] + xi[j] * xi[j];
+    *reinterpret_cast<VecT<T, A>*>(p + v * A) = y;
   }
 }
 
@@ -368,22 +312,20 @@ __device__ __forceinline__ double wave_sum(double x) {
 
 // f64 partial sums of sign(i & mask) * |psi_i|^2 (norm when mask == 0; the expectation
 // of sample.rs:177-200 otherwise), one partial per block; fixed order, so repeatable.
-template <typename T, int W>
-__global__ __launch_bounds__(kBlock) void k_reduce(const T* __restrict__ re,
-                                                   const T* __restrict__ im, uint64_t n_items,
+template <typename T, int A>
+__global__ __launch_bounds__(kBlock) void k_reduce(const T* __restrict__ amp, uint64_t n_items,
                                                    uint64_t mask, double* __restrict__ part) {
   __shared__ double ws[kBlock / 64];
   double acc = 0.0;
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
   for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
-    T xr[W], xi[W];
-    vload<T, W>(re + v * W, xr);
-    vload<T, W>(im + v * W, xi);
+    T xr[A], xi[A];
+    cload<T, A>(amp, v * A, xr, xi);
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
+    for (int j = 0; j < A; ++j) {
       const double a = double(xr[j]), b = double(xi[j]);
       const double pj = a * a + b * b;
-      acc += (__popcll((v * W + j) & mask) & 1) ? -pj : pj;
+      acc += (__popcll((v * A + j) & mask) & 1) ? -pj : pj;
     }
   }
   acc = wave_sum(acc);
@@ -411,30 +353,22 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double* __restric
   }
 }
 
-template <typename T, int W, int U, bool NT>
-__global__ __launch_bounds__(kBlock) void k_copy(const T* __restrict__ sr,
-                                                 const T* __restrict__ si, T* __restrict__ dr,
-                                                 T* __restrict__ di, uint64_t n_items,
-                                                 int contig) {
+template <typename T, int A, int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_copy(const T* __restrict__ src, T* __restrict__ dst,
+                                                 uint64_t n_items, int contig, int xcd) {
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t tid = xcd_block(xcd) * blockDim.x + threadIdx.x;
   for (uint64_t v0 = tid; v0 < n_items; v0 += nthr * U) {
-    T a[U][W], b[U][W];
+    T a[U][A], b[U][A];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
-      if (v < n_items) {
-        vload<T, W, NT>(sr + v * W, a[u]);
-        vload<T, W, NT>(si + v * W, b[u]);
-      }
+      if (v < n_items) cload<T, A, NT>(src, v * A, a[u], b[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t v = item_of(v0, tid, u, U, nthr, contig);
-      if (v < n_items) {
-        vstore<T, W, NT>(dr + v * W, a[u]);
-        vstore<T, W, NT>(di + v * W, b[u]);
-      }
+      if (v < n_items) cstore<T, A, NT>(dst, v * A, a[u], b[u]);
     }
   }
 }
@@ -467,38 +401,24 @@ hipError_t with_shape(const Tuning& t, F&& f) {
   }
 }
 
-template <typename T>
-constexpr int vec_width() {
-  return 16 / int(sizeof(T));
-}
-
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
 
-template <typename T, int W>
+template <typename T, int A>
 hipError_t launch_1q_t(const State& st, uint32_t q, const G2<T>& g, const Tuning& t,
                        hipStream_t s) {
   const uint64_t dim = st.dim();
-  T* re = static_cast<T*>(st.re);
-  T* im = static_cast<T*>(st.im);
+  T* amp = static_cast<T*>(st.amp);
   return with_shape(t, [&](auto uc, auto ntc) {
     constexpr int U = decltype(uc)::value;
     constexpr bool NT = decltype(ntc)::value;
-    if (int(q) >= ilog2(W)) {
-      const uint64_t items = (dim >> 1) / W;
-      k_1q_wide<T, W, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, q, g,
-                                                                   t.contig);
-    } else {
-      const uint64_t items = dim / W;
-      if constexpr (W >= 2) {
-        if (q == 0)
-          k_1q_intra<T, W, 0, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, g,
-                                                                          t.contig);
-      }
-      if constexpr (W >= 4) {
-        if (q == 1)
-          k_1q_intra<T, W, 1, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, g,
-                                                                          t.contig);
-      }
+    if (int(q) >= ilog2(A)) {
+      const uint64_t items = (dim >> 1) / A;
+      k_1q_wide<T, A, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(amp, items, q, g,
+                                                                   t.contig, t.xcd);
+    } else if constexpr (A >= 2) {
+      const uint64_t items = dim / A;
+      k_1q_intra<T, A, 0, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(amp, items, g,
+                                                                      t.contig, t.xcd);
     }
     return hipGetLastError();
   });
@@ -512,8 +432,8 @@ hipError_t launch_1q_any(const State& st, uint32_t q, const double* gd, const Tu
     g.r[e] = T(gd[2 * e]);
     g.i[e] = T(gd[2 * e + 1]);
   }
-  constexpr int W = vec_width<T>();
-  if (st.dim() >= uint64_t(2 * W)) return launch_1q_t<T, W>(st, q, g, t, s);
+  constexpr int A = amps_per_chunk<T>();
+  if (st.dim() >= uint64_t(A)) return launch_1q_t<T, A>(st, q, g, t, s);
   return launch_1q_t<T, 1>(st, q, g, t, s);
 }
 
@@ -527,18 +447,19 @@ hipError_t launch_diag_any(const State& st, uint32_t m, const uint32_t* q, const
     a.dr[e] = e < (1u << m) ? T(d[2 * e]) : T(0);
     a.di[e] = e < (1u << m) ? T(d[2 * e + 1]) : T(0);
   }
-  T* re = static_cast<T*>(st.re);
-  T* im = static_cast<T*>(st.im);
-  constexpr int W = vec_width<T>();
+  T* amp = static_cast<T*>(st.amp);
+  constexpr int A = amps_per_chunk<T>();
   return with_shape(t, [&](auto uc, auto ntc) {
     constexpr int U = decltype(uc)::value;
     constexpr bool NT = decltype(ntc)::value;
-    if (st.dim() >= uint64_t(W)) {
-      const uint64_t items = st.dim() / W;
-      k_diag<T, W, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, a, t.contig);
+    if (st.dim() >= uint64_t(A)) {
+      const uint64_t items = st.dim() / A;
+      k_diag<T, A, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(amp, items, a, t.contig,
+                                                                  t.xcd);
     } else {
       const uint64_t items = st.dim();
-      k_diag<T, 1, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(re, im, items, a, t.contig);
+      k_diag<T, 1, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(amp, items, a, t.contig,
+                                                                  t.xcd);
     }
     return hipGetLastError();
   });
@@ -573,15 +494,14 @@ hipError_t launch_mq_reg(const State& st, const uint32_t* q, const double* gd,
     g.r[e] = T(gd[2 * e]);
     g.i[e] = T(gd[2 * e + 1]);
   }
-  T* re = static_cast<T*>(st.re);
-  T* im = static_cast<T*>(st.im);
-  constexpr int W = vec_width<T>();
+  T* amp = static_cast<T*>(st.amp);
+  constexpr int A = amps_per_chunk<T>();
   const uint64_t blocks = st.dim() >> M;
-  if (int(idx.sorted[0]) >= ilog2(W) && blocks >= uint64_t(W)) {
-    const uint64_t items = blocks / W;
-    k_mq_reg<T, M, W><<<grid_for(items, 1, t), kBlock, 0, s>>>(re, im, items, idx, g);
+  if (int(idx.sorted[0]) >= ilog2(A) && blocks >= uint64_t(A)) {
+    const uint64_t items = blocks / A;
+    k_mq_reg<T, M, A><<<grid_for(items, 1, t), kBlock, 0, s>>>(amp, items, idx, g, t.xcd);
   } else {
-    k_mq_reg<T, M, 1><<<grid_for(blocks, 1, t), kBlock, 0, s>>>(re, im, blocks, idx, g);
+    k_mq_reg<T, M, 1><<<grid_for(blocks, 1, t), kBlock, 0, s>>>(amp, blocks, idx, g, t.xcd);
   }
   return hipGetLastError();
 }
@@ -594,8 +514,7 @@ hipError_t launch_mq_lds(const State& st, const uint32_t* q, const void* dmat,
   const uint64_t blocks = st.dim() >> M;
   uint64_t tiles = (blocks + C - 1) / C;
   if (tiles > 65536) tiles = 65536;
-  k_mq_lds<T, M><<<unsigned(tiles), kBlock, 0, s>>>(static_cast<T*>(st.re),
-                                                     static_cast<T*>(st.im), blocks, idx,
+  k_mq_lds<T, M><<<unsigned(tiles), kBlock, 0, s>>>(static_cast<T*>(st.amp), blocks, idx,
                                                      static_cast<const T*>(dmat));
   return hipGetLastError();
 }
@@ -615,30 +534,29 @@ hipError_t launch_mq_any(const State& st, uint32_t m, const uint32_t* q, const d
 
 template <typename T>
 hipError_t launch_init_t(const State& st, uint64_t basis, hipStream_t s) {
-  constexpr int W = vec_width<T>();
-  T* re = static_cast<T*>(st.re);
-  T* im = static_cast<T*>(st.im);
+  constexpr int A = amps_per_chunk<T>();
+  T* amp = static_cast<T*>(st.amp);
   Tuning t;
-  if (st.dim() >= uint64_t(W)) {
-    const uint64_t items = st.dim() / W;
-    k_init_basis<T, W><<<grid_for(items, 1, t), kBlock, 0, s>>>(re, im, items, basis);
+  if (st.dim() >= uint64_t(A)) {
+    const uint64_t items = st.dim() / A;
+    k_init_basis<T, A><<<grid_for(items, 1, t), kBlock, 0, s>>>(amp, items, basis, t.xcd);
   } else {
-    k_init_basis<T, 1><<<1, kBlock, 0, s>>>(re, im, st.dim(), basis);
+    k_init_basis<T, 1><<<1, kBlock, 0, s>>>(amp, st.dim(), basis, 0);
   }
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_probs_t(const State& st, void* out, hipStream_t s) {
-  constexpr int W = vec_width<T>();
-  const T* re = static_cast<const T*>(st.re);
-  const T* im = static_cast<const T*>(st.im);
+  constexpr int A = amps_per_chunk<T>();
+  const T* amp = static_cast<const T*>(st.amp);
   Tuning t;
-  if (st.dim() >= uint64_t(W)) {
-    const uint64_t items = st.dim() / W;
-    k_probs<T, W><<<grid_for(items, 1, t), kBlock, 0, s>>>(re, im, static_cast<T*>(out), items);
+  if (st.dim() >= uint64_t(A)) {
+    const uint64_t items = st.dim() / A;
+    k_probs<T, A><<<grid_for(items, 1, t), kBlock, 0, s>>>(amp, static_cast<T*>(out), items,
+                                                          t.xcd);
   } else {
-    k_probs<T, 1><<<1, kBlock, 0, s>>>(re, im, static_cast<T*>(out), st.dim());
+    k_probs<T, 1><<<1, kBlock, 0, s>>>(amp, static_cast<T*>(out), st.dim(), 0);
   }
   return hipGetLastError();
 }
@@ -646,37 +564,35 @@ hipError_t launch_probs_t(const State& st, void* out, hipStream_t s) {
 template <typename T>
 hipError_t launch_reduce_t(const State& st, uint64_t mask, double* part, int max_part,
                            double* result, hipStream_t s) {
-  constexpr int W = vec_width<T>();
-  const T* re = static_cast<const T*>(st.re);
-  const T* im = static_cast<const T*>(st.im);
-  const bool vec = st.dim() >= uint64_t(W);
-  const uint64_t items = vec ? st.dim() / W : st.dim();
+  constexpr int A = amps_per_chunk<T>();
+  const T* amp = static_cast<const T*>(st.amp);
+  const bool vec = st.dim() >= uint64_t(A);
+  const uint64_t items = vec ? st.dim() / A : st.dim();
   uint64_t g = (items + kBlock * 8 - 1) / (kBlock * 8);
   if (g < 1) g = 1;
   if (g > uint64_t(max_part)) g = max_part;
   if (vec)
-    k_reduce<T, W><<<unsigned(g), kBlock, 0, s>>>(re, im, items, mask, part);
+    k_reduce<T, A><<<unsigned(g), kBlock, 0, s>>>(amp, items, mask, part);
   else
-    k_reduce<T, 1><<<unsigned(g), kBlock, 0, s>>>(re, im, items, mask, part);
+    k_reduce<T, 1><<<unsigned(g), kBlock, 0, s>>>(amp, items, mask, part);
   k_reduce_final<<<1, kBlock, 0, s>>>(part, int(g), result);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_copy_t(const State& st, void* dr, void* di, const Tuning& t, hipStream_t s) {
-  constexpr int W = vec_width<T>();
-  const T* sr = static_cast<const T*>(st.re);
-  const T* si = static_cast<const T*>(st.im);
+hipError_t launch_copy_t(const State& st, void* dst, const Tuning& t, hipStream_t s) {
+  constexpr int A = amps_per_chunk<T>();
+  const T* src = static_cast<const T*>(st.amp);
+  T* d = static_cast<T*>(dst);
   return with_shape(t, [&](auto uc, auto ntc) {
     constexpr int U = decltype(uc)::value;
     constexpr bool NT = decltype(ntc)::value;
-    if (st.dim() >= uint64_t(W)) {
-      const uint64_t items = st.dim() / W;
-      k_copy<T, W, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(
-          sr, si, static_cast<T*>(dr), static_cast<T*>(di), items, t.contig);
+    if (st.dim() >= uint64_t(A)) {
+      const uint64_t items = st.dim() / A;
+      k_copy<T, A, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(src, d, items, t.contig,
+                                                                  t.xcd);
     } else {
-      k_copy<T, 1, U, NT><<<1, kBlock, 0, s>>>(sr, si, static_cast<T*>(dr),
-                                               static_cast<T*>(di), st.dim(), 0);
+      k_copy<T, 1, U, NT><<<1, kBlock, 0, s>>>(src, d, st.dim(), 0, 0);
     }
     return hipGetLastError();
   });
@@ -716,9 +632,9 @@ hipError_t launch_reduce(const State& st, uint64_t mask, double* part, int max_p
                        : launch_reduce_t<float>(st, mask, part, max_part, result, s);
 }
 
-hipError_t launch_copy(const State& st, void* dr, void* di, const Tuning& t, hipStream_t s) {
-  return st.dtype == 0 ? launch_copy_t<double>(st, dr, di, t, s)
-                       : launch_copy_t<float>(st, dr, di, t, s);
+hipError_t launch_copy(const State& st, void* dst, const Tuning& t, hipStream_t s) {
+  return st.dtype == 0 ? launch_copy_t<double>(st, dst, t, s)
+                       : launch_copy_t<float>(st, dst, t, s);
 }
 
 }  // namespace qsvh

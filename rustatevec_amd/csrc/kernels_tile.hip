@@ -17,6 +17,7 @@
 // one (crates/qsv-core/src/backend/cpu.rs:71-114,221-247).
 #include <hip/hip_runtime.h>
 
+#include "amp.hpp"
 #include "engine.hpp"
 #include "tile_ops.hpp"
 
@@ -26,31 +27,30 @@ namespace {
 constexpr int kThreads = 1 << kTileLaneBits;
 
 template <typename T, int TB>
-__global__ __launch_bounds__(kThreads) void k_tile(T* __restrict__ re, T* __restrict__ im,
-                                                   uint64_t n_tiles, TilePassArg pa,
+// Two workgroups per CU (LDS allows it up to 64 KiB tiles) need <= 128 VGPRs per lane.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_tile(T* __restrict__ amp, uint64_t n_tiles,
+                                                   TilePassArg pa,
                                                    const TileOp* __restrict__ ops,
                                                    const TileDiag* __restrict__ diags,
                                                    const T* __restrict__ pool) {
   constexpr uint32_t N = 1u << TB;
-  constexpr int W = 16 / int(sizeof(T));        // elements per 16-byte access
-  constexpr uint32_t PER = N / (kThreads * W);  // vector chunks per thread
-  static_assert(PER >= 1, "tile smaller than one vector per thread");
+  constexpr int A = amps_per_chunk<T>();        // amplitudes per 16-byte access
+  constexpr uint32_t PER = N / (kThreads * A);  // chunks per thread
+  static_assert(PER >= 1, "tile smaller than one chunk per thread");
   __shared__ T sr[N], si[N];
-  using V = T __attribute__((ext_vector_type(W)));
 
   for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
     const uint64_t base = tile::insert_zero_bits(t, pa.ts, TB);
-    // W consecutive tile-local indices are W consecutive addresses (ts starts 0, 1, 2, 3)
+    // A consecutive tile-local indices are A consecutive amplitudes (ts starts 0, 1, 2, 3)
 #pragma unroll
     for (uint32_t c = 0; c < PER; ++c) {
-      const uint32_t e = (threadIdx.x + kThreads * c) * W;
-      const uint64_t a = base + tile::offset(e, pa.ts, TB);
-      const V vr = *reinterpret_cast<const V*>(re + a);
-      const V vi = *reinterpret_cast<const V*>(im + a);
+      const uint32_t e = (threadIdx.x + kThreads * c) * A;
+      T r[A], m[A];
+      cload<T, A>(amp, base + tile::offset(e, pa.ts, TB), r, m);
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        sr[e + w] = vr[w];
-        si[e + w] = vi[w];
+      for (int w = 0; w < A; ++w) {
+        sr[e + w] = r[w];
+        si[e + w] = m[w];
       }
     }
     __syncthreads();
@@ -58,16 +58,14 @@ __global__ __launch_bounds__(kThreads) void k_tile(T* __restrict__ re, T* __rest
                                                   diags, pool);
 #pragma unroll
     for (uint32_t c = 0; c < PER; ++c) {
-      const uint32_t e = (threadIdx.x + kThreads * c) * W;
-      const uint64_t a = base + tile::offset(e, pa.ts, TB);
-      V vr, vi;
+      const uint32_t e = (threadIdx.x + kThreads * c) * A;
+      T r[A], m[A];
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        vr[w] = sr[e + w];
-        vi[w] = si[e + w];
+      for (int w = 0; w < A; ++w) {
+        r[w] = sr[e + w];
+        m[w] = si[e + w];
       }
-      *reinterpret_cast<V*>(re + a) = vr;
-      *reinterpret_cast<V*>(im + a) = vi;
+      cstore<T, A>(amp, base + tile::offset(e, pa.ts, TB), r, m);
     }
     __syncthreads();
   }
@@ -79,9 +77,8 @@ hipError_t launch_tb(const State& st, const TilePassArg& pa, const TileOp* ops,
   const uint64_t n_tiles = st.dim() >> TB;
   uint64_t g = n_tiles;
   if (g > uint64_t(max_blocks)) g = max_blocks;
-  k_tile<T, TB><<<unsigned(g), kThreads, 0, s>>>(static_cast<T*>(st.re), static_cast<T*>(st.im),
-                                                n_tiles, pa, ops, diags,
-                                                static_cast<const T*>(pool));
+  k_tile<T, TB><<<unsigned(g), kThreads, 0, s>>>(static_cast<T*>(st.amp), n_tiles, pa, ops,
+                                                diags, static_cast<const T*>(pool));
   return hipGetLastError();
 }
 

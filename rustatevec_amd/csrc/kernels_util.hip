@@ -23,8 +23,8 @@ __device__ __forceinline__ uint64_t splitmix_output(uint64_t seed, uint64_t k) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_init_random(T* __restrict__ re, T* __restrict__ im,
-                                                        uint64_t dim, uint64_t seed) {
+__global__ __launch_bounds__(kBlock) void k_init_random(T* __restrict__ amp, uint64_t dim,
+                                                        uint64_t seed) {
   const double inv53 = 1.0 / 9007199254740992.0;  // 2^-53
   const double two_pi = 6.28318530717958647692;
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
@@ -34,19 +34,19 @@ __global__ __launch_bounds__(kBlock) void k_init_random(T* __restrict__ re, T* _
     const double r = sqrt(-2.0 * log(1.0 - u1));
     double s, c;
     sincos(two_pi * u2, &s, &c);
-    re[i] = T(r * c);
-    im[i] = T(r * s);
+    amp[2 * i] = T(r * c);
+    amp[2 * i + 1] = T(r * s);
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_scale(T* __restrict__ re, T* __restrict__ im,
-                                                  uint64_t dim, const double* __restrict__ norm2) {
+__global__ __launch_bounds__(kBlock) void k_scale(T* __restrict__ amp, uint64_t dim,
+                                                  const double* __restrict__ norm2) {
   const double inv = 1.0 / sqrt(*norm2);
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += nthr) {
-    re[i] = T(double(re[i]) * inv);
-    im[i] = T(double(im[i]) * inv);
+    amp[2 * i] = T(double(amp[2 * i]) * inv);
+    amp[2 * i + 1] = T(double(amp[2 * i + 1]) * inv);
   }
 }
 
@@ -63,16 +63,14 @@ __device__ __forceinline__ double wave_max(double x) {
 
 // Five f64 partials per block: max |a - b|, Re <a|b>, Im <a|b>, <a|a>, <b|b>.
 template <typename A, typename B>
-__global__ __launch_bounds__(kBlock) void k_compare(const A* __restrict__ ar,
-                                                    const A* __restrict__ ai,
-                                                    const B* __restrict__ br,
-                                                    const B* __restrict__ bi, uint64_t dim,
+__global__ __launch_bounds__(kBlock) void k_compare(const A* __restrict__ a,
+                                                    const B* __restrict__ b, uint64_t dim,
                                                     double* __restrict__ part) {
   __shared__ double ws[5][kBlock / 64];
   double mx = 0, sr = 0, si = 0, na = 0, nb = 0;
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += nthr) {
-    const double xr = ar[i], xi = ai[i], yr = br[i], yi = bi[i];
+    const double xr = a[2 * i], xi = a[2 * i + 1], yr = b[2 * i], yi = b[2 * i + 1];
     const double dr = xr - yr, di = xi - yi;
     mx = fmax(mx, sqrt(dr * dr + di * di));
     sr += xr * yr + xi * yi;  // conj(a) * b
@@ -122,25 +120,23 @@ unsigned grid_of(uint64_t dim) {
 template <typename T>
 hipError_t init_random_t(const State& st, uint64_t seed, double* part, int max_part,
                          double* d_norm, hipStream_t s) {
-  T* re = static_cast<T*>(st.re);
-  T* im = static_cast<T*>(st.im);
+  T* amp = static_cast<T*>(st.amp);
   uint64_t g = (st.dim() + kBlock - 1) / kBlock;
   if (g > 65536) g = 65536;
-  k_init_random<T><<<unsigned(g), kBlock, 0, s>>>(re, im, st.dim(), seed);
+  k_init_random<T><<<unsigned(g), kBlock, 0, s>>>(amp, st.dim(), seed);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = launch_reduce(st, 0, part, max_part, d_norm, s);
   if (e != hipSuccess) return e;
-  k_scale<T><<<unsigned(g), kBlock, 0, s>>>(re, im, st.dim(), d_norm);
+  k_scale<T><<<unsigned(g), kBlock, 0, s>>>(amp, st.dim(), d_norm);
   return hipGetLastError();
 }
 
 template <typename A, typename B>
 hipError_t compare_t(const State& a, const State& b, double* part, double* out, hipStream_t s) {
   const unsigned g = grid_of(a.dim());
-  k_compare<A, B><<<g, kBlock, 0, s>>>(
-      static_cast<const A*>(a.re), static_cast<const A*>(a.im), static_cast<const B*>(b.re),
-      static_cast<const B*>(b.im), a.dim(), part);
+  k_compare<A, B><<<g, kBlock, 0, s>>>(static_cast<const A*>(a.amp),
+                                       static_cast<const B*>(b.amp), a.dim(), part);
   k_compare_final<<<1, 64, 0, s>>>(part, int(g), out);
   return hipGetLastError();
 }
