@@ -100,15 +100,27 @@ hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, con
                              int precision, int max_blocks, hipStream_t s);
 
 // ---- kernels_tile.hip ------------------------------------------------------------------
-// One op of a tile pass.  kind 0: 2x2 gate on tile bit lq[0]; 1: run of n_diag diagonal
-// gates starting at diags[off]; 2, 3: dense gate on tile bits lq[0..kind).  The op runs
-// on a tile only when (tile base & cmask) == cval.  Matrices live at pool[off] (complex,
-// interleaved, state precision, row-major).
+// One op of a tile pass, applied to the tile held in LDS; it runs on a tile only when
+// (tile base & cmask) == cval (controls outside the tile).
+//  kind 1: a run of n_diag diagonal gates starting at diags[off].
+//  kind 0 (dense) and 2 (permutation): a block on m = 1..3 tile-local action bits lq[0..m)
+//  (gate order), applied to the amplitudes whose tile-local condition bits (controls
+//  inside the tile) equal lcval; ins[0..nins) are the action and condition bits ascending.
+//  Dense: row-major 4^m complex matrix at pool[off]; `real`: every entry has a zero
+//  imaginary part.  Permutation: output row r is pool[off + r] times input perm[r]; bit r
+//  of pid set: that coefficient is exactly 1 + 0i (a move).
 struct TileOp {
   uint32_t kind;
   uint32_t m;
   uint32_t lq[3];
   uint32_t n_diag;
+  uint32_t nins;
+  uint32_t ins[3];
+  uint32_t insm[3];  // ~((1 << ins[j]) - 1) for j < nins, else 0: x + (x & insm) inserts a 0
+  uint32_t lcval;
+  uint32_t real;
+  uint32_t pid;
+  uint8_t perm[8];
   uint64_t cmask, cval;
   uint64_t off;
 };

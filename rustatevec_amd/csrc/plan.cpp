@@ -250,6 +250,7 @@ void TileBuilder::lower(const Launch& l, uint64_t tmask, const uint32_t* pos) {
     uint64_t cmask = 0;
     for (uint32_t k = 0; k < nout; ++k) cmask |= 1ull << l.q[out_j[k]];
     const uint32_t Din = 1u << nin;
+    std::vector<double> sub(2 * size_t(Din) * Din);
     for (uint32_t v = 0; v < (1u << nout); ++v) {
       // sub-block for outside value v over the inside bits (in gate order)
       auto full = [&](uint32_t s) {
@@ -258,28 +259,130 @@ void TileBuilder::lower(const Launch& l, uint64_t tmask, const uint32_t* pos) {
         for (uint32_t k = 0; k < nout; ++k) r |= ((v >> k) & 1u) << out_j[k];
         return r;
       };
-      std::vector<double> sub(2 * size_t(Din) * Din);
-      bool identity = true;
       for (uint32_t r = 0; r < Din; ++r)
         for (uint32_t c = 0; c < Din; ++c) {
           const size_t e = size_t(full(r)) * D + full(c);
           sub[2 * (size_t(r) * Din + c)] = l.mat[2 * e];
           sub[2 * (size_t(r) * Din + c) + 1] = l.mat[2 * e + 1];
-          if (l.mat[2 * e] != (r == c ? 1.0 : 0.0) || l.mat[2 * e + 1] != 0.0) identity = false;
         }
-      if (identity && nout > 0) continue;
-      TileOp op{};
-      op.kind = nin == 1 ? 0 : nin;
-      op.m = nin;
-      for (uint32_t k = 0; k < nin; ++k) op.lq[k] = pos[l.q[in_j[k]]];
-      op.cmask = cmask;
       uint64_t cval = 0;
       for (uint32_t k = 0; k < nout; ++k) cval |= uint64_t((v >> k) & 1u) << l.q[out_j[k]];
-      op.cval = cval;
-      op.off = add_matrix(sub.data(), size_t(Din) * Din);
-      ops.push_back(op);
+      lower_block(sub.data(), nin, in_j, l.q, pos, cmask, cval);
     }
   }
+
+// Emit the ops of one sub-block (gate order over the inside bits in_j, nin of them, with
+// the outside condition cmask/cval).  Inside bits the block is block-diagonal in (no
+// nonzero entry links a row and a column that differ in the bit: controls) become
+// conditions on the tile-local index; each non-identity block over the remaining action
+// bits becomes a permutation op (every row and column holding exactly one nonzero) or a
+// dense op.  Skipping exact zero and exact identity factors changes no value: the
+// multiplies skipped would add or multiply by exact 0 or 1 (up to the sign of a zero).
+void TileBuilder::lower_block(const double* sub, uint32_t nin, const uint32_t* in_j,
+                              const uint32_t* q, const uint32_t* pos, uint64_t cmask,
+                              uint64_t cval) {
+  const uint32_t Din = 1u << nin;
+  auto nz = [&](uint32_t r, uint32_t c) {
+    const size_t e = 2 * (size_t(r) * Din + c);
+    return sub[e] != 0.0 || sub[e + 1] != 0.0;
+  };
+  bool is_identity = true;
+  for (uint32_t r = 0; r < Din; ++r)
+    for (uint32_t c = 0; c < Din; ++c) {
+      const size_t e = 2 * (size_t(r) * Din + c);
+      if (sub[e] != (r == c ? 1.0 : 0.0) || sub[e + 1] != 0.0) is_identity = false;
+    }
+  if (is_identity) return;
+  uint32_t ctrl = 0;  // sub-index bits that are conditions
+  for (uint32_t k = 0; k < nin; ++k) {
+    bool blockdiag = true;
+    for (uint32_t r = 0; r < Din; ++r)
+      for (uint32_t c = 0; c < Din; ++c)
+        if ((((r ^ c) >> k) & 1u) && nz(r, c)) blockdiag = false;
+    if (blockdiag) ctrl |= 1u << k;
+  }
+  if (ctrl == Din - 1) ctrl = 0;  // diagonal block (not from a diagonal gate): keep it dense
+  uint32_t act[3], na = 0, cb[3], nc = 0;
+  for (uint32_t k = 0; k < nin; ++k) {
+    if ((ctrl >> k) & 1u)
+      cb[nc++] = k;
+    else
+      act[na++] = k;
+  }
+  const uint32_t Da = 1u << na;
+  std::vector<double> blk(2 * size_t(Da) * Da);
+  for (uint32_t w = 0; w < (1u << nc); ++w) {
+    auto idx = [&](uint32_t a) {
+      uint32_t s = 0;
+      for (uint32_t i = 0; i < na; ++i) s |= ((a >> i) & 1u) << act[i];
+      for (uint32_t i = 0; i < nc; ++i) s |= ((w >> i) & 1u) << cb[i];
+      return s;
+    };
+    bool ident = true, real = true;
+    for (uint32_t r = 0; r < Da; ++r)
+      for (uint32_t c = 0; c < Da; ++c) {
+        const size_t e = 2 * (size_t(idx(r)) * Din + idx(c));
+        const size_t o = 2 * (size_t(r) * Da + c);
+        blk[o] = sub[e];
+        blk[o + 1] = sub[e + 1];
+        if (blk[o] != (r == c ? 1.0 : 0.0) || blk[o + 1] != 0.0) ident = false;
+        if (blk[o + 1] != 0.0) real = false;
+      }
+    if (ident) continue;
+    TileOp op{};
+    op.m = na;
+    for (uint32_t i = 0; i < na; ++i) op.lq[i] = pos[q[in_j[act[i]]]];
+    uint32_t ins[3], ni = 0;
+    for (uint32_t i = 0; i < na; ++i) ins[ni++] = op.lq[i];
+    for (uint32_t i = 0; i < nc; ++i) {
+      const uint32_t b = pos[q[in_j[cb[i]]]];
+      ins[ni++] = b;
+      op.lcval |= ((w >> i) & 1u) << b;
+    }
+    std::sort(ins, ins + ni);
+    op.nins = ni;
+    for (uint32_t i = 0; i < ni; ++i) {
+      op.ins[i] = ins[i];
+      op.insm[i] = ~((1u << ins[i]) - 1u);
+    }
+    op.cmask = cmask;
+    op.cval = cval;
+    // permutation: exactly one nonzero per row and per column
+    bool perm = true;
+    uint32_t col_used = 0;
+    for (uint32_t r = 0; r < Da && perm; ++r) {
+      int found = -1;
+      for (uint32_t c = 0; c < Da; ++c) {
+        const size_t o = 2 * (size_t(r) * Da + c);
+        if (blk[o] != 0.0 || blk[o + 1] != 0.0) {
+          if (found >= 0) perm = false;
+          found = int(c);
+        }
+      }
+      if (found < 0 || ((col_used >> found) & 1u)) perm = false;
+      if (perm) {
+        col_used |= 1u << found;
+        op.perm[r] = uint8_t(found);
+      }
+    }
+    if (perm) {
+      op.kind = 2;
+      std::vector<double> coef(2 * Da);
+      for (uint32_t r = 0; r < Da; ++r) {
+        const size_t o = 2 * (size_t(r) * Da + op.perm[r]);
+        coef[2 * r] = blk[o];
+        coef[2 * r + 1] = blk[o + 1];
+        if (blk[o] == 1.0 && blk[o + 1] == 0.0) op.pid |= 1u << r;
+      }
+      op.off = add_matrix(coef.data(), Da);
+    } else {
+      op.kind = 0;
+      op.real = real ? 1u : 0u;
+      op.off = add_matrix(blk.data(), size_t(Da) * Da);
+    }
+    ops.push_back(op);
+  }
+}
 
 int plan_tiles(Program* p, int tile_qubits) {
   State& st = *p->st;

@@ -44,6 +44,8 @@ struct TileBuilder {
   // Lower launch l into ops for a pass whose tile bits are `tmask` (pos[q] = index of
   // state bit q among the tile bits).
   void lower(const Launch& l, uint64_t tmask, const uint32_t* pos);
+  void lower_block(const double* sub, uint32_t nin, const uint32_t* in_j, const uint32_t* q,
+                   const uint32_t* pos, uint64_t cmask, uint64_t cval);
 };
 
 // Tile schedule of one circuit's classified launches on n qubits with tb tile bits

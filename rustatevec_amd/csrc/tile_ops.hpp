@@ -40,52 +40,108 @@ __device__ __forceinline__ void cmul(T& xr, T& xi, T dr, T di) {
   xi = b;
 }
 
-// Dense M-qubit op on tile bits op.lq[0..M) (gate order): mq_block in LDS.
-template <typename T, int M, int NT>
-__device__ __forceinline__ void apply_mq(T* sr, T* si, uint32_t N, const TileOp& op,
-                                         const T* pool) {
+// First tile-local index of block `blk` of a dense or permutation op: zeros inserted at
+// the op's action and condition bits (ascending), condition bits set to their values.
+__device__ __forceinline__ uint32_t op_base(uint32_t blk, const TileOp& op) {
+  // x + (x & ~(2^p - 1)) inserts a zero bit at p (ascending p); a zero mask is a no-op.
+  // Constant field indices only: a dynamically indexed op field would put the op in scratch.
+  uint32_t b = blk;
+  b += b & op.insm[0];
+  b += b & op.insm[1];
+  b += b & op.insm[2];
+  return b | op.lcval;
+}
+
+// Tile-local index of sub-index s of the block at b0 (bit j of s is action bit lq[j]).
+template <int M>
+__device__ __forceinline__ uint32_t op_index(uint32_t b0, uint32_t s, const TileOp& op) {
+  uint32_t i = b0;
+#pragma unroll
+  for (int j = 0; j < M; ++j) i |= ((s >> j) & 1u) << op.lq[j];
+  return i;
+}
+
+// Dense M-qubit block in LDS: mq_block (cpu.rs:221-247), y_r = sum_c U[r][c] x_c from an
+// exact zero in ascending c.  For M = 1 this equals one_qubit_block's y = g0 x0 + g1 x1
+// (cpu.rs:84-93): 0 + a is a.  REAL: every entry has a zero imaginary part, whose products
+// (exact zeros) are not formed.
+template <typename T, int M, int NT, int KMAX, bool REAL>
+__device__ __forceinline__ void apply_dense(T* sr, T* si, uint32_t N, const TileOp& op,
+                                            const T* __restrict__ pool) {
   constexpr uint32_t D = 1u << M;
-  uint32_t srt[M];
+  constexpr int kRowUnroll = M == 1 ? 2 : 1;
+  // blocks per thread without conditions (a compile-time trip count lets the loads of
+  // all blocks issue together)
+  constexpr int IT = (KMAX >> M) > 0 ? (KMAX >> M) : 1;
+  const T* __restrict__ g = pool + 2 * op.off;
+  const uint32_t nblk = N >> op.nins;
 #pragma unroll
-  for (int j = 0; j < M; ++j) srt[j] = op.lq[j];
-#pragma unroll
-  for (int a = 1; a < M; ++a)
-#pragma unroll
-    for (int c = a; c > 0; --c)
-      if (srt[c - 1] > srt[c]) {
-        const uint32_t x = srt[c];
-        srt[c] = srt[c - 1];
-        srt[c - 1] = x;
-      }
-  const T* g = pool + 2 * op.off;
-  for (uint32_t blk = threadIdx.x; blk < (N >> M); blk += NT) {
-    uint32_t b0 = blk;
-#pragma unroll
-    for (int j = 0; j < M; ++j) b0 = insert_zero_bit32(b0, srt[j]);
+  for (int it = 0; it < IT; ++it) {
+    const uint32_t blk = threadIdx.x + uint32_t(it) * NT;
+    if (blk >= nblk) continue;
+    const uint32_t b0 = op_base(blk, op);
     T ar[D], ai[D];
-    uint32_t idx[D];
 #pragma unroll
     for (uint32_t s = 0; s < D; ++s) {
-      uint32_t i = b0;
-#pragma unroll
-      for (int j = 0; j < M; ++j) i |= ((s >> j) & 1u) << op.lq[j];
-      idx[s] = i;
+      const uint32_t i = op_index<M>(b0, s, op);
       ar[s] = sr[i];
       ai[s] = si[i];
     }
-#pragma unroll
+    // rows one at a time (M >= 2): the matrix is read row by row instead of being held
+    // in registers for the whole block loop
+#pragma unroll kRowUnroll
     for (uint32_t r = 0; r < D; ++r) {
-      T accr = T(0), acci = T(0);
+      // the sum starts at the first term: the reference's 0 + t is t (up to a zero's sign)
+      T accr, acci;
 #pragma unroll
       for (uint32_t c = 0; c < D; ++c) {
-        const T er = g[2 * (r * D + c)], ei = g[2 * (r * D + c) + 1];
-        const T pr = er * ar[c] - ei * ai[c];
-        const T pi = er * ai[c] + ei * ar[c];
-        accr = accr + pr;
-        acci = acci + pi;
+        const T er = g[2 * (r * D + c)];
+        T pr, pi;
+        if constexpr (REAL) {
+          pr = er * ar[c];
+          pi = er * ai[c];
+        } else {
+          const T ei = g[2 * (r * D + c) + 1];
+          pr = er * ar[c] - ei * ai[c];
+          pi = er * ai[c] + ei * ar[c];
+        }
+        accr = c == 0 ? pr : accr + pr;
+        acci = c == 0 ? pi : acci + pi;
       }
-      sr[idx[r]] = accr;
-      si[idx[r]] = acci;
+      const uint32_t i = op_index<M>(b0, r, op);
+      sr[i] = accr;
+      si[i] = acci;
+    }
+  }
+}
+
+// Permutation block: y_r = e_r x_{perm[r]}; a coefficient exactly 1 + 0i is a move.  The
+// dense product would add exact zeros to the one nonzero term, so values are the same.
+template <typename T, int M, int NT, int KMAX>
+__device__ __forceinline__ void apply_perm(T* sr, T* si, uint32_t N, const TileOp& op,
+                                           const T* __restrict__ pool) {
+  constexpr uint32_t D = 1u << M;
+  constexpr int IT = (KMAX >> M) > 0 ? (KMAX >> M) : 1;
+  const T* __restrict__ e = pool + 2 * op.off;
+  const uint32_t nblk = N >> op.nins;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const uint32_t blk = threadIdx.x + uint32_t(it) * NT;
+    if (blk >= nblk) continue;
+    const uint32_t b0 = op_base(blk, op);
+    T yr[D], yi[D];
+#pragma unroll
+    for (uint32_t r = 0; r < D; ++r) {
+      const uint32_t i = op_index<M>(b0, op.perm[r], op);
+      yr[r] = sr[i];
+      yi[r] = si[i];
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < D; ++r) {
+      if (!((op.pid >> r) & 1u)) cmul(yr[r], yi[r], e[2 * r], e[2 * r + 1]);
+      const uint32_t i = op_index<M>(b0, r, op);
+      sr[i] = yr[r];
+      si[i] = yi[r];
     }
   }
 }
@@ -139,10 +195,22 @@ __device__ __forceinline__ void with_kpos(uint32_t p, F&& f) {
 template <typename T, int NT, int KMAX, int KA, int KB>
 __device__ __forceinline__ void diag_uniform(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N,
                                              const T (&pr)[4], const T (&pi)[4], uint32_t id) {
+  auto cidx = [](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    return (KA >= 0 ? (k >> (KA < 0 ? 0 : KA)) & 1 : 0) |
+           (KB >= 0 ? ((k >> (KB < 0 ? 0 : KB)) & 1) << 1 : 0);
+  };
+  if (id == 0) {  // no identity phase: no branches
+    static_for<0, KMAX>([&](auto kc) {
+      constexpr int k = kc.value;
+      const int c = cidx(kc);
+      if (threadIdx.x + uint32_t(k) * NT < N) cmul(xr[k], xi[k], pr[c], pi[c]);
+    });
+    return;
+  }
   static_for<0, KMAX>([&](auto kc) {
     constexpr int k = kc.value;
-    constexpr int c = (KA >= 0 ? (k >> (KA < 0 ? 0 : KA)) & 1 : 0) |
-                      (KB >= 0 ? ((k >> (KB < 0 ? 0 : KB)) & 1) << 1 : 0);
+    const int c = cidx(kc);
     if (!((id >> c) & 1u) && threadIdx.x + uint32_t(k) * NT < N) {
       asm volatile("");  // keeps the uniform skip a branch (no speculated multiply + select)
       cmul(xr[k], xi[k], pr[c], pi[c]);
@@ -154,6 +222,14 @@ __device__ __forceinline__ void diag_uniform(T (&xr)[KMAX], T (&xi)[KMAX], uint3
 template <typename T, int NT, int KMAX, int KA>
 __device__ __forceinline__ void diag_lane(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N, T p0r,
                                           T p0i, T p1r, T p1i, bool skip0, bool skip1) {
+  if (!skip0 && !skip1) {  // no branches
+    static_for<0, KMAX>([&](auto kc) {
+      constexpr int k = kc.value;
+      constexpr bool hi = KA >= 0 && ((k >> (KA < 0 ? 0 : KA)) & 1);
+      if (threadIdx.x + uint32_t(k) * NT < N) cmul(xr[k], xi[k], hi ? p1r : p0r, hi ? p1i : p0i);
+    });
+    return;
+  }
   static_for<0, KMAX>([&](auto kc) {
     constexpr int k = kc.value;
     constexpr bool hi = KA >= 0 && ((k >> (KA < 0 ? 0 : KA)) & 1);
@@ -284,29 +360,29 @@ __device__ __forceinline__ void run_ops(T* sr, T* si, uint32_t N, uint64_t base,
   for (uint32_t o = 0; o < n_ops; ++o) {
     const TileOp op = ops[o];
     if ((base & op.cmask) != op.cval) continue;  // uniform over the workgroup
-    if (op.kind == 0) {
-      const T* g = pool + 2 * op.off;
-      const T g0r = g[0], g0i = g[1], g1r = g[2], g1i = g[3];
-      const T g2r = g[4], g2i = g[5], g3r = g[6], g3i = g[7];
-      const uint32_t b = op.lq[0];
-      for (uint32_t p = threadIdx.x; p < N / 2; p += NT) {
-        const uint32_t i0 = insert_zero_bit32(p, b), i1 = i0 | (1u << b);
-        const T x0r = sr[i0], x0i = si[i0], x1r = sr[i1], x1i = si[i1];
-        const T ar = g0r * x0r - g0i * x0i, ai = g0r * x0i + g0i * x0r;
-        const T br = g1r * x1r - g1i * x1i, bi = g1r * x1i + g1i * x1r;
-        const T cr = g2r * x0r - g2i * x0i, ci = g2r * x0i + g2i * x0r;
-        const T dr = g3r * x1r - g3i * x1i, di = g3r * x1i + g3i * x1r;
-        sr[i0] = ar + br;
-        si[i0] = ai + bi;
-        sr[i1] = cr + dr;
-        si[i1] = ci + di;
-      }
-    } else if (op.kind == 1) {
+    if (op.kind == 1) {
       apply_diag_run<T, NT, KMAX>(sr, si, N, base, op, diags, pool);
-    } else if (op.kind == 2) {
-      apply_mq<T, 2, NT>(sr, si, N, op, pool);
+    } else if (op.kind == 0) {
+      if (op.m == 1) {
+        if (op.real)
+          apply_dense<T, 1, NT, KMAX, true>(sr, si, N, op, pool);
+        else
+          apply_dense<T, 1, NT, KMAX, false>(sr, si, N, op, pool);
+      } else if (op.m == 2) {
+        if (op.real)
+          apply_dense<T, 2, NT, KMAX, true>(sr, si, N, op, pool);
+        else
+          apply_dense<T, 2, NT, KMAX, false>(sr, si, N, op, pool);
+      } else {
+        apply_dense<T, 3, NT, KMAX, false>(sr, si, N, op, pool);
+      }
     } else {
-      apply_mq<T, 3, NT>(sr, si, N, op, pool);
+      if (op.m == 1)
+        apply_perm<T, 1, NT, KMAX>(sr, si, N, op, pool);
+      else if (op.m == 2)
+        apply_perm<T, 2, NT, KMAX>(sr, si, N, op, pool);
+      else
+        apply_perm<T, 3, NT, KMAX>(sr, si, N, op, pool);
     }
     __syncthreads();
   }
