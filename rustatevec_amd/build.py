@@ -22,6 +22,14 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value"]
+# The tile interpreters keep each lane's amplitudes in register arrays indexed by
+# compile-time constants in many specialised branches; sinking the branches' common code
+# into their successor turns those accesses into a select of addresses and moves the
+# arrays to scratch, so it is disabled there.
+NO_SINK = ["-mllvm", "-simplifycfg-sink-common=false"]
+# Files whose kernels must not use scratch (private memory): checked from the compiler's
+# resource-usage remarks at every build.
+NO_SCRATCH = {"kernels_plain.hip", "kernels_mfma.hip", "kernels_tile.hip", "kernels_batch.hip"}
 # file -> extra flags
 SOURCES = {
     "ir.cpp": ["-ffp-contract=off"],
@@ -31,8 +39,8 @@ SOURCES = {
     "kernels_compose.hip": ["-ffp-contract=off"],
     "kernels_util.hip": [],
     "kernels_mfma.hip": [],
-    "kernels_tile.hip": ["-ffp-contract=off"],
-    "kernels_batch.hip": ["-ffp-contract=off"],
+    "kernels_tile.hip": ["-ffp-contract=off", *NO_SINK],
+    "kernels_batch.hip": ["-ffp-contract=off", *NO_SINK],
     "batch.cpp": [],
     "kernels_mlp.hip": [],
     "mlp.cpp": [],
@@ -44,13 +52,32 @@ def _compile(name: str, flags: list[str], verbose: bool) -> str:
     src = os.path.join(CSRC, name)
     obj = os.path.join(BUILD, name + ".o")
     lang = ["-x", "hip"] if name.endswith(".hip") else []
-    cmd = [HIPCC, *COMMON, *flags, *lang, "-c", src, "-o", obj]
+    remarks = ["-Rpass-analysis=kernel-resource-usage"] if name in NO_SCRATCH else []
+    cmd = [HIPCC, *COMMON, *flags, *remarks, *lang, "-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {name}:\n{r.stderr}")
+    if remarks:
+        _check_no_scratch(name, r.stderr)
     return obj
+
+
+def _check_no_scratch(name: str, remarks: str) -> None:
+    """Fail the build when a kernel of `name` uses private memory (spills or arrays
+    the compiler could not keep in registers)."""
+    func = None
+    bad = []
+    for line in remarks.splitlines():
+        if "Function Name:" in line:
+            func = line.split("Function Name:")[1].split()[0]
+        elif "ScratchSize [bytes/lane]:" in line:
+            size = int(line.split("ScratchSize [bytes/lane]:")[1].split()[0])
+            if size > 0:
+                bad.append(f"{func}: {size} bytes/lane")
+    if bad:
+        raise RuntimeError(f"{name}: kernels using scratch memory:\n  " + "\n  ".join(bad))
 
 
 def build(verbose: bool = False, jobs: int = 8) -> str:

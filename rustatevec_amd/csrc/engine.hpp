@@ -106,9 +106,13 @@ hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, con
 //  kind 0 (dense) and 2 (permutation): a block on m = 1..3 tile-local action bits lq[0..m)
 //  (gate order), applied to the amplitudes whose tile-local condition bits (controls
 //  inside the tile) equal lcval; ins[0..nins) are the action and condition bits ascending.
-//  Dense: row-major 4^m complex matrix at pool[off]; `real`: every entry has a zero
-//  imaginary part.  Permutation: output row r is pool[off + r] times input perm[r]; bit r
-//  of pid set: that coefficient is exactly 1 + 0i (a move).
+//  Dense: row-major 4^m complex matrix at pool[off]; `real`: for m >= 2, 1 when every
+//  entry has a zero imaginary part; for m = 1 the pair variant (0 general, 1 real, 2
+//  anti-diagonal, 3 swap).  Permutation (m >= 2): output row r is pool[off + r] times
+//  input perm[r]; bit r of pid set: that coefficient is exactly 1 + 0i (a move).
+//  kind 3: a register block on bits lq[0..m); its n_diag members (1-qubit dense ops whose
+//  lq[0] is an index into the block's bits and insm[0] / lcval a condition on the block's
+//  sub-index) follow it in the op list.
 struct TileOp {
   uint32_t kind;
   uint32_t m;

@@ -151,6 +151,7 @@ void schedule_tiles(const std::vector<Launch>& in, uint32_t n, uint32_t tb, Tile
         bld.lower(in[i], tmask, pos);
         order.push_back(i);
       }
+      bld.close_block();
       pa.n_ops = uint32_t(bld.ops.size()) - pa.op_begin;
       Launch l;
       l.kind = Launch::kTile;
@@ -183,6 +184,7 @@ uint64_t TileBuilder::add_matrix(const double* g, size_t n_complex) {
 
 void TileBuilder::lower(const Launch& l, uint64_t tmask, const uint32_t* pos) {
     if (l.kind == Launch::kDiag) {
+      close_block();
       if (ops.size() == pass_begin || ops.back().kind != 1 ||
           ops.back().off + ops.back().n_diag != diags.size()) {
         TileOp op{};
@@ -365,7 +367,14 @@ void TileBuilder::lower_block(const double* sub, uint32_t nin, const uint32_t* i
         op.perm[r] = uint8_t(found);
       }
     }
-    if (perm) {
+    if (na == 1) {
+      // 1-qubit blocks keep the dense 2x2; `real` names the pair variant of the kernel
+      op.kind = 0;
+      const bool anti = perm && op.perm[0] == 1;
+      const bool unit = anti && blk[2] == 1.0 && blk[3] == 0.0 && blk[4] == 1.0 && blk[5] == 0.0;
+      op.real = unit ? 3u : anti ? 2u : real ? 1u : 0u;
+      op.off = add_matrix(blk.data(), 4);
+    } else if (perm) {
       op.kind = 2;
       std::vector<double> coef(2 * Da);
       for (uint32_t r = 0; r < Da; ++r) {
@@ -380,7 +389,84 @@ void TileBuilder::lower_block(const double* sub, uint32_t nin, const uint32_t* i
       op.real = real ? 1u : 0u;
       op.off = add_matrix(blk.data(), size_t(Da) * Da);
     }
+    emit(op);
+  }
+}
+
+// Register blocks take 1-qubit ops on tile-local bits >= kBlockMinBit only: a block on
+// bits 0..2 has every lane 8 amplitudes apart in LDS (16-way bank conflicts), where a
+// standalone pair op keeps consecutive lanes on consecutive amplitudes.  All members of a
+// block share its outside condition, so a tile that fails it skips the whole block.
+constexpr uint32_t kBlockMinBit = 3;
+
+void TileBuilder::emit(const TileOp& op) {
+  uint32_t bits = 0;
+  for (uint32_t i = 0; i < op.nins; ++i) bits |= 1u << op.ins[i];
+  if (op.m == 1 && op.kind == 0 && (bits & ((1u << kBlockMinBit) - 1u)) == 0) {
+    if (open_block >= 0 && __builtin_popcount(block_bits | bits) <= 3 &&
+        ops[size_t(open_block)].cmask == op.cmask && ops[size_t(open_block)].cval == op.cval) {
+      block_bits |= bits;
+      ops[size_t(open_block)].n_diag++;
+      ops.push_back(op);
+      return;
+    }
+    close_block();
+    TileOp b{};
+    b.kind = 3;
+    b.n_diag = 1;
+    b.cmask = op.cmask;
+    b.cval = op.cval;
+    open_block = int64_t(ops.size());
+    block_bits = bits;
+    ops.push_back(b);
     ops.push_back(op);
+    return;
+  }
+  close_block();
+  ops.push_back(op);
+}
+
+// Finalise the open register block: its bits (ascending) become lq / ins of the block
+// op; each member's target becomes an index j into them and its conditions a mask and
+// value over the block's sub-index.  A block of one member is just that member.
+void TileBuilder::close_block() {
+  if (open_block < 0) return;
+  const size_t h = size_t(open_block);
+  open_block = -1;
+  TileOp& b = ops[h];
+  if (b.n_diag == 1) {
+    ops.erase(ops.begin() + std::ptrdiff_t(h));
+    return;
+  }
+  uint32_t nb = 0, bit[3];
+  for (uint32_t t = 0; t < 32; ++t)
+    if ((block_bits >> t) & 1u) bit[nb++] = t;
+  b.m = nb;
+  b.nins = nb;
+  for (uint32_t j = 0; j < nb; ++j) {
+    b.lq[j] = bit[j];
+    b.ins[j] = bit[j];
+    b.insm[j] = ~((1u << bit[j]) - 1u);
+  }
+  auto index_of = [&](uint32_t t) {
+    for (uint32_t j = 0; j < nb; ++j)
+      if (bit[j] == t) return j;
+    return 0u;
+  };
+  for (size_t k = h + 1; k < h + 1 + b.n_diag; ++k) {
+    TileOp& g = ops[k];
+    const uint32_t target = g.lq[0];
+    uint32_t cm = 0, cv = 0;
+    for (uint32_t i = 0; i < g.nins; ++i) {
+      const uint32_t t = g.ins[i];
+      if (t == target) continue;
+      const uint32_t j = index_of(t);
+      cm |= 1u << j;
+      cv |= ((g.lcval >> t) & 1u) << j;
+    }
+    g.lq[0] = index_of(target);
+    g.insm[0] = cm;
+    g.lcval = cv;
   }
 }
 

@@ -46,6 +46,12 @@ struct TileBuilder {
   void lower(const Launch& l, uint64_t tmask, const uint32_t* pos);
   void lower_block(const double* sub, uint32_t nin, const uint32_t* in_j, const uint32_t* q,
                    const uint32_t* pos, uint64_t cmask, uint64_t cval);
+  // Consecutive 1-qubit ops whose target and condition bits fit in 3 tile bits are
+  // gathered under one register-block op (kind 3), whose members follow it.
+  int64_t open_block = -1;  // index of the open register-block op, or -1
+  uint32_t block_bits = 0;  // tile-local bits of the open block
+  void emit(const TileOp& op);
+  void close_block();
 };
 
 // Tile schedule of one circuit's classified launches on n qubits with tb tile bits

@@ -136,9 +136,13 @@ __device__ __forceinline__ void apply_perm(T* sr, T* si, uint32_t N, const TileO
       yr[r] = sr[i];
       yi[r] = si[i];
     }
+    if (op.pid != (1u << D) - 1u) {  // some coefficient is not 1: multiply those
+#pragma unroll
+      for (uint32_t r = 0; r < D; ++r)
+        if (!((op.pid >> r) & 1u)) cmul(yr[r], yi[r], e[2 * r], e[2 * r + 1]);
+    }
 #pragma unroll
     for (uint32_t r = 0; r < D; ++r) {
-      if (!((op.pid >> r) & 1u)) cmul(yr[r], yi[r], e[2 * r], e[2 * r + 1]);
       const uint32_t i = op_index<M>(b0, r, op);
       sr[i] = yr[r];
       si[i] = yi[r];
@@ -151,6 +155,8 @@ constexpr uint32_t ilog2c(uint32_t x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
 // f(integral_constant<int, k>) for k = K0 .. K1-1.  Register arrays indexed only through
 // it are indexed by constants from the start, so they become plain registers (a loop
 // unrolled later leaves them as one vector value that control flow then copies whole).
+// Every lambda passed here and to the other dispatchers is always_inline: an outlined
+// lambda would take its captured register arrays by address and put them in scratch.
 template <int K0, int K1, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
   if constexpr (K0 < K1) {
@@ -195,20 +201,20 @@ __device__ __forceinline__ void with_kpos(uint32_t p, F&& f) {
 template <typename T, int NT, int KMAX, int KA, int KB>
 __device__ __forceinline__ void diag_uniform(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N,
                                              const T (&pr)[4], const T (&pi)[4], uint32_t id) {
-  auto cidx = [](auto kc) {
+  auto cidx = [](auto kc) __attribute__((always_inline)) {
     constexpr int k = decltype(kc)::value;
     return (KA >= 0 ? (k >> (KA < 0 ? 0 : KA)) & 1 : 0) |
            (KB >= 0 ? ((k >> (KB < 0 ? 0 : KB)) & 1) << 1 : 0);
   };
   if (id == 0) {  // no identity phase: no branches
-    static_for<0, KMAX>([&](auto kc) {
+    static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
       constexpr int k = kc.value;
       const int c = cidx(kc);
       if (threadIdx.x + uint32_t(k) * NT < N) cmul(xr[k], xi[k], pr[c], pi[c]);
     });
     return;
   }
-  static_for<0, KMAX>([&](auto kc) {
+  static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
     constexpr int k = kc.value;
     const int c = cidx(kc);
     if (!((id >> c) & 1u) && threadIdx.x + uint32_t(k) * NT < N) {
@@ -223,14 +229,14 @@ template <typename T, int NT, int KMAX, int KA>
 __device__ __forceinline__ void diag_lane(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N, T p0r,
                                           T p0i, T p1r, T p1i, bool skip0, bool skip1) {
   if (!skip0 && !skip1) {  // no branches
-    static_for<0, KMAX>([&](auto kc) {
+    static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
       constexpr int k = kc.value;
       constexpr bool hi = KA >= 0 && ((k >> (KA < 0 ? 0 : KA)) & 1);
       if (threadIdx.x + uint32_t(k) * NT < N) cmul(xr[k], xi[k], hi ? p1r : p0r, hi ? p1i : p0i);
     });
     return;
   }
-  static_for<0, KMAX>([&](auto kc) {
+  static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
     constexpr int k = kc.value;
     constexpr bool hi = KA >= 0 && ((k >> (KA < 0 ? 0 : KA)) & 1);
     if (!(hi ? skip1 : skip0) && threadIdx.x + uint32_t(k) * NT < N) {
@@ -251,7 +257,7 @@ __device__ __forceinline__ void diag_gate_small(T (&xr)[KMAX], T (&xi)[KMAX], ui
   if (nout > 1) so |= uint32_t((base >> dg->oq1) & 1u) << 1;
   const T* __restrict__ tab = pool + 2 * (dg->tab + 4 * so);
   T er[4], ei[4];
-  static_for<0, 4>([&](auto c) {
+  static_for<0, 4>([&](auto c) __attribute__((always_inline)) {
     er[c.value] = tab[2 * c.value];
     ei[c.value] = tab[2 * c.value + 1];
   });
@@ -262,12 +268,12 @@ __device__ __forceinline__ void diag_gate_small(T (&xr)[KMAX], T (&xi)[KMAX], ui
     diag_uniform<T, NT, KMAX, -1, -1>(xr, xi, N, er, ei, id);
   } else if (kind0 >= 2) {  // register bits only: phases uniform for each k
     if (kind1 == 0) {
-      with_kpos<LK>(kind0 - 2, [&](auto A) {
+      with_kpos<LK>(kind0 - 2, [&](auto A) __attribute__((always_inline)) {
         diag_uniform<T, NT, KMAX, A.value, -1>(xr, xi, N, er, ei, id);
       });
     } else {
-      with_kpos<LK>(kind0 - 2, [&](auto A) {
-        with_kpos<LK>(kind1 - 2, [&](auto B) {
+      with_kpos<LK>(kind0 - 2, [&](auto A) __attribute__((always_inline)) {
+        with_kpos<LK>(kind1 - 2, [&](auto B) __attribute__((always_inline)) {
           if constexpr (A.value < B.value)
             diag_uniform<T, NT, KMAX, A.value, B.value>(xr, xi, N, er, ei, id);
         });
@@ -290,7 +296,7 @@ __device__ __forceinline__ void diag_gate_small(T (&xr)[KMAX], T (&xi)[KMAX], ui
       const T p0r = l0 ? er[1] : er[0], p0i = l0 ? ei[1] : ei[0];
       const T p1r = l0 ? er[3] : er[2], p1i = l0 ? ei[3] : ei[2];
       const bool skip0 = (id & 3u) == 3u, skip1 = (id >> 2) == 3u;
-      with_kpos<LK>(kind1 - 2, [&](auto A) {
+      with_kpos<LK>(kind1 - 2, [&](auto A) __attribute__((always_inline)) {
         diag_lane<T, NT, KMAX, A.value>(xr, xi, N, p0r, p0i, p1r, p1i, skip0, skip1);
       });
     }
@@ -304,7 +310,7 @@ __device__ __forceinline__ void diag_gate_wide(T (&xr)[KMAX], T (&xi)[KMAX], uin
                                                const T* __restrict__ pool) {
   const uint32_t m = dg->m;
   const T* __restrict__ tab = pool + 2 * dg->tab;
-  static_for<0, KMAX>([&](auto kc) {
+  static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
     constexpr int k = kc.value;
     const uint32_t e = threadIdx.x + uint32_t(k) * NT;
     if (e < N) {
@@ -328,7 +334,7 @@ __device__ __forceinline__ void apply_diag_run(T* sr, T* si, uint32_t N, uint64_
                                                const TileDiag* __restrict__ diags,
                                                const T* __restrict__ pool) {
   T xr[KMAX], xi[KMAX];
-  static_for<0, KMAX>([&](auto kc) {
+  static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
     const uint32_t e = threadIdx.x + uint32_t(kc.value) * NT;
     if (e < N) {
       xr[kc.value] = sr[e];
@@ -342,13 +348,143 @@ __device__ __forceinline__ void apply_diag_run(T* sr, T* si, uint32_t N, uint64_
     else
       diag_gate_wide<T, NT, KMAX>(xr, xi, N, base, dg, pool);
   }
-  static_for<0, KMAX>([&](auto kc) {
+  static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
     const uint32_t e = threadIdx.x + uint32_t(kc.value) * NT;
     if (e < N) {
       sr[e] = xr[kc.value];
       si[e] = xi[kc.value];
     }
   });
+}
+
+// Math of a 1-qubit op on the pair (x0, x1): one_qubit_block's y0 = g00 x0 + g01 x1,
+// y1 = g10 x0 + g11 x1 (cpu.rs:84-93) in the same operation order.  V = 0: general;
+// 1: real matrix (the products of its zero imaginary parts are not formed); 2:
+// anti-diagonal (y0 = g01 x1, y1 = g10 x0: the zero entries' products are not formed);
+// 3: a swap (anti-diagonal with unit entries).  Same values as the dense product: only
+// exact zeros are left out (up to the sign of a zero).
+constexpr uint32_t kPairGeneral = 0, kPairReal = 1, kPairAnti = 2, kPairSwap = 3;
+
+template <typename T, uint32_t V>
+__device__ __forceinline__ void pair_math(T& x0r, T& x0i, T& x1r, T& x1i,
+                                          const T* __restrict__ m) {
+  if constexpr (V == kPairSwap) {
+    const T tr = x0r, ti = x0i;
+    x0r = x1r;
+    x0i = x1i;
+    x1r = tr;
+    x1i = ti;
+  } else if constexpr (V == kPairAnti) {
+    const T y0r = x1r, y0i = x1i, y1r = x0r, y1i = x0i;
+    x0r = y0r;
+    x0i = y0i;
+    x1r = y1r;
+    x1i = y1i;
+    cmul(x0r, x0i, m[2], m[3]);  // g01
+    cmul(x1r, x1i, m[4], m[5]);  // g10
+  } else if constexpr (V == kPairReal) {
+    const T a = m[0], b = m[2], c = m[4], d = m[6];
+    const T y0r = a * x0r + b * x1r, y0i = a * x0i + b * x1i;
+    const T y1r = c * x0r + d * x1r, y1i = c * x0i + d * x1i;
+    x0r = y0r;
+    x0i = y0i;
+    x1r = y1r;
+    x1i = y1i;
+  } else {
+    const T g0r = m[0], g0i = m[1], g1r = m[2], g1i = m[3];
+    const T g2r = m[4], g2i = m[5], g3r = m[6], g3i = m[7];
+    const T ar = g0r * x0r - g0i * x0i, ai = g0r * x0i + g0i * x0r;
+    const T br = g1r * x1r - g1i * x1i, bi = g1r * x1i + g1i * x1r;
+    const T cr = g2r * x0r - g2i * x0i, ci = g2r * x0i + g2i * x0r;
+    const T dr = g3r * x1r - g3i * x1i, di = g3r * x1i + g3i * x1r;
+    x0r = ar + br;
+    x0i = ai + bi;
+    x1r = cr + dr;
+    x1i = ci + di;
+  }
+}
+
+// Call f(integral_constant<uint32_t, V>) for a 1-qubit op's pair variant (uniform).
+template <typename F>
+__device__ __forceinline__ void with_pair_variant(uint32_t v, F&& f) {
+  using std::integral_constant;
+  if (v == kPairSwap) return f(integral_constant<uint32_t, kPairSwap>{});
+  if (v == kPairAnti) return f(integral_constant<uint32_t, kPairAnti>{});
+  if (v == kPairReal) return f(integral_constant<uint32_t, kPairReal>{});
+  return f(integral_constant<uint32_t, kPairGeneral>{});
+}
+
+// A standalone 1-qubit op (kind 0, m = 1) on action bit lq[0] with conditions.
+template <typename T, uint32_t V, int NT, int KMAX>
+__device__ __forceinline__ void apply_1q(T* sr, T* si, uint32_t N, const TileOp& op,
+                                         const T* __restrict__ pool) {
+  constexpr int IT = (KMAX >> 1) > 0 ? (KMAX >> 1) : 1;
+  const T* __restrict__ m = pool + 2 * op.off;
+  const uint32_t nblk = N >> op.nins;
+  const uint32_t h = 1u << op.lq[0];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const uint32_t blk = threadIdx.x + uint32_t(it) * NT;
+    if (blk >= nblk) continue;
+    const uint32_t i0 = op_base(blk, op), i1 = i0 | h;
+    T x0r = sr[i0], x0i = si[i0], x1r = sr[i1], x1i = si[i1];
+    pair_math<T, V>(x0r, x0i, x1r, x1i, m);
+    sr[i0] = x0r;
+    si[i0] = x0i;
+    sr[i1] = x1r;
+    si[i1] = x1i;
+  }
+}
+
+// Register block (kind 3): each lane loads blocks of 2^NB amplitudes (the block's bits
+// lq[0..NB)), applies the op's members in order -- 1-qubit ops on block bit j, skipped for
+// sub-indices whose condition bits differ -- and stores them: one LDS round trip and one
+// barrier for the whole run instead of one per gate.
+template <typename T, int NB, int NT, int KMAX>
+__device__ __forceinline__ void apply_regblock(T* sr, T* si, uint32_t N, uint64_t base,
+                                               const TileOp& op,
+                                               const TileOp* __restrict__ mem,
+                                               const T* __restrict__ pool) {
+  constexpr uint32_t D = 1u << NB;
+  constexpr int IT = (KMAX >> NB) > 0 ? (KMAX >> NB) : 1;
+  const uint32_t nblk = N >> NB;
+  const uint32_t nmem = op.n_diag;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const uint32_t blk = threadIdx.x + uint32_t(it) * NT;
+    if (blk >= nblk) continue;
+    const uint32_t b0 = op_base(blk, op);
+    T xr[D], xi[D];
+    static_for<0, int(D)>([&](auto sc) __attribute__((always_inline)) {
+      const uint32_t i = op_index<NB>(b0, uint32_t(sc.value), op);
+      xr[sc.value] = sr[i];
+      xi[sc.value] = si[i];
+    });
+    for (uint32_t u = 0; u < nmem; ++u) {
+      const TileOp& g = mem[u];
+      if ((base & g.cmask) != g.cval) continue;
+      const uint32_t cm = g.insm[0], cv = g.lcval;
+      const T* __restrict__ mat = pool + 2 * g.off;
+      with_pair_variant(g.real, [&](auto vc) __attribute__((always_inline)) {
+        constexpr uint32_t V = vc.value;
+        with_kpos<NB>(g.lq[0], [&](auto jc) __attribute__((always_inline)) {
+          constexpr int J = jc.value;
+          static_for<0, int(D)>([&](auto sc) __attribute__((always_inline)) {
+            constexpr uint32_t S = uint32_t(sc.value);
+            if constexpr (((S >> J) & 1u) == 0) {
+              constexpr uint32_t S1 = S | (1u << J);
+              if ((S & cm) == cv) pair_math<T, V>(xr[S], xi[S], xr[S1], xi[S1], mat);
+            }
+          });
+        });
+      });
+    }
+    static_for<0, int(D)>([&](auto sc) __attribute__((always_inline)) {
+      const uint32_t i = op_index<NB>(b0, uint32_t(sc.value), op);
+      sr[i] = xr[sc.value];
+      si[i] = xi[sc.value];
+    });
+  }
 }
 
 // Interpret ops[0..n_ops) on the tile (LDS); ends with a workgroup barrier.
@@ -359,15 +495,29 @@ __device__ __forceinline__ void run_ops(T* sr, T* si, uint32_t N, uint64_t base,
                                         const T* __restrict__ pool) {
   for (uint32_t o = 0; o < n_ops; ++o) {
     const TileOp op = ops[o];
+    if (op.kind == 3) {  // register block; its op.n_diag members follow it
+      if ((base & op.cmask) != op.cval) {  // the members' common outside condition
+        o += op.n_diag;
+        continue;
+      }
+      if (op.m == 1)
+        apply_regblock<T, 1, NT, KMAX>(sr, si, N, base, op, ops + o + 1, pool);
+      else if (op.m == 2)
+        apply_regblock<T, 2, NT, KMAX>(sr, si, N, base, op, ops + o + 1, pool);
+      else
+        apply_regblock<T, 3, NT, KMAX>(sr, si, N, base, op, ops + o + 1, pool);
+      o += op.n_diag;
+      __syncthreads();
+      continue;
+    }
     if ((base & op.cmask) != op.cval) continue;  // uniform over the workgroup
     if (op.kind == 1) {
       apply_diag_run<T, NT, KMAX>(sr, si, N, base, op, diags, pool);
     } else if (op.kind == 0) {
       if (op.m == 1) {
-        if (op.real)
-          apply_dense<T, 1, NT, KMAX, true>(sr, si, N, op, pool);
-        else
-          apply_dense<T, 1, NT, KMAX, false>(sr, si, N, op, pool);
+        with_pair_variant(op.real, [&](auto vc) __attribute__((always_inline)) {
+          apply_1q<T, vc.value, NT, KMAX>(sr, si, N, op, pool);
+        });
       } else if (op.m == 2) {
         if (op.real)
           apply_dense<T, 2, NT, KMAX, true>(sr, si, N, op, pool);
@@ -377,9 +527,7 @@ __device__ __forceinline__ void run_ops(T* sr, T* si, uint32_t N, uint64_t base,
         apply_dense<T, 3, NT, KMAX, false>(sr, si, N, op, pool);
       }
     } else {
-      if (op.m == 1)
-        apply_perm<T, 1, NT, KMAX>(sr, si, N, op, pool);
-      else if (op.m == 2)
+      if (op.m == 2)
         apply_perm<T, 2, NT, KMAX>(sr, si, N, op, pool);
       else
         apply_perm<T, 3, NT, KMAX>(sr, si, N, op, pool);
