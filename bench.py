@@ -292,6 +292,35 @@ def main():
     st64.free()
     ref.free()
 
+    # layered(28) through the same forms: greedy / learned fusion on the matrix cores and
+    # exact complex64 tile passes, each judged against the plain complex128 run
+    ref = b.alloc(n_layered, rs.C128)
+    b.init_basis(ref, 0)
+    p = rs.Program(ref, lay)
+    p.launch()
+    del p
+    st64 = b.alloc(n_layered, rs.C64)
+    reset64 = lambda: b.init_basis(st64, 0)  # noqa: E731
+    planned, _, chosen = planner.plan(b, model, lay, 6)
+    lay_legs = {}
+    cands = [("greedy_k%d" % k, rs.fuse(b, lay, k), rs.PREC_MFMA_BF16X3, 0) for k in (4, 5, 6)]
+    cands += [("learned", planned, rs.PREC_MFMA_BF16X3, 0),
+              ("tiled_c64_exact", lay, rs.PREC_EXACT, 13)]
+    for name, circ, prec, tq in cands:
+        p = rs.Program(st64, circ, precision=prec, tile_qubits=tq)
+        s = timed_steps(b, st64, p, 2, 1, reset64)
+        reset64()
+        p.launch()
+        err, fid = b.compare(ref, st64)
+        lay_legs[name] = {"passes": p.num_kernels(), "circuit_s": round(s, 5),
+                          "speedup_vs_plain_c128": round(lay_s / s, 2),
+                          "max_abs_err": err, "one_minus_fidelity": 1 - fid}
+        del p
+    lay_legs["learned"]["chosen"] = "dp" if chosen == 0 else f"greedy_k{chosen}"
+    log(f"layered({n_layered}) fused / learned / tiled c64: {lay_legs}")
+    st64.free()
+    ref.free()
+
     # ------------------------------------------------------------ GHZ(20)
     st = b.alloc(20, rs.C128)
     b.init_basis(st, 0)
@@ -340,7 +369,8 @@ def main():
         "fused": fused_out,
         "mfma_roofline": mfma_roof,
         "layered28": {"plain_s": round(lay_s, 4), "plain_passes": lay_passes,
-                      "tiled_s": round(lay_t, 4), "tiled_passes": lay_tp},
+                      "tiled_s": round(lay_t, 4), "tiled_passes": lay_tp,
+                      "c64_forms": lay_legs},
         "batched_hea": batch_rec,
         "learned_planner": learned_rec,
         "ghz20_us": round(ghz_ms * 1e3, 2),
