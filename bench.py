@@ -146,6 +146,17 @@ def main():
     reset()
     rs.Program(st, qft).launch()
 
+    # on-device sampling of the QFT(30) output state (inverse CDF, 1e6 shots)
+    shots = 1_000_000
+    b.sample(st, shots, seed)
+    t0 = time.perf_counter()
+    outcomes, _ = b.sample(st, shots, seed)
+    sample_s = time.perf_counter() - t0
+    sampling = {"qubits": n, "shots": shots, "wall_ms": round(sample_s * 1e3, 3),
+                "state_bytes_read": (1 << n) * 16,
+                "distinct_outcomes": int(len(np.unique(outcomes)))}
+    log(f"sampling {shots} shots of QFT({n}): {sampling}")
+
     # ------------------------------------------------------------ tiled exact c128
     tiled = {}
     st2 = b.alloc(n, rs.C128)
@@ -292,6 +303,39 @@ def main():
     st64.free()
     ref.free()
 
+    # the same model across the C5 size range (24..32 qubits, 40 n gates): the reference's
+    # default greedy k = 4, greedy k = 6 and the learned plan, each against plain c128
+    sweep = []
+    for nq2 in (() if args.quick else (24, 32)):
+        mc = rs.mix(nq2, 40 * nq2, seed + nq2)
+        ref = b.alloc(nq2, rs.C128)
+        resetr = lambda: b.init_basis(ref, 0)  # noqa: E731
+        p = rs.Program(ref, mc)
+        plain_s = timed_steps(b, ref, p, 1, 0, resetr)
+        resetr()
+        p.launch()
+        del p
+        st64 = b.alloc(nq2, rs.C64)
+        reset64 = lambda: b.init_basis(st64, 0)  # noqa: E731
+        planned, _, chosen = planner.plan(b, model, mc, 6)
+        rec = {"qubits": nq2, "gates": len(mc), "plain_c128_s": round(plain_s, 4),
+               "chosen": "dp" if chosen == 0 else f"greedy_k{chosen}"}
+        for name, circ in (("greedy_k4", rs.fuse(b, mc, 4)), ("greedy_k6", rs.fuse(b, mc, 6)),
+                           ("learned", planned)):
+            p = rs.Program(st64, circ, precision=rs.PREC_MFMA_BF16X3)
+            s_ = timed_steps(b, st64, p, 1, 1, reset64)
+            reset64()
+            p.launch()
+            err, fid = b.compare(ref, st64)
+            rec[name] = {"circuit_s": round(s_, 5), "speedup_vs_plain_c128": round(plain_s / s_, 2),
+                         "max_abs_err": err, "one_minus_fidelity": 1 - fid}
+            del p
+        log(f"learned planner at {nq2} qubits: {rec}")
+        sweep.append(rec)
+        st64.free()
+        ref.free()
+    learned_rec["size_sweep"] = sweep
+
     # layered(28) through the same forms: greedy / learned fusion on the matrix cores and
     # exact complex64 tile passes, each judged against the plain complex128 run
     ref = b.alloc(n_layered, rs.C128)
@@ -366,6 +410,7 @@ def main():
         "copy_kernel_gbs": round(copy_gbs, 1),
         "h_gate_gbs": {str(q): round(per_launch / (t * 1e-3) / 1e9, 1) for q, t in ms_h.items()},
         "tiled": tiled,
+        "sampling": sampling,
         "fused": fused_out,
         "mfma_roofline": mfma_roof,
         "layered28": {"plain_s": round(lay_s, 4), "plain_passes": lay_passes,

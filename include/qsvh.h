@@ -175,6 +175,14 @@ int qsvh_norm_sqr(qsvh_state* st, double* out);
 /* Backend::expectation_z (mod.rs:57-59, sample.rs:177-200): sum_i |psi_i|^2 *
  * (-1)^popcount(i & mask), f64 accumulation on the device. */
 int qsvh_expectation_z(qsvh_state* st, uint64_t mask, double* out);
+/* Backend::sample (mod.rs:52-54, which a device backend overrides "to sample on-device (no
+ * full copy back)"): `shots` outcomes in [0, 2^n) drawn from |psi_i|^2 on the device by
+ * inverse CDF as in cdf_sample (sample.rs:140-153): an f64 prefix sum of the probabilities,
+ * then for draw k of SplitMix64 started at state `seed` (the caller's rng.state; the stream
+ * advances by `shots` draws) u = unit() * total and outcome = min(first i with
+ * cdf_i >= u, 2^n - 1).  Writes `shots` u64 to host out.  Temporary device memory:
+ * 8 bytes per amplitude plus 8 per shot; at most 2^31 shots per call and 34 qubits. */
+int qsvh_sample(qsvh_state* st, uint64_t shots, uint64_t seed, uint64_t* out);
 /* Backend::download (mod.rs:61-62): the full SoA state to host re/im (2^n each). */
 int qsvh_download(qsvh_state* st, void* re, void* im);
 /* Host -> device copy of a full state.  No reference counterpart (the trait has no

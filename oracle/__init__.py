@@ -87,6 +87,10 @@ def lib():
         L.or_probabilities_f64.argtypes = [u64, P, P, P]
         L.or_expectation_z_f64.restype = f64
         L.or_expectation_z_f64.argtypes = [u64, P, u64]
+        L.or_prefix_sum_f64.argtypes = [u64, P, P]
+        L.or_cdf_sample.argtypes = [u64, P, u64, P, P]
+        L.or_alias_build.argtypes = [u64, P, P, P]
+        L.or_alias_sample.argtypes = [u64, P, P, u64, P, P]
         L.or_num_threads.restype = C.c_int
         L.or_num_threads.argtypes = []
         _lib = L
@@ -349,6 +353,39 @@ def probabilities(state):
 def expectation_z(probs, mask):
     p = np.ascontiguousarray(probs, dtype=np.float64)
     return float(lib().or_expectation_z_f64(len(p), _p(p), mask))
+
+
+def prefix_sum(p):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.empty_like(p)
+    lib().or_prefix_sum_f64(len(p), _p(p), _p(out))
+    return out
+
+
+def cdf_sample(p, shots, seed):
+    """cdf_sample (sample.rs:140-153) with SplitMix64(seed); returns (outcomes, end state)."""
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.empty(shots, dtype=np.uint64)
+    st = C.c_uint64(seed)
+    lib().or_cdf_sample(len(p), _p(p), shots, C.byref(st), _p(out))
+    return out, st.value
+
+
+def alias_build(p):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    prob = np.empty(len(p))
+    alias = np.empty(len(p), dtype=np.uint64)
+    lib().or_alias_build(len(p), _p(p), _p(prob), _p(alias))
+    return prob, alias
+
+
+def alias_sample(p, shots, seed):
+    """alias_sample (sample.rs:102-105) with SplitMix64(seed); returns (outcomes, end state)."""
+    prob, alias = alias_build(p)
+    out = np.empty(shots, dtype=np.uint64)
+    st = C.c_uint64(seed)
+    lib().or_alias_sample(len(prob), _p(prob), _p(alias), shots, C.byref(st), _p(out))
+    return out, st.value
 
 
 def num_threads():

@@ -777,3 +777,77 @@ double or_expectation_z_f64(uint64_t dim, const double* probs, uint64_t mask) {
   }
   return s;
 }
+
+/* ---- sampling (sample.rs:33-173) ------------------------------------------------- */
+void or_prefix_sum_f64(uint64_t dim, const double* p, double* out) {
+  double acc = 0.0;
+  for (uint64_t i = 0; i < dim; i++) {
+    acc += p[i];
+    out[i] = acc;
+  }
+}
+
+void or_cdf_sample(uint64_t dim, const double* p, uint64_t shots, uint64_t* state,
+                   uint64_t* out) {
+  double* cdf = (double*)malloc(sizeof(double) * (dim ? dim : 1));
+  or_prefix_sum_f64(dim, p, cdf);
+  const double total = dim ? cdf[dim - 1] : 0.0;
+  const uint64_t last = dim ? dim - 1 : 0;
+  for (uint64_t k = 0; k < shots; k++) {
+    const double u = or_sm64_unit(state) * total;
+    /* partition_point(|c| c < u): first index whose cdf is >= u */
+    uint64_t lo = 0, hi = dim;
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (cdf[mid] < u)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    out[k] = lo < last ? lo : last;
+  }
+  free(cdf);
+}
+
+void or_alias_build(uint64_t dim, const double* p, double* prob, uint64_t* alias) {
+  double total = 0.0;
+  for (uint64_t i = 0; i < dim; i++) total += p[i];
+  const double scale = total > 0.0 ? (double)dim / total : 0.0;
+  double* scaled = (double*)malloc(sizeof(double) * dim);
+  uint64_t* small = (uint64_t*)malloc(sizeof(uint64_t) * dim);
+  uint64_t* large = (uint64_t*)malloc(sizeof(uint64_t) * dim);
+  uint64_t ns = 0, nl = 0;
+  for (uint64_t i = 0; i < dim; i++) {
+    scaled[i] = p[i] * scale;
+    prob[i] = 0.0;
+    alias[i] = 0;
+    if (scaled[i] < 1.0)
+      small[ns++] = i;
+    else
+      large[nl++] = i;
+  }
+  while (ns > 0 && nl > 0) {
+    const uint64_t sm = small[--ns], lg = large[--nl];
+    prob[sm] = scaled[sm];
+    alias[sm] = lg;
+    scaled[lg] = (scaled[lg] + scaled[sm]) - 1.0;
+    if (scaled[lg] < 1.0)
+      small[ns++] = lg;
+    else
+      large[nl++] = lg;
+  }
+  while (nl > 0) prob[large[--nl]] = 1.0;
+  while (ns > 0) prob[small[--ns]] = 1.0;
+  free(scaled);
+  free(small);
+  free(large);
+}
+
+void or_alias_sample(uint64_t dim, const double* prob, const uint64_t* alias, uint64_t shots,
+                     uint64_t* state, uint64_t* out) {
+  for (uint64_t k = 0; k < shots; k++) {
+    uint64_t i = (uint64_t)(or_sm64_unit(state) * (double)dim);
+    if (i > dim - 1) i = dim - 1;
+    out[k] = or_sm64_unit(state) < prob[i] ? i : alias[i];
+  }
+}

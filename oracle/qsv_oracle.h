@@ -115,6 +115,19 @@ double or_norm_sqr_f64(uint64_t dim, const double* re, const double* im);
 void or_probabilities_f64(uint64_t dim, const double* re, const double* im, double* out);
 double or_expectation_z_f64(uint64_t dim, const double* probs, uint64_t mask);
 
+/* ---- sampling (crates/qsv-core/src/sample.rs:33-173) ---- */
+/* parallel_prefix_sum's serial path (sample.rs:114-160): out[i] = sum_{k<=i} p[k] in f64. */
+void or_prefix_sum_f64(uint64_t dim, const double* p, double* out);
+/* cdf_sample (sample.rs:140-153): draw k of SplitMix64(*state) gives u = unit() * total and
+ * outcome min(partition_point(cdf < u), dim - 1).  Advances *state by `shots` draws. */
+void or_cdf_sample(uint64_t dim, const double* p, uint64_t shots, uint64_t* state,
+                   uint64_t* out);
+/* AliasTable::build (sample.rs:38-82, Vose; the same small/large stack discipline) and
+ * draw (sample.rs:89-97: bucket from one unit(), accept test on the next). */
+void or_alias_build(uint64_t dim, const double* p, double* prob, uint64_t* alias);
+void or_alias_sample(uint64_t dim, const double* prob, const uint64_t* alias, uint64_t shots,
+                     uint64_t* state, uint64_t* out);
+
 /* Number of OpenMP threads the parallel paths use. */
 int or_num_threads(void);
 

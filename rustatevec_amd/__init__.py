@@ -100,6 +100,7 @@ def lib() -> C.CDLL:
         "qsvh_norm_sqr": (i32, [P, C.POINTER(C.c_double)]),
         "qsvh_expectation_z": (i32, [P, u64, C.POINTER(C.c_double)]),
         "qsvh_download": (i32, [P, P, P]),
+        "qsvh_sample": (i32, [P, u64, u64, P]),
         "qsvh_upload": (i32, [P, P, P]),
         "qsvh_synchronize": (i32, [P]),
         "qsvh_init_random_normal": (i32, [P, u64]),
@@ -442,6 +443,14 @@ class HipBackend:
         v = C.c_double(0)
         _check(lib().qsvh_expectation_z(state.handle, mask, C.byref(v)), self.ctx)
         return v.value
+
+    def sample(self, state: State, shots: int, seed: int):
+        """Backend::sample (crates/qsv-core/src/backend/mod.rs:52-54) on the device by
+        inverse CDF (sample.rs:140-153) with SplitMix64 started at `seed`; returns the
+        outcomes and the generator state after the `shots` draws."""
+        out = np.empty(shots, dtype=np.uint64)
+        _check(lib().qsvh_sample(state.handle, shots, seed & (2**64 - 1), _ptr(out)), self.ctx)
+        return out, (seed + shots * 0x9E3779B97F4A7C15) & (2**64 - 1)
 
     def download(self, state: State) -> np.ndarray:
         re = np.empty(1 << state.n_qubits, dtype=state.np_dtype)

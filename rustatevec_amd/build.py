@@ -29,7 +29,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",
 NO_SINK = ["-mllvm", "-simplifycfg-sink-common=false"]
 # Files whose kernels must not use scratch (private memory): checked from the compiler's
 # resource-usage remarks at every build.
-NO_SCRATCH = {"kernels_plain.hip", "kernels_mfma.hip", "kernels_tile.hip", "kernels_batch.hip"}
+NO_SCRATCH = {"kernels_plain.hip", "kernels_mfma.hip", "kernels_tile.hip", "kernels_batch.hip",
+              "kernels_sample.hip"}
 # file -> extra flags
 SOURCES = {
     "ir.cpp": ["-ffp-contract=off"],
@@ -43,6 +44,7 @@ SOURCES = {
     "kernels_batch.hip": ["-ffp-contract=off", *NO_SINK],
     "batch.cpp": [],
     "kernels_mlp.hip": [],
+    "kernels_sample.hip": ["-ffp-contract=off"],
     "mlp.cpp": [],
     "planner.cpp": ["-ffp-contract=off"],
 }

@@ -182,6 +182,13 @@ hipError_t launch_batch_reduce(const uint32_t* part_off, const uint32_t* n_part,
                                uint32_t n_circ, const double* partials, double* out,
                                hipStream_t s);
 
+// ---- kernels_sample.hip ----------------------------------------------------------------
+// Inverse-CDF sampling of `shots` outcomes with SplitMix64(seed) draws.  Buffers: local
+// (dim doubles), totals (sample_chunks(dim)), offset (sample_chunks(dim) + 1), out (shots).
+uint64_t sample_chunks(uint64_t dim);
+hipError_t launch_sample(const State& st, uint64_t shots, uint64_t seed, double* local,
+                         double* totals, double* offset, uint64_t* out, hipStream_t s);
+
 // ---- kernels_util.hip ------------------------------------------------------------------
 // Random normalised start state (counter-based SplitMix64 + Box-Muller); `d_norm` receives
 // the pre-normalisation <psi|psi>.

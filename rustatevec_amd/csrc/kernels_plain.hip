@@ -374,12 +374,18 @@ __global__ __launch_bounds__(kBlock) void k_copy(const T* __restrict__ src, T* _
 }
 
 // ------------------------------------------------------------------- host helpers
+// A dispatch holds fewer than 2^32 work-items (the packet's grid size is 32-bit), so
+// grids are capped at 2^31 work-items -- a multiple of 8 blocks, keeping the XCD order --
+// and the kernels' grid-stride loops cover the rest (complex128 at 32 qubits has 2^32
+// one-chunk items).
+constexpr uint64_t kMaxStreamBlocks = (uint64_t(1) << 31) / kBlock;
+
 unsigned grid_for(uint64_t items, int unroll, const Tuning& t) {
   const uint64_t per = uint64_t(kBlock) * uint64_t(unroll);
   uint64_t g = (items + per - 1) / per;
   if (t.stream_max_blocks > 0 && g > uint64_t(t.stream_max_blocks)) g = t.stream_max_blocks;
   if (g < 1) g = 1;
-  if (g > 0x7fffffffull) g = 0x7fffffffull;
+  if (g > kMaxStreamBlocks) g = kMaxStreamBlocks;
   return unsigned(g);
 }
 

@@ -277,3 +277,24 @@ def test_graph_replays_match_eager(backend):
     del pg, pe
     a.free()
     b.free()
+
+
+def test_32_qubit_state_launch_limits(backend):
+    """A complex128 state of 32 qubits has 2^32 one-chunk work items, one more than a
+    dispatch can hold: the streaming kernels must cap their grid and stride over the rest
+    (regression: init_basis failed with an invalid launch configuration)."""
+    n = 32
+    st = backend.alloc(n, rs.C128)
+    backend.init_basis(st, 0)
+    for q in (0, n - 1):
+        backend.apply(st, rs.gate_matrix("h"), [q])
+    backend.apply(st, rs.gate_matrix("cp", 0.5), [0, n - 1])
+    h = 0.5
+    for i, want in ((0, h), (1, h), (1 << (n - 1), h), ((1 << (n - 1)) + 1, h * np.exp(0.5j)),
+                    (2, 0.0), ((1 << n) - 1, 0.0)):
+        a = backend.amplitude(st, i)
+        assert abs(a - want) < 1e-15, (i, a)
+    assert abs(backend.norm_sqr(st) - 1.0) < 1e-14
+    out, _ = backend.sample(st, 4000, 3)
+    assert set(np.unique(out).tolist()) <= {0, 1, 1 << (n - 1), (1 << (n - 1)) + 1}
+    st.free()
