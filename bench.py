@@ -146,6 +146,21 @@ def main():
     reset()
     rs.Program(st, qft).launch()
 
+    # the same per-gate program with subspace launches: CP touches only the amplitudes
+    # whose phase is not 1 (a quarter), the rest as before; must equal the full passes
+    st_s = b.alloc(n, rs.C128)
+    reset_s = lambda: b.init_random_normal(st_s, seed)  # noqa: E731
+    prog_s = rs.Program(st_s, qft, subspace=True)
+    sub_s = timed_steps(b, st_s, prog_s, max(1, args.steps), 1, reset_s)
+    reset_s()
+    prog_s.launch()
+    err_s, _ = b.compare(st, st_s)
+    subspace = {"passes": prog_s.num_kernels(), "circuit_s": round(sub_s, 5),
+                "speedup_vs_plain": round(step_s / sub_s, 2), "max_abs_err": err_s}
+    del prog_s
+    st_s.free()
+    log(f"QFT({n}) per-gate with subspace launches: {subspace}")
+
     # on-device sampling of the QFT(30) output state (inverse CDF, 1e6 shots)
     shots = 1_000_000
     b.sample(st, shots, seed)
@@ -410,6 +425,7 @@ def main():
         "copy_kernel_gbs": round(copy_gbs, 1),
         "h_gate_gbs": {str(q): round(per_launch / (t * 1e-3) / 1e9, 1) for q, t in ms_h.items()},
         "tiled": tiled,
+        "plain_subspace": subspace,
         "sampling": sampling,
         "fused": fused_out,
         "mfma_roofline": mfma_roof,

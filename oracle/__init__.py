@@ -271,10 +271,16 @@ def run(backend, n, flat: Flat, re, im):
     return re, im
 
 
-def execute(backend, n, ops_or_recs, dtype=np.float64):
-    """Backend::execute (crates/qsv-core/src/backend/mod.rs:66-73): |0> then every op."""
+def execute(backend, n, ops_or_recs, dtype=np.float64, start=None):
+    """Backend::execute (crates/qsv-core/src/backend/mod.rs:66-73): |0> (or the given
+    start state) then every op."""
     flat = flatten(ops_or_recs)
-    re, im = basis(n, 0, dtype)
+    if start is None:
+        re, im = basis(n, 0, dtype)
+    else:
+        start = np.asarray(start)
+        re = np.ascontiguousarray(start.real, dtype=dtype).copy()
+        im = np.ascontiguousarray(start.imag, dtype=dtype).copy()
     run(backend, n, flat, re, im)
     return re + 1j * im
 

@@ -94,6 +94,10 @@ hipError_t issue(const Program& p, const Launch& l, const Tuning& t, hipStream_t
     case Launch::kTile:
       return launch_tile(st, p.tile_passes[l.arena_off], p.tile_ops, p.tile_diags, p.tile_pool,
                          t.tile_blocks, s);
+    case Launch::kCtrl1q:
+      return launch_ctrl1q(st, l.m, l.q, l.fix, l.mat.data(), t, s);
+    case Launch::kDiagSub:
+      return launch_diag_sub(st, l.m, l.q, l.fix, l.mat.data(), t, s);
     default:
       return hipErrorInvalidValue;
   }
@@ -168,6 +172,7 @@ int program_build(State* st, const Circuit& c, const qsvh_program_opts* opts, Pr
   if (rc != kOk) return rc;
   rc = plan_mfma(p);
   if (rc != kOk) return rc;
+  if (opts && opts->subspace) plan_subspace(p);
   return build_arena(p);
 }
 

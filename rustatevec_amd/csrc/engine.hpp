@@ -60,15 +60,19 @@ struct MqIndex {
   uint64_t off[64];
 };
 
-// One kernel launch of the plain executor.
+// One kernel launch of the plain executor.  kCtrl1q: 2x2 matrix `mat` on q[0], applied
+// where the state bits q[1..m) equal `fix`'s bits (a controlled 1-qubit gate on its control
+// subspace); kDiagSub: the phase mat[0..1] on the amplitudes whose bits q[0..m) equal
+// `fix`'s (the one non-unit entry of a diagonal gate).
 struct Launch {
-  enum Kind { k1q, kDiag, kMq, kFusedMfma, kTile } kind;
+  enum Kind { k1q, kDiag, kMq, kFusedMfma, kTile, kCtrl1q, kDiagSub } kind;
   uint32_t m = 0;
   uint32_t q[kMaxGateQubits] = {0, 0, 0, 0, 0, 0};
   // matrix in f64 (interleaved re, im), row-major; for diagonal launches the 2^m phases.
   std::vector<double> mat;
   uint64_t arena_off = 0;  // element offset of the matrix in the device arena
   uint64_t mfma_off = 0;   // byte offset of the block matrix in the program's MFMA buffer
+  uint64_t fix = 0;        // kCtrl1q / kDiagSub: required values of the fixed state bits
 };
 
 // ---- launchers (kernels_plain.hip) ---------------------------------------------------
@@ -82,6 +86,11 @@ hipError_t launch_diag(const State& st, uint32_t m, const uint32_t* q,
 // (interleaved re, im), needed for m >= 4 (smaller gates travel as kernel arguments).
 hipError_t launch_mq(const State& st, uint32_t m, const uint32_t* q, const double* g,
                      const void* dmat, const Tuning& t, hipStream_t s);
+// Subspace launches (kCtrl1q, kDiagSub; see Launch).
+hipError_t launch_ctrl1q(const State& st, uint32_t m, const uint32_t* q, uint64_t fix,
+                         const double* g, const Tuning& t, hipStream_t s);
+hipError_t launch_diag_sub(const State& st, uint32_t m, const uint32_t* q, uint64_t fix,
+                           const double* phase, const Tuning& t, hipStream_t s);
 hipError_t launch_probs(const State& st, void* out, hipStream_t s);
 // f64 sum of |psi_i|^2 * sign(i & mask) (mask 0: the norm) into *d_result.
 hipError_t launch_reduce(const State& st, uint64_t mask, double* d_partials, int max_partials,

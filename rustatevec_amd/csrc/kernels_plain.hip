@@ -373,6 +373,62 @@ __global__ __launch_bounds__(kBlock) void k_copy(const T* __restrict__ src, T* _
   }
 }
 
+// ---------------------------------------------------------------------- subspace
+// Zero bits inserted at up to six ascending positions (x + (x & ~(2^p - 1)) inserts a zero
+// at p; unused masks are 0), then the fixed bits set: the index of the x-th amplitude of a
+// subspace with some bits held at given values.
+struct SubIdx {
+  uint64_t insm[kMaxGateQubits];
+  uint64_t fix;
+};
+
+__device__ __forceinline__ uint64_t sub_index(uint64_t x, const SubIdx& a) {
+#pragma unroll
+  for (int j = 0; j < kMaxGateQubits; ++j) x += x & a.insm[j];
+  return x | a.fix;
+}
+
+// A controlled 1-qubit gate on its control subspace: only the pairs (lo, lo + 2^t) whose
+// control bits hold the required values are read and written; elsewhere the dense product
+// (mq_block, cpu.rs:221-247) returns each amplitude unchanged, and on the subspace its two
+// nonzero terms are the pair update's, in the same order.
+template <typename T, int A>
+__global__ __launch_bounds__(kBlock) void k_ctrl1q(T* __restrict__ amp, uint64_t n_items,
+                                                   uint32_t t, SubIdx a, G2<T> g, int xcd) {
+  const uint64_t half = 1ull << t;
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v = xcd_block(xcd) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+    const uint64_t lo = sub_index(v * A, a);
+    T ar[A], ai[A], br[A], bi[A];
+    cload<T, A>(amp, lo, ar, ai);
+    cload<T, A>(amp, lo + half, br, bi);
+#pragma unroll
+    for (int j = 0; j < A; ++j) pair_update(ar[j], ai[j], br[j], bi[j], g);
+    cstore<T, A>(amp, lo, ar, ai);
+    cstore<T, A>(amp, lo + half, br, bi);
+  }
+}
+
+// The one non-unit phase of a diagonal gate on its amplitudes (diag_chunk's multiply,
+// cpu.rs:99-114; the others would be multiplied by exactly 1 + 0i).
+template <typename T, int A>
+__global__ __launch_bounds__(kBlock) void k_diag_sub(T* __restrict__ amp, uint64_t n_items,
+                                                     SubIdx a, T dr, T di, int xcd) {
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v = xcd_block(xcd) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+    const uint64_t i = sub_index(v * A, a);
+    T xr[A], xi[A];
+    cload<T, A>(amp, i, xr, xi);
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+      const T x_r = xr[j], x_i = xi[j];
+      xr[j] = dr * x_r - di * x_i;
+      xi[j] = dr * x_i + di * x_r;
+    }
+    cstore<T, A>(amp, i, xr, xi);
+  }
+}
+
 // ------------------------------------------------------------------- host helpers
 // A dispatch holds fewer than 2^32 work-items (the packet's grid size is 32-bit), so
 // grids are capped at 2^31 work-items -- a multiple of 8 blocks, keeping the XCD order --
@@ -604,7 +660,74 @@ hipError_t launch_copy_t(const State& st, void* dst, const Tuning& t, hipStream_
   });
 }
 
+// Subspace of the m gate qubits q (any order) with their bits held at fix.
+SubIdx make_sub(uint32_t m, const uint32_t* q, uint64_t fix, uint32_t* lowest) {
+  uint32_t srt[kMaxGateQubits];
+  for (uint32_t j = 0; j < m; ++j) srt[j] = q[j];
+  for (uint32_t a = 1; a < m; ++a)
+    for (uint32_t b = a; b > 0 && srt[b - 1] > srt[b]; --b) {
+      const uint32_t x = srt[b];
+      srt[b] = srt[b - 1];
+      srt[b - 1] = x;
+    }
+  SubIdx s{};
+  for (uint32_t j = 0; j < kMaxGateQubits; ++j)
+    s.insm[j] = j < m ? ~((uint64_t(1) << srt[j]) - 1) : 0;
+  s.fix = fix;
+  *lowest = srt[0];
+  return s;
+}
+
+template <typename T>
+hipError_t launch_ctrl1q_t(const State& st, uint32_t m, const uint32_t* q, uint64_t fix,
+                           const double* gd, const Tuning& t, hipStream_t s) {
+  G2<T> g;
+  for (int e = 0; e < 4; ++e) {
+    g.r[e] = T(gd[2 * e]);
+    g.i[e] = T(gd[2 * e + 1]);
+  }
+  uint32_t lowest;
+  const SubIdx a = make_sub(m, q, fix, &lowest);
+  constexpr int A = amps_per_chunk<T>();
+  const uint64_t items = st.dim() >> m;
+  T* amp = static_cast<T*>(st.amp);
+  if (int(lowest) >= ilog2(A) && items >= uint64_t(A))
+    k_ctrl1q<T, A><<<grid_for(items / A, 1, t), kBlock, 0, s>>>(amp, items / A, q[0], a, g, t.xcd);
+  else
+    k_ctrl1q<T, 1><<<grid_for(items, 1, t), kBlock, 0, s>>>(amp, items, q[0], a, g, t.xcd);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_diag_sub_t(const State& st, uint32_t m, const uint32_t* q, uint64_t fix,
+                             const double* ph, const Tuning& t, hipStream_t s) {
+  uint32_t lowest;
+  const SubIdx a = make_sub(m, q, fix, &lowest);
+  constexpr int A = amps_per_chunk<T>();
+  const uint64_t items = st.dim() >> m;
+  T* amp = static_cast<T*>(st.amp);
+  if (int(lowest) >= ilog2(A) && items >= uint64_t(A))
+    k_diag_sub<T, A><<<grid_for(items / A, 1, t), kBlock, 0, s>>>(amp, items / A, a, T(ph[0]),
+                                                                 T(ph[1]), t.xcd);
+  else
+    k_diag_sub<T, 1><<<grid_for(items, 1, t), kBlock, 0, s>>>(amp, items, a, T(ph[0]), T(ph[1]),
+                                                             t.xcd);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_ctrl1q(const State& st, uint32_t m, const uint32_t* q, uint64_t fix,
+                         const double* g, const Tuning& t, hipStream_t s) {
+  return st.dtype == 0 ? launch_ctrl1q_t<double>(st, m, q, fix, g, t, s)
+                       : launch_ctrl1q_t<float>(st, m, q, fix, g, t, s);
+}
+
+hipError_t launch_diag_sub(const State& st, uint32_t m, const uint32_t* q, uint64_t fix,
+                           const double* phase, const Tuning& t, hipStream_t s) {
+  return st.dtype == 0 ? launch_diag_sub_t<double>(st, m, q, fix, phase, t, s)
+                       : launch_diag_sub_t<float>(st, m, q, fix, phase, t, s);
+}
 
 hipError_t launch_init_basis(const State& st, uint64_t basis, hipStream_t s) {
   return st.dtype == 0 ? launch_init_t<double>(st, basis, s) : launch_init_t<float>(st, basis, s);

@@ -515,6 +515,94 @@ int plan_tiles(Program* p, int tile_qubits) {
   return rc;
 }
 
+// Subspace forms of per-gate launches (qsvh_program_opts::subspace): a diagonal gate with
+// one entry other than exactly 1 + 0i touches only that entry's amplitudes; an m = 2, 3
+// gate that is a 1-qubit gate on one qubit under controls on the others (one control value
+// with a non-identity block, identity blocks elsewhere) touches only that control subspace.
+void plan_subspace(Program* p) {
+  for (Launch& l : p->launches) {
+    if (l.kind == Launch::kDiag) {
+      const uint32_t d = 1u << l.m;
+      int one = -1, count = 0;
+      for (uint32_t s = 0; s < d; ++s)
+        if (l.mat[2 * s] != 1.0 || l.mat[2 * s + 1] != 0.0) {
+          one = int(s);
+          ++count;
+        }
+      if (count != 1) continue;
+      uint64_t fix = 0;
+      for (uint32_t j = 0; j < l.m; ++j) fix |= uint64_t((uint32_t(one) >> j) & 1u) << l.q[j];
+      const double pr = l.mat[2 * one], pi = l.mat[2 * one + 1];
+      l.kind = Launch::kDiagSub;
+      l.fix = fix;
+      l.mat = {pr, pi};
+    } else if (l.kind == Launch::kMq && l.m <= 3) {
+      const uint32_t D = 1u << l.m;
+      auto nz = [&](uint32_t r, uint32_t c) {
+        const size_t e = 2 * (size_t(r) * D + c);
+        return l.mat[e] != 0.0 || l.mat[e + 1] != 0.0;
+      };
+      int target = -1;
+      bool ok = true;
+      for (uint32_t j = 0; j < l.m && ok; ++j) {
+        bool control = true;
+        for (uint32_t r = 0; r < D && control; ++r)
+          for (uint32_t c = 0; c < D; ++c)
+            if ((((r ^ c) >> j) & 1u) && nz(r, c)) {
+              control = false;
+              break;
+            }
+        if (!control) {
+          if (target >= 0) ok = false;  // two action qubits
+          target = int(j);
+        }
+      }
+      if (!ok || target < 0) continue;
+      // control values w (over the other bits, in gate order) with a non-identity block
+      int hit = -1, hits = 0;
+      std::vector<double> blk(8);
+      for (uint32_t w = 0; w < (D >> 1); ++w) {
+        auto idx = [&](uint32_t bit) {  // sub-index with target bit `bit`, controls = w
+          const uint32_t lowmask = (1u << target) - 1u;
+          return ((w & ~lowmask) << 1) | (w & lowmask) | (bit << target);
+        };
+        bool ident = true;
+        for (uint32_t r = 0; r < 2; ++r)
+          for (uint32_t c = 0; c < 2; ++c) {
+            const size_t e = 2 * (size_t(idx(r)) * D + idx(c));
+            if (l.mat[e] != (r == c ? 1.0 : 0.0) || l.mat[e + 1] != 0.0) ident = false;
+          }
+        if (!ident) {
+          hit = int(w);
+          ++hits;
+          for (uint32_t r = 0; r < 2; ++r)
+            for (uint32_t c = 0; c < 2; ++c) {
+              const size_t e = 2 * (size_t(idx(r)) * D + idx(c));
+              blk[2 * (2 * r + c)] = l.mat[e];
+              blk[2 * (2 * r + c) + 1] = l.mat[e + 1];
+            }
+        }
+      }
+      if (hits != 1) continue;
+      uint32_t q[kMaxGateQubits];
+      uint64_t fix = 0;
+      q[0] = l.q[target];
+      uint32_t k = 1;
+      const uint32_t lowmask = (1u << target) - 1u;
+      const uint32_t full = ((uint32_t(hit) & ~lowmask) << 1) | (uint32_t(hit) & lowmask);
+      for (uint32_t j = 0; j < l.m; ++j) {
+        if (int(j) == target) continue;
+        q[k++] = l.q[j];
+        fix |= uint64_t((full >> j) & 1u) << l.q[j];
+      }
+      std::copy(q, q + l.m, l.q);
+      l.kind = Launch::kCtrl1q;
+      l.fix = fix;
+      l.mat = blk;
+    }
+  }
+}
+
 int plan_mfma(Program* p) {
   if (p->precision == QSVH_PREC_EXACT) return kOk;
   if (p->precision != QSVH_PREC_MFMA_F32 && p->precision != QSVH_PREC_MFMA_BF16X3)

@@ -65,6 +65,7 @@ void schedule_tiles(const std::vector<Launch>& in, uint32_t n, uint32_t tb, Tile
 
 // Passes that rewrite the plain launch list (plan.cpp).
 int plan_tiles(Program* p, int tile_qubits);
+void plan_subspace(Program* p);
 int plan_mfma(Program* p);
 void release_extra(Program* p);
 

@@ -298,3 +298,45 @@ def test_32_qubit_state_launch_limits(backend):
     out, _ = backend.sample(st, 4000, 3)
     assert set(np.unique(out).tolist()) <= {0, 1, 1 << (n - 1), (1 << (n - 1)) + 1}
     st.free()
+
+
+def _run_program(backend, circ, dtype, **opts):
+    st = backend.alloc(circ.n_qubits, dtype)
+    backend.init_basis(st, 0)
+    p = rs.Program(st, circ, **opts)
+    p.launch()
+    out = backend.download(st)
+    k = p.num_kernels()
+    del p
+    st.free()
+    return out, k
+
+
+@pytest.mark.parametrize("dtype", [rs.C128, rs.C64])
+def test_subspace_launches_match_the_full_passes(backend, dtype):
+    """Program option `subspace`: controlled 1-qubit gates on their control subspace and
+    diagonal gates with one non-unit phase on that phase's amplitudes give the same values
+    as the full passes (bit for bit against the oracle's CpuBackend), with the same number
+    of launches."""
+    npdt = np.float64 if dtype == rs.C128 else np.float32
+    circs = [rs.qft(11), rs.ghz_chain(12), rs.random_circuit(12, 300, 3),
+             rs.mix(12, 400, 4), rs.layered(10, 4, 5)]
+    c = rs.Circuit(9)
+    for q in range(9):
+        c.h(q)
+    u = rs.gate_matrix("u3", 0.3, 0.5, 0.7)
+    cu0 = np.eye(8, dtype=np.complex128)  # U on qubit 2 when qubit 0 = 1 and qubit 5 = 0
+    for r in range(2):
+        for cc in range(2):
+            cu0[(r << 2) | 1, (cc << 2) | 1] = u[r, cc]
+    c.push(cu0, [0, 5, 2])
+    c.push(np.diag([1, 1j, 1, 1]), [3, 7])  # one non-unit entry at sub-index 1
+    c.ccx(8, 1, 0).ccz(2, 6, 7).cx(4, 0).cz(1, 3).cphase(0, 8, 0.4)
+    circs.append(c)
+    for circ in circs:
+        n = circ.n_qubits
+        got, k_sub = _run_program(backend, circ, dtype, subspace=True)
+        _, k_full = _run_program(backend, circ, dtype)
+        want = oracle.execute(oracle.CPU_PARALLEL, n, circ.ops(), dtype=npdt)
+        assert max_abs_diff(got, want) == 0.0, n
+        assert k_sub == k_full

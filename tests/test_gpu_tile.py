@@ -138,3 +138,66 @@ def test_bad_tile_size_is_an_error(backend):
         rs.Program(st, rs.qft(14), tile_qubits=13)
     assert e.value.status == "INVALID_ARG"
     st.free()
+
+
+def _perm_matrix(perm, phases):
+    d = len(perm)
+    u = np.zeros((d, d), dtype=np.complex128)
+    for r, c in enumerate(perm):
+        u[r, c] = phases[r]
+    return u
+
+
+def test_lowering_forms(backend):
+    """Every form the tile lowering produces, bit for bit against the oracle from a random
+    start: real and complex dense 1q ops, anti-diagonal (X, Y) and swap pair variants,
+    controls inside and outside the tile as conditions, register blocks of 1q runs,
+    permutation blocks with unit and non-unit coefficients (SWAP, CCX, custom phased 2q/3q
+    permutations), real dense 2q blocks, and diagonal blocks of non-diagonal gates."""
+    rng = np.random.default_rng(5)
+    for n, tile, dtype in ((13, 10, rs.C128), (14, 12, rs.C128), (15, 13, rs.C64),
+                           (12, 12, rs.C128)):
+        c = rs.Circuit(n)
+        for rep in range(3):
+            for q in range(n):
+                c.h(q)               # real dense
+                c.ry(q, 0.3 + q)     # real dense
+                c.x((q + rep) % n)   # swap variant
+                c.y((q + 2) % n)     # anti-diagonal with phases
+                c.u3(q, 0.1 * q, 0.2, 0.3 + rep)  # complex dense, 1q runs -> register blocks
+            for q in range(0, n - 1, 2):
+                c.cx(q, q + 1)       # control inside the tile (condition)
+                if n - 1 - q != q:
+                    c.cx(n - 1 - q, q)   # control often outside the tile
+                c.swap(q, (q + 3) % n)
+            c.ccx(0, 1, 2)
+            c.ccx(n - 1, n - 2, 3)
+            c.ccz(1, 4, n - 1)
+            # custom permutations with phases
+            ph2 = np.exp(1j * rng.uniform(0, 2 * np.pi, 4))
+            c.push(_perm_matrix([2, 0, 3, 1], ph2), [1, 5])
+            c.push(_perm_matrix([1, 0, 2, 3], [1, 1, 1, 1j]), [2, 3])
+            ph3 = np.exp(1j * rng.uniform(0, 2 * np.pi, 8))
+            c.push(_perm_matrix([3, 6, 1, 0, 7, 2, 5, 4], ph3), [0, 4, 6])
+            # real dense 2q (not a permutation), and a 2q gate diagonal in one qubit only
+            hh = np.kron([[1, 1], [1, -1]], [[1, 1], [1, -1]]) / 2.0
+            c.push(hh, [3, 7])
+            blk = np.zeros((4, 4), dtype=np.complex128)
+            blk[:2, :2] = np.diag([1j, -1])
+            blk[2:, 2:] = [[0.6, 0.8], [-0.8, 0.6]]
+            c.push(blk, [2, 9])      # bit 0 (qubit 2) acts, bit 1 (qubit 9) controls
+        ops = c.ops()
+        st = backend.alloc(n, dtype)
+        backend.init_random_normal(st, n)
+        start = backend.download(st)
+        p = rs.Program(st, c, tile_qubits=tile)
+        p.launch()
+        got = backend.download(st)
+        order = order_of(p)
+        del p
+        st.free()
+        assert sorted(order) == list(range(len(ops)))
+        npdt = np.float64 if dtype == rs.C128 else np.float32
+        want = oracle.execute(oracle.CPU_PARALLEL, n, [ops[i] for i in order], dtype=npdt,
+                              start=start)
+        assert max_abs_diff(got, want) == 0.0, (n, tile)
