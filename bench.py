@@ -45,7 +45,7 @@ def log(msg):
 def measured_traffic():
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE scaled by the copy kernel), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_k_diag_30q_c128.md")
+    path = os.path.join(ROOT, "profiles", "traffic_k_diag_small_30q_c128.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -420,7 +420,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic() if n == 30 else None,
-                     "kernel": f"k_diag (controlled phase, {n}q c128)",
+                     "kernel": f"k_diag_small (controlled phase, {n}q c128)",
                      "bytes_per_launch": per_launch, "ms_per_launch": round(ms_cp, 4)},
         "copy_kernel_gbs": round(copy_gbs, 1),
         "h_gate_gbs": {str(q): round(per_launch / (t * 1e-3) / 1e9, 1) for q, t in ms_h.items()},

@@ -87,7 +87,8 @@ int qsvh_abi_version(void);
 /* Launch-shape knobs of the streaming (1q, diagonal, copy) kernels, per context:
  * "max_blocks" (0: one thread per work item; n > 0: grid-stride loop over n blocks),
  * "unroll" (1, 2 or 4 independent 16-byte items per thread per iteration) and
- * "nontemporal" (0/1: streaming load/store hints), "contig" (0: a thread's unrolled items
+ * "nontemporal" (0/1: streaming load/store hints; 2: only for states larger than the
+ * 256 MiB infinity cache), "contig" (0: a thread's unrolled items
  * are a grid apart; 1: adjacent), "xcd" (1: consecutive blocks of a streaming kernel take
  * neighbouring address ranges within one XCD; 0: launch order); "mfma_blocks" and
  * "tile_blocks" (workgroups of the persistent matrix-core group and tile-pass kernels).

@@ -301,7 +301,7 @@ int qsvh_tune_set(qsvh_ctx* h, const char* key, int64_t value) {
     t.stream_max_blocks = int(value);
   } else if (k == "unroll" && (value == 1 || value == 2 || value == 4)) {
     t.unroll = int(value);
-  } else if (k == "nontemporal" && (value == 0 || value == 1)) {
+  } else if (k == "nontemporal" && value >= 0 && value <= 2) {
     t.nontemporal = int(value);
   } else if (k == "mfma_blocks" && value >= 1 && value <= (1ll << 20)) {
     t.mfma_blocks = int(value);

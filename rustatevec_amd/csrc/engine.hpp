@@ -17,7 +17,8 @@ struct Tuning {
   int stream_block = 256;      // threads per block for the streaming kernels
   int stream_max_blocks = 0;   // 0: one thread per work item; >0: grid-stride cap
   int unroll = 1;              // independent vector items per thread per iteration
-  int nontemporal = 0;         // 1: non-temporal (streaming) loads and stores
+  int nontemporal = 2;         // 1: non-temporal (streaming) loads and stores; 2: where
+                               //    they pay (streaming kernels, states beyond the MALL)
   int mfma_blocks = 2048;      // workgroups of the persistent MFMA group kernels
   int tile_blocks = 1024;      // workgroups of the persistent tile-pass kernel
   int contig = 0;              // 1: a thread's unrolled items are adjacent (else strided)

@@ -164,7 +164,39 @@ __global__ __launch_bounds__(kBlock) void k_diag(T* __restrict__ amp, uint64_t n
   }
 }
 
-// ---------------------------------------------------------------------- m-qubit
+// Diagonal gates on 1 or 2 qubits (RZ, CP, CZ, RZZ: almost every diagonal launch) take
+// their phases as scalar arguments: no LDS table, no barrier, nothing per block but the
+// loads.  Same multiply as k_diag.
+template <typename T>
+struct Diag4 {
+  T r[4], i[4];
+};
+
+template <typename T, int A, int M, bool NT>
+__global__ __launch_bounds__(kBlock) void k_diag_small(T* __restrict__ amp, uint64_t n_items,
+                                                       uint32_t q0, uint32_t q1, Diag4<T> d,
+                                                       int xcd) {
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v = xcd_block(xcd) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+    T xr[A], xi[A];
+    cload<T, A, NT>(amp, v * A, xr, xi);
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+      const uint64_t i = v * A + j;
+      const uint32_t b0 = uint32_t(i >> q0) & 1u;
+      const uint32_t b1 = M == 2 ? uint32_t(i >> q1) & 1u : 0u;
+      const T lo_r = b0 ? d.r[1] : d.r[0], lo_i = b0 ? d.i[1] : d.i[0];
+      const T hi_r = b0 ? d.r[3] : d.r[2], hi_i = b0 ? d.i[3] : d.i[2];
+      const T dr = b1 ? hi_r : lo_r, di = b1 ? hi_i : lo_i;
+      const T x_r = xr[j], x_i = xi[j];
+      xr[j] = dr * x_r - di * x_i;
+      xi[j] = dr * x_i + di * x_r;
+    }
+    cstore<T, A, NT>(amp, v * A, xr, xi);
+  }
+}
+
+// ---------------------------------------------------------------------- m-qubit// ---------------------------------------------------------------------- m-qubit
 // insert_zero_bits over strictly ascending positions (layout.rs:70-87).
 template <int M>
 __device__ __forceinline__ uint64_t insert_zero_bits(uint64_t x, const uint32_t* sorted) {
@@ -445,11 +477,21 @@ unsigned grid_for(uint64_t items, int unroll, const Tuning& t) {
   return unsigned(g);
 }
 
+// Non-temporal accesses: knob 0 never, 1 always, 2 (auto) where they were measured to pay
+// on MI355X -- for a kernel that touches each cache line once (`streams`) and a state
+// larger than the 256 MiB infinity cache.  A smaller state stays cache-resident between
+// gates, which non-temporal accesses forfeit; a 1q gate whose pair halves share 128-byte
+// lines, and the LDS-table diagonal, ran slower with them.
+inline bool nontemporal_for(const State& st, const Tuning& t, bool streams) {
+  if (t.nontemporal == 2) return streams && st.bytes() > (size_t(256) << 20);
+  return t.nontemporal != 0;
+}
+
 // Call f(unroll, nontemporal) with the tuning's values as compile-time constants.
 template <typename F>
-hipError_t with_shape(const Tuning& t, F&& f) {
+hipError_t with_shape(const State& st, const Tuning& t, bool streams, F&& f) {
   using std::integral_constant;
-  const bool nt = t.nontemporal != 0;
+  const bool nt = nontemporal_for(st, t, streams);
   switch (t.unroll) {
     case 1:
       return nt ? f(integral_constant<int, 1>{}, std::true_type{})
@@ -470,7 +512,8 @@ hipError_t launch_1q_t(const State& st, uint32_t q, const G2<T>& g, const Tuning
                        hipStream_t s) {
   const uint64_t dim = st.dim();
   T* amp = static_cast<T*>(st.amp);
-  return with_shape(t, [&](auto uc, auto ntc) {
+  const bool streams = (uint64_t(2 * st.elem()) << q) >= 128;  // pair stride, bytes
+  return with_shape(st, t, streams, [&](auto uc, auto ntc) {
     constexpr int U = decltype(uc)::value;
     constexpr bool NT = decltype(ntc)::value;
     if (int(q) >= ilog2(A)) {
@@ -511,7 +554,27 @@ hipError_t launch_diag_any(const State& st, uint32_t m, const uint32_t* q, const
   }
   T* amp = static_cast<T*>(st.amp);
   constexpr int A = amps_per_chunk<T>();
-  return with_shape(t, [&](auto uc, auto ntc) {
+  if (m <= 2 && st.dim() >= uint64_t(A)) {
+    Diag4<T> d4{};
+    for (uint32_t e = 0; e < 4; ++e) {
+      d4.r[e] = a.dr[e < (1u << m) ? e : 0];
+      d4.i[e] = a.di[e < (1u << m) ? e : 0];
+    }
+    const uint64_t items = st.dim() / A;
+    const uint32_t q1 = m == 2 ? q[1] : 0;
+    const dim3 grid = grid_for(items, 1, t);
+    const bool nt = nontemporal_for(st, t, true);
+    if (m == 1 && nt)
+      k_diag_small<T, A, 1, true><<<grid, kBlock, 0, s>>>(amp, items, q[0], q1, d4, t.xcd);
+    else if (m == 1)
+      k_diag_small<T, A, 1, false><<<grid, kBlock, 0, s>>>(amp, items, q[0], q1, d4, t.xcd);
+    else if (nt)
+      k_diag_small<T, A, 2, true><<<grid, kBlock, 0, s>>>(amp, items, q[0], q1, d4, t.xcd);
+    else
+      k_diag_small<T, A, 2, false><<<grid, kBlock, 0, s>>>(amp, items, q[0], q1, d4, t.xcd);
+    return hipGetLastError();
+  }
+  return with_shape(st, t, false, [&](auto uc, auto ntc) {
     constexpr int U = decltype(uc)::value;
     constexpr bool NT = decltype(ntc)::value;
     if (st.dim() >= uint64_t(A)) {
@@ -646,7 +709,7 @@ hipError_t launch_copy_t(const State& st, void* dst, const Tuning& t, hipStream_
   constexpr int A = amps_per_chunk<T>();
   const T* src = static_cast<const T*>(st.amp);
   T* d = static_cast<T*>(dst);
-  return with_shape(t, [&](auto uc, auto ntc) {
+  return with_shape(st, t, true, [&](auto uc, auto ntc) {
     constexpr int U = decltype(uc)::value;
     constexpr bool NT = decltype(ntc)::value;
     if (st.dim() >= uint64_t(A)) {

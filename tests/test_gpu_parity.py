@@ -300,6 +300,33 @@ def test_32_qubit_state_launch_limits(backend):
     st.free()
 
 
+@pytest.mark.parametrize("dtype,n", [(rs.C128, 25), (rs.C64, 26)])
+def test_nontemporal_knob_is_bitwise_neutral(backend, dtype, n):
+    """States beyond the 256 MiB infinity cache take non-temporal loads and stores in the
+    streaming kernels by default ("nontemporal" 2); the hints change no arithmetic, so the
+    1q (pair halves sharing a cache line or not), diagonal (1, 2 and 3 qubits) and cx
+    results are bitwise those of the plain accesses (0) and of hints everywhere (1)."""
+    gates = [("h", [0]), ("h", [2]), ("h", [3]), ("h", [4]), ("h", [n - 1]),
+             ("rz", [3], 0.4), ("cp", [1, n - 2], 0.7), ("ccz", [0, 9, n - 1]),
+             ("cx", [n - 1, 5]), ("h", [12])]
+    outs = []
+    old = backend.tune()["nontemporal"]
+    try:
+        for nt in (0, 1, 2):
+            backend.tune(nontemporal=nt)
+            st = backend.alloc(n, dtype)
+            backend.init_random_normal(st, 5)
+            for g in gates:
+                backend.apply(st, rs.gate_matrix(g[0], *g[2:]), g[1])
+            outs.append(backend.download(st))
+            st.free()
+    finally:
+        backend.tune(nontemporal=old)
+    assert np.all(np.isfinite(outs[0]))
+    for got in outs[1:]:
+        assert np.array_equal(got, outs[0])
+
+
 def _run_program(backend, circ, dtype, **opts):
     st = backend.alloc(circ.n_qubits, dtype)
     backend.init_basis(st, 0)
