@@ -132,6 +132,32 @@ def test_apply_every_target_and_size(backend):
         st.free()
 
 
+@pytest.mark.parametrize("dtype", [rs.C128, rs.C64])
+def test_1q_lane_and_pair_kernels_agree_on_every_target(backend, dtype):
+    """The default 1q kernel pairs lanes 0..31 with 32..63 of a wave (one chunk per lane);
+    "unroll" 2 selects the pair-per-thread kernel.  Both must give the oracle's result,
+    bitwise, for every target, on states from one wave of chunks upward."""
+    npt = np.float64 if dtype == rs.C128 else np.float32
+    rng = np.random.default_rng(3)
+    old = backend.tune()["unroll"]
+    try:
+        for n in (6, 7, 8, 13, 18):
+            psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)) * 0.01
+            ops = [(oracle.gate_matrix("u3", [0.3 + q, 1.1, -0.4 * q]), [q]) for q in range(n)]
+            want = oracle.execute(oracle.CPU_SERIAL, n, ops, dtype=npt, start=psi)
+            for unroll in (1, 2):
+                backend.tune(unroll=unroll)
+                st = backend.alloc(n, dtype)
+                backend.upload(st, psi)
+                for g, qs in ops:
+                    backend.apply(st, g, qs)
+                got = backend.download(st)
+                st.free()
+                assert np.array_equal(got, want), (n, unroll)
+    finally:
+        backend.tune(unroll=old)
+
+
 def test_upload_download_roundtrip_and_qft_from_random_start(backend, golden):
     z, _ = golden
     v = z["start_random7"]
