@@ -49,7 +49,8 @@ typedef enum qsvh_status {
   QSVH_ERR_OOM = 6,             /* device (or host) allocation failed                 */
   QSVH_ERR_HIP = 7,             /* a HIP runtime call failed; see qsvh_last_error     */
   QSVH_ERR_DTYPE = 8,           /* operation not available for this state dtype       */
-  QSVH_ERR_NO_DEVICE = 9        /* no HIP device with that ordinal                    */
+  QSVH_ERR_NO_DEVICE = 9,       /* no HIP device with that ordinal                    */
+  QSVH_ERR_PARSE = 10           /* OpenQASM text rejected; see the message buffer     */
 } qsvh_status;
 
 typedef enum qsvh_dtype { QSVH_C128 = 0, QSVH_C64 = 1 } qsvh_dtype;
@@ -136,6 +137,17 @@ int qsvh_circuit_layered(uint32_t n, uint32_t depth, uint64_t seed, qsvh_circuit
 int qsvh_circuit_hea(uint32_t n, uint32_t depth, uint64_t seed, qsvh_circuit** out);
 /* Random gate mix: 50% U3, 30% CX/CZ, 15% RZZ, 5% CCX/CCZ on random qubits. */
 int qsvh_circuit_mix(uint32_t n, uint64_t n_gates, uint64_t seed, qsvh_circuit** out);
+
+/* OpenQASM 3 (or 2.0) program text -> circuit: the QASM3 front end the reference's CLI
+ * names as future work (crates/qsv-cli/src/main.rs:1-2, README.md:63).  Registers are
+ * concatenated in declaration order (q[0] of the first register = qubit 0, the least
+ * significant index bit); operands map to the gate's qubits in order, control first
+ * (gate.rs:8-12).  Standard gates (stdgates.inc / qelib1.inc names), gate definitions,
+ * inv / pow(int) / ctrl / negctrl modifiers, gphase, broadcast and ranges are read;
+ * barrier and terminal measure are skipped; anything else is QSVH_ERR_PARSE with
+ * "line L: reason" in err (NUL-terminated, truncated to err_cap; err may be NULL). */
+int qsvh_circuit_from_qasm(const char* text, size_t len, qsvh_circuit** out, char* err,
+                           size_t err_cap);
 
 /* fusion::fuse (crates/qsv-core/src/fusion.rs:39-77) with FusionConfig{max_qubits}:
  * greedy consecutive grouping on the host; every multi-op group's matrix is composed in

@@ -29,7 +29,7 @@ PREC_EXACT, PREC_MFMA_F32, PREC_MFMA_BF16X3 = 0, 1, 2
 
 STATUS = {
     0: "OK", 1: "INVALID_ARG", 2: "INVALID_QUBIT", 3: "REPEATED_QUBIT", 4: "ARITY",
-    5: "TOO_WIDE", 6: "OOM", 7: "HIP_ERROR", 8: "DTYPE", 9: "NO_DEVICE",
+    5: "TOO_WIDE", 6: "OOM", 7: "HIP_ERROR", 8: "DTYPE", 9: "NO_DEVICE", 10: "PARSE",
 }
 
 
@@ -76,6 +76,7 @@ def lib() -> C.CDLL:
         "qsvh_circuit_destroy": (None, [P]),
         "qsvh_circuit_push": (i32, [P, u32, P, P, P]),
         "qsvh_circuit_push_gate": (i32, [P, C.c_char_p, P, P, u32]),
+        "qsvh_circuit_from_qasm": (i32, [C.c_char_p, C.c_size_t, P, C.c_char_p, C.c_size_t]),
         "qsvh_circuit_n_qubits": (u32, [P]),
         "qsvh_circuit_len": (C.c_size_t, [P]),
         "qsvh_circuit_op": (i32, [P, C.c_size_t, C.POINTER(C.c_uint32), P, P, P]),
@@ -259,6 +260,24 @@ def _gen(fn: str, *args) -> Circuit:
     h = C.c_void_p()
     _check(getattr(lib(), fn)(*args, C.byref(h)))
     return Circuit(_handle=h)
+
+
+def from_qasm(text: str) -> Circuit:
+    """An OpenQASM 3 (or 2.0) program as a circuit (qsvh_circuit_from_qasm); raises
+    QsvhError(10, "line L: reason") for what the reader does not accept."""
+    raw = text.encode()
+    h = C.c_void_p()
+    err = C.create_string_buffer(512)
+    rc = lib().qsvh_circuit_from_qasm(raw, len(raw), C.byref(h), err, len(err))
+    if rc != 0:
+        raise QsvhError(rc, err.value.decode())
+    return Circuit(_handle=h)
+
+
+def read_qasm(path: str) -> Circuit:
+    """from_qasm of a file's contents."""
+    with open(path, encoding="utf-8") as f:
+        return from_qasm(f.read())
 
 
 def ghz(n: int) -> Circuit:

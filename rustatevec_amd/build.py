@@ -34,6 +34,7 @@ NO_SCRATCH = {"kernels_plain.hip", "kernels_mfma.hip", "kernels_tile.hip", "kern
 # file -> extra flags
 SOURCES = {
     "ir.cpp": ["-ffp-contract=off"],
+    "qasm.cpp": ["-ffp-contract=off"],
     "api.cpp": [],
     "plan.cpp": [],
     "kernels_plain.hip": ["-ffp-contract=off"],

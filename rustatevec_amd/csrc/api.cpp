@@ -381,6 +381,28 @@ int qsvh_circuit_push_gate(qsvh_circuit* c, const char* name, const uint32_t* qu
   return push_gate(&c->c, name, qubits, params, n_params);
 }
 
+int qsvh_circuit_from_qasm(const char* text, size_t len, qsvh_circuit** out, char* err,
+                           size_t err_cap) {
+  if (err && err_cap) err[0] = 0;
+  if (!text || !out) return kInvalidArg;
+  *out = nullptr;
+  std::string msg;
+  auto* c = new (std::nothrow) qsvh_circuit;
+  if (!c) return kOom;
+  const int st = parse_qasm(std::string(text, len), &c->c, &msg);
+  if (st != kOk) {
+    delete c;
+    if (err && err_cap) {
+      const size_t k = std::min(msg.size(), err_cap - 1);
+      std::memcpy(err, msg.data(), k);
+      err[k] = 0;
+    }
+    return st;
+  }
+  *out = c;
+  return kOk;
+}
+
 uint32_t qsvh_circuit_n_qubits(const qsvh_circuit* c) { return c ? c->c.n : 0; }
 size_t qsvh_circuit_len(const qsvh_circuit* c) { return c ? c->c.ops.size() : 0; }
 

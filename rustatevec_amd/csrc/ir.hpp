@@ -40,6 +40,7 @@ enum Status : int {
   kHip = 7,
   kDtype = 8,
   kNoDevice = 9,
+  kParse = 10,
 };
 
 // Circuit::push validation (crates/qsv-core/src/circuit.rs:54-78) as a status code.
@@ -87,5 +88,9 @@ struct Group {
   uint32_t q[kMaxGateQubits] = {0, 0, 0, 0, 0, 0};
 };
 std::vector<Group> greedy_groups(const Circuit& c, uint32_t max_qubits);
+
+// OpenQASM 3 / 2.0 program text -> circuit (qasm.cpp).  kParse with "line L: reason" in
+// *err on failure.
+int parse_qasm(const std::string& text, Circuit* out, std::string* err);
 
 }  // namespace qsvh
