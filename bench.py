@@ -55,6 +55,12 @@ def measured_traffic():
     return None
 
 
+def dense_group_6():
+    """A Haar-like dense 64 x 64 unitary (QR of a complex normal matrix)."""
+    return np.linalg.qr(np.random.default_rng(1).normal(size=(64, 64))
+                        + 1j * np.random.default_rng(2).normal(size=(64, 64)))[0]
+
+
 def timed_steps(backend, state, prog, steps, warmup, reset=None):
     """Mean wall seconds per launch; `reset` (untimed) restores the start state."""
     total = 0.0
@@ -96,6 +102,9 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only 5 launches each of the dominant kernel (controlled-phase "
                          "diagonal, 30q c128) and of the copy kernel, for counter runs")
+    ap.add_argument("--mfma-only", action="store_true",
+                    help="only 5 launches each of one dense 6-qubit group on a 30q c64 "
+                         "state in FP32 and in split-BF16x3 MFMA, for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -112,6 +121,21 @@ def main():
         ms_copy = b.time_copy(st, 4)
         print(json.dumps({"diag_cp_ms": ms_cp, "copy_ms": ms_copy,
                           "bytes_per_launch": (1 << 30) * 32}), flush=True)
+        return
+
+    if args.mfma_only:
+        b = rs.HipBackend(0)
+        st = b.alloc(30, rs.C64)
+        b.init_random_normal(st, 1)
+        g6 = rs.Circuit(30).push(dense_group_6(), [3, 9, 14, 20, 25, 29])
+        rec = {}
+        for prec, name in ((rs.PREC_MFMA_F32, "f32"), (rs.PREC_MFMA_BF16X3, "bf16x3")):
+            p = rs.Program(st, g6, precision=prec)
+            rec[name + "_ms"] = [round(p.time(), 4) for _ in range(5)]
+            del p
+        print(json.dumps(rec), flush=True)
+        st.free()
+        b.close()
         return
 
     n = 24 if args.quick else 30
@@ -209,9 +233,7 @@ def main():
             log(f"QFT({n}) fused k={k} {name}: {rec}")
             del p
     # matrix-core roofline: one dense 6-qubit group, FP32 MFMA, on the 30q c64 state
-    U = np.linalg.qr(np.random.default_rng(1).normal(size=(64, 64))
-                     + 1j * np.random.default_rng(2).normal(size=(64, 64)))[0]
-    g6 = rs.Circuit(n).push(U, [3, 9, 14, 20, 25, 29])
+    g6 = rs.Circuit(n).push(dense_group_6(), [3, 9, 14, 20, 25, 29])
     for prec, peak, name in ((rs.PREC_MFMA_F32, FP32_MFMA_PEAK_TFLOPS, "f32"),
                              (rs.PREC_MFMA_BF16X3, BF16_MFMA_PEAK_TFLOPS, "bf16x3")):
         p = rs.Program(st64, g6, precision=prec)
