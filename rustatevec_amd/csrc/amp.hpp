@@ -54,6 +54,18 @@ __device__ __forceinline__ void cstore(T* __restrict__ amp, uint64_t i, const T 
     *p = v;
 }
 
+// One 16-byte vector (4 floats) at p; NT: non-temporal.
+template <bool NT, typename V>
+__device__ __forceinline__ V vload(const V* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename V>
+__device__ __forceinline__ void vstore(V* p, const V& v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // Block index in launch order remapped so that each XCD's blocks cover one contiguous
 // range of the grid (the hardware deals block b to XCD b % 8); identity when the grid is
 // not a multiple of 8 or `on` is 0.

@@ -90,7 +90,7 @@ hipError_t issue(const Program& p, const Launch& l, const Tuning& t, hipStream_t
       return launch_mq(st, l.m, l.q, l.mat.data(), dmat, t, s);
     case Launch::kFusedMfma:
       return launch_fused_mfma(st, l.m, l.q, static_cast<const char*>(p.mfma) + l.mfma_off,
-                               p.precision, t.mfma_blocks, s);
+                               p.precision, t, s);
     case Launch::kTile:
       return launch_tile(st, p.tile_passes[l.arena_off], p.tile_ops, p.tile_diags, p.tile_pool,
                          t.tile_blocks, s);

@@ -52,6 +52,17 @@ struct State {
   size_t bytes() const { return size_t(dim()) * 2 * elem(); }
 };
 
+// Non-temporal accesses: knob 0 never, 1 always, 2 (auto) where they were measured to pay
+// on MI355X -- for a kernel that touches each cache line once (`streams`) and a state
+// larger than the 256 MiB infinity cache.  A smaller state stays cache-resident between
+// gates, which non-temporal accesses forfeit; the pair-per-thread 1q kernel where its pair
+// halves share 128-byte lines, the in-chunk 1q kernel and the LDS-table diagonal ran slower
+// with them.
+inline bool nontemporal_for(const State& st, const Tuning& t, bool streams) {
+  if (t.nontemporal == 2) return streams && st.bytes() > (size_t(256) << 20);
+  return t.nontemporal != 0;
+}
+
 // ---- gate descriptors passed by value to the kernels -------------------------------
 // Qubit index data for an m-qubit gate: sorted positions (for insert_zero_bits) and
 // the offset of each internal sub-index s, scatter_bits(s, q) (layout.rs:54-60).
@@ -107,7 +118,7 @@ hipError_t upload_join(const State& s, const void* re, const void* im);
 // qubits; `mats` is the 2D x 2D real block matrix in f32 (precision 1) or bf16 hi, lo
 // (precision 2).
 hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, const void* mats,
-                             int precision, int max_blocks, hipStream_t s);
+                             int precision, const Tuning& t, hipStream_t s);
 
 // ---- kernels_tile.hip ------------------------------------------------------------------
 // One op of a tile pass, applied to the tile held in LDS; it runs on a tile only when

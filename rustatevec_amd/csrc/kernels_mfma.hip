@@ -21,6 +21,7 @@
 // (the lo*lo term, ~2^-16 relative, is dropped).
 #include <hip/hip_runtime.h>
 
+#include "amp.hpp"
 #include "engine.hpp"
 
 namespace qsvh {
@@ -88,10 +89,11 @@ struct F32Shape {
   static constexpr int NTILE_COLS = NT / 32;       // 32-wide output blocks per row band
 };
 
-template <int K>
+template <int K, bool NTA>
 __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
                                                       uint64_t n_tiles, TileMap map,
-                                                      const float* __restrict__ ablock) {
+                                                      const float* __restrict__ ablock,
+                                                      int xcd) {
   using S = F32Shape<K>;
   constexpr int D = S::D, R = 2 * D, STRIDE = S::STRIDE;
   __shared__ float xr[D * STRIDE];
@@ -122,12 +124,12 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
   }
 
   floatx4 v[kChunks];
-  uint64_t t = blockIdx.x;
+  uint64_t t = xcd_block(xcd);  // consecutive tiles stay on one XCD
   if (t < n_tiles) {
     const uint64_t base = tile_base(t, map);
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
-      v[c] = *reinterpret_cast<const floatx4*>(amp + 2 * (base + off[c]));
+      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (base + off[c])));
     }
   }
   for (; t < n_tiles; t += gridDim.x) {
@@ -146,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
       const uint64_t nb = tile_base(tn, map);
 #pragma unroll
       for (int c = 0; c < kChunks; ++c) {
-        v[c] = *reinterpret_cast<const floatx4*>(amp + 2 * (nb + off[c]));
+        v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (nb + off[c])));
       }
     }
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
@@ -176,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
         y[2 * e] = xr[pos[c][e]];
         y[2 * e + 1] = xi[pos[c][e]];
       }
-      *reinterpret_cast<floatx4*>(amp + 2 * (base + off[c])) = y;
+      vstore<NTA>(reinterpret_cast<floatx4*>(amp + 2 * (base + off[c])), y);
     }
   }
 }
@@ -199,11 +201,12 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
   lo = static_cast<__bf16>(x - static_cast<float>(hi));
 }
 
-template <int K>
+template <int K, bool NTA>
 __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp,
                                                          uint64_t n_tiles, TileMap map,
                                                          const __bf16* __restrict__ ahi,
-                                                         const __bf16* __restrict__ alo) {
+                                                         const __bf16* __restrict__ alo,
+                                                         int xcd) {
   using S = B16Shape<K>;
   constexpr int D = S::D, R = S::R, CS = S::CSTRIDE, NT = S::NT;
   __shared__ float xb[NT * CS];
@@ -238,12 +241,12 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
   }
 
   floatx4 v[kChunks];
-  uint64_t t = blockIdx.x;
+  uint64_t t = xcd_block(xcd);  // consecutive tiles stay on one XCD
   if (t < n_tiles) {
     const uint64_t base = tile_base(t, map);
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
-      v[c] = *reinterpret_cast<const floatx4*>(amp + 2 * (base + off[c]));
+      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (base + off[c])));
     }
   }
   for (; t < n_tiles; t += gridDim.x) {
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
       const uint64_t nb = tile_base(tn, map);
 #pragma unroll
       for (int c = 0; c < kChunks; ++c) {
-        v[c] = *reinterpret_cast<const floatx4*>(amp + 2 * (nb + off[c]));
+        v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (nb + off[c])));
       }
     }
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
@@ -300,27 +303,35 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
         y[2 * e] = xb[pos[c][e]];
         y[2 * e + 1] = xb[pos[c][e] + D];
       }
-      *reinterpret_cast<floatx4*>(amp + 2 * (base + off[c])) = y;
+      vstore<NTA>(reinterpret_cast<floatx4*>(amp + 2 * (base + off[c])), y);
     }
   }
 }
 
-template <int K>
+template <int K, bool NT>
 hipError_t launch_k(const State& st, const TileMap& map, const void* mats, int precision,
-                    int max_blocks, hipStream_t s) {
+                    const Tuning& t, hipStream_t s) {
   const uint64_t n_tiles = st.dim() >> kTileBits;
   uint64_t g = n_tiles;
-  if (g > uint64_t(max_blocks)) g = max_blocks;
+  if (g > uint64_t(t.mfma_blocks)) g = t.mfma_blocks;
   float* amp = static_cast<float*>(st.amp);
   if (precision == 1) {
-    k_fused_f32<K><<<unsigned(g), kBlock, 0, s>>>(amp, n_tiles, map,
-                                                  static_cast<const float*>(mats));
+    k_fused_f32<K, NT><<<unsigned(g), kBlock, 0, s>>>(amp, n_tiles, map,
+                                                      static_cast<const float*>(mats), t.xcd);
   } else {
     constexpr size_t n = size_t(4) << (2 * K);  // (2D)^2 entries
     const __bf16* hi = static_cast<const __bf16*>(mats);
-    k_fused_bf16x3<K><<<unsigned(g), kBlock, 0, s>>>(amp, n_tiles, map, hi, hi + n);
+    k_fused_bf16x3<K, NT><<<unsigned(g), kBlock, 0, s>>>(amp, n_tiles, map, hi, hi + n,
+                                                         t.xcd);
   }
   return hipGetLastError();
+}
+
+template <int K>
+hipError_t launch_k(const State& st, const TileMap& map, const void* mats, int precision,
+                    const Tuning& t, hipStream_t s) {
+  return nontemporal_for(st, t, true) ? launch_k<K, true>(st, map, mats, precision, t, s)
+                                      : launch_k<K, false>(st, map, mats, precision, t, s);
 }
 
 }  // namespace
@@ -329,7 +340,7 @@ hipError_t launch_k(const State& st, const TileMap& map, const void* mats, int p
 // [[Ur, -Ui], [Ui, Ur]] (row-major, 2D x 2D) in f32 (precision 1) or as bf16 hi then lo
 // arrays (precision 2).  Requires a complex64 state of at least 11 qubits.
 hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, const void* mats,
-                             int precision, int max_blocks, hipStream_t s) {
+                             int precision, const Tuning& t, hipStream_t s) {
   if (st.dtype != 1 || st.n < uint32_t(kTileBits) || k < 4 || k > 6) return hipErrorInvalidValue;
   // Tile bits: the group bits plus the (11 - k) lowest other bits.
   uint32_t ts[kTileBits];
@@ -358,9 +369,9 @@ hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, con
     }
   }
   switch (k) {
-    case 4: return launch_k<4>(st, map, mats, precision, max_blocks, s);
-    case 5: return launch_k<5>(st, map, mats, precision, max_blocks, s);
-    default: return launch_k<6>(st, map, mats, precision, max_blocks, s);
+    case 4: return launch_k<4>(st, map, mats, precision, t, s);
+    case 5: return launch_k<5>(st, map, mats, precision, t, s);
+    default: return launch_k<6>(st, map, mats, precision, t, s);
   }
 }
 

@@ -531,17 +531,6 @@ unsigned grid_for(uint64_t items, int unroll, const Tuning& t) {
   return unsigned(g);
 }
 
-// Non-temporal accesses: knob 0 never, 1 always, 2 (auto) where they were measured to pay
-// on MI355X -- for a kernel that touches each cache line once (`streams`) and a state
-// larger than the 256 MiB infinity cache.  A smaller state stays cache-resident between
-// gates, which non-temporal accesses forfeit; the pair-per-thread 1q kernel where its pair
-// halves share 128-byte lines, the in-chunk 1q kernel and the LDS-table diagonal ran slower
-// with them.
-inline bool nontemporal_for(const State& st, const Tuning& t, bool streams) {
-  if (t.nontemporal == 2) return streams && st.bytes() > (size_t(256) << 20);
-  return t.nontemporal != 0;
-}
-
 // Call f(unroll, nontemporal) with the tuning's values as compile-time constants.
 template <typename F>
 hipError_t with_shape(const State& st, const Tuning& t, bool streams, F&& f) {
