@@ -63,7 +63,12 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict_
     const uint64_t base = tile::insert_zero_bits(t, ts, tb);
     T* amp = amp_pool + 2 * bp->state_off;
     const uint32_t flags = bp->flags;
-    for (uint32_t e = threadIdx.x * A; e < N; e += kThreads * A) {
+    // chunk c of lane tid (e = (tid + c NT) A): offset = per-lane part | uniform part
+    constexpr uint32_t LA = tile::ilog2c(kThreads * A);
+    const uint32_t la = tb < LA ? tb : LA;
+    const uint64_t off_lane = tile::offset((threadIdx.x * A) & (N - 1), ts, la);
+    for (uint32_t e = threadIdx.x * A, c = 0; e < N; e += kThreads * A, ++c) {
+      const uint64_t off = off_lane | tile::chunk_offset(c, ts, LA);
       if (flags & kBatchInit) {
         // |0...0>: amplitude 1 at state index 0, which is tile 0, local index 0
 #pragma unroll
@@ -73,7 +78,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict_
         }
       } else {
         T r[A], m[A];
-        cload<T, A>(amp, base + tile::offset(e, ts, tb), r, m);
+        cload<T, A>(amp, base + off, r, m);
 #pragma unroll
         for (int v = 0; v < A; ++v) {
           sr[e + v] = r[v];
@@ -100,14 +105,14 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict_
         partials[bp->part_off + t] = s;
       }
     }
-    for (uint32_t e = threadIdx.x * A; e < N; e += kThreads * A) {
+    for (uint32_t e = threadIdx.x * A, c = 0; e < N; e += kThreads * A, ++c) {
       T r[A], m[A];
 #pragma unroll
       for (int v = 0; v < A; ++v) {
         r[v] = sr[e + v];
         m[v] = si[e + v];
       }
-      cstore<T, A>(amp, base + tile::offset(e, ts, tb), r, m);
+      cstore<T, A>(amp, base + (off_lane | tile::chunk_offset(c, ts, LA)), r, m);
     }
     __syncthreads();
   }

@@ -28,7 +28,8 @@ constexpr int kThreads = 1 << kTileLaneBits;
 
 template <typename T, int TB>
 // Two workgroups per CU (LDS allows it up to 64 KiB tiles) need <= 128 VGPRs per lane.
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_tile(T* __restrict__ amp, uint64_t n_tiles,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_tile(
+                                                   T* __restrict__ amp, uint64_t n_tiles,
                                                    TilePassArg pa,
                                                    const TileOp* __restrict__ ops,
                                                    const TileDiag* __restrict__ diags,
@@ -39,6 +40,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   static_assert(PER >= 1, "tile smaller than one chunk per thread");
   __shared__ T sr[N], si[N];
 
+  // Chunk c of lane tid is tile-local index e = (tid + c NT) A, whose state offset is the
+  // OR of a per-lane part (the bits of tid A) and a uniform part (the bits of c NT A).
+  constexpr uint32_t LA = tile::ilog2c(kThreads * A);
+  const uint64_t off_lane = tile::offset(threadIdx.x * A, pa.ts, LA);
   for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
     const uint64_t base = tile::insert_zero_bits(t, pa.ts, TB);
     // A consecutive tile-local indices are A consecutive amplitudes (ts starts 0, 1, 2, 3)
@@ -46,7 +51,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     for (uint32_t c = 0; c < PER; ++c) {
       const uint32_t e = (threadIdx.x + kThreads * c) * A;
       T r[A], m[A];
-      cload<T, A>(amp, base + tile::offset(e, pa.ts, TB), r, m);
+      cload<T, A>(amp, base + (off_lane | tile::chunk_offset(c, pa.ts, LA)), r, m);
 #pragma unroll
       for (int w = 0; w < A; ++w) {
         sr[e + w] = r[w];
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         r[w] = sr[e + w];
         m[w] = si[e + w];
       }
-      cstore<T, A>(amp, base + tile::offset(e, pa.ts, TB), r, m);
+      cstore<T, A>(amp, base + (off_lane | tile::chunk_offset(c, pa.ts, LA)), r, m);
     }
     __syncthreads();
   }

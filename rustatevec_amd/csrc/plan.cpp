@@ -152,6 +152,7 @@ void schedule_tiles(const std::vector<Launch>& in, uint32_t n, uint32_t tb, Tile
         order.push_back(i);
       }
       bld.close_block();
+      bld.close_run();
       pa.n_ops = uint32_t(bld.ops.size()) - pa.op_begin;
       Launch l;
       l.kind = Launch::kTile;
@@ -185,13 +186,6 @@ uint64_t TileBuilder::add_matrix(const double* g, size_t n_complex) {
 void TileBuilder::lower(const Launch& l, uint64_t tmask, const uint32_t* pos) {
     if (l.kind == Launch::kDiag) {
       close_block();
-      if (ops.size() == pass_begin || ops.back().kind != 1 ||
-          ops.back().off + ops.back().n_diag != diags.size()) {
-        TileOp op{};
-        op.kind = 1;
-        op.off = diags.size();
-        ops.push_back(op);
-      }
       TileDiag d{};
       d.m = l.m;
       for (uint32_t j = 0; j < l.m; ++j) {
@@ -237,7 +231,16 @@ void TileBuilder::lower(const Launch& l, uint64_t tmask, const uint32_t* pos) {
         d.tab = uint32_t(add_matrix(tab.data(), tab.size() / 2));
       }
       diags.push_back(d);
-      ops.back().n_diag++;
+      if (open_run >= 0 && ops.back().kind == 1 && ops.size() > size_t(open_run) + 1 &&
+          ops.back().off + ops.back().n_diag + 1 == diags.size()) {
+        ops.back().n_diag++;  // extends the run's trailing diagonal member
+        return;
+      }
+      TileOp mem{};
+      mem.kind = 1;
+      mem.off = diags.size() - 1;
+      mem.n_diag = 1;
+      run_member(mem);
       return;
     }
     const uint32_t D = 1u << l.m;
@@ -399,7 +402,35 @@ void TileBuilder::lower_block(const double* sub, uint32_t nin, const uint32_t* i
 // block share its outside condition, so a tile that fails it skips the whole block.
 constexpr uint32_t kBlockMinBit = 3;
 
+// Lane-run members: 1-qubit ops on tile-local bits below kRunLaneBits (lanes of one row of
+// 16: the pair partner is one or two DPP moves away) or from kTileLaneBits up (register
+// bits).  Bits 4 and 5 are lanes too, but their exchange (permlane swaps) costs more than
+// sharing a register block's LDS round trip with the neighbouring bits, as bits 6..8 do.
+constexpr uint32_t kRunLaneBits = 4;
+
+void TileBuilder::run_member(const TileOp& m) {
+  if (open_run < 0) {
+    TileOp h{};
+    h.kind = 4;
+    open_run = int64_t(ops.size());
+    ops.push_back(h);
+  }
+  ops[size_t(open_run)].n_diag++;
+  ops.push_back(m);
+}
+
 void TileBuilder::emit(const TileOp& op) {
+  if (op.kind == 0 && op.m == 1 && (op.lq[0] < kRunLaneBits || op.lq[0] >= kTileLaneBits)) {
+    close_block();
+    TileOp mem = op;
+    uint32_t cm = 0;  // condition bits: the op's inserted bits other than its target
+    for (uint32_t i = 0; i < op.nins; ++i)
+      if (op.ins[i] != op.lq[0]) cm |= 1u << op.ins[i];
+    mem.insm[0] = cm;
+    run_member(mem);
+    return;
+  }
+  close_run();
   uint32_t bits = 0;
   for (uint32_t i = 0; i < op.nins; ++i) bits |= 1u << op.ins[i];
   if (op.m == 1 && op.kind == 0 && (bits & ((1u << kBlockMinBit) - 1u)) == 0) {

@@ -52,6 +52,11 @@ struct TileBuilder {
   uint32_t block_bits = 0;  // tile-local bits of the open block
   void emit(const TileOp& op);
   void close_block();
+  // Diagonal gates and 1-qubit ops on lane bits (< 6) or register bits (>= kTileLaneBits)
+  // are gathered, in order, under one lane-run op (kind 4) whose members follow it.
+  int64_t open_run = -1;  // index of the open lane-run op, or -1
+  void run_member(const TileOp& m);
+  void close_run() { open_run = -1; }
 };
 
 // Tile schedule of one circuit's classified launches on n qubits with tb tile bits

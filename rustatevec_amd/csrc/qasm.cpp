@@ -65,7 +65,8 @@ std::vector<Tok> lex(const std::string& src) {
       }
       if (i + 1 >= n) throw ParseError{"unterminated comment", start};
       i += 2;
-    } else if (std::isdigit(c) || (c == '.' && i + 1 < n && std::isdigit((unsigned char)src[i + 1]))) {
+    } else if (std::isdigit(c) ||
+               (c == '.' && i + 1 < n && std::isdigit((unsigned char)src[i + 1]))) {
       std::string num;
       while (i < n && (std::isdigit((unsigned char)src[i]) || src[i] == '.' || src[i] == '_')) {
         if (src[i] != '_') num += src[i];
@@ -412,7 +413,9 @@ class Parser {
       std::vector<Gate> gs = call(nullptr, 0);
       for (Gate& g : gs) {
         for (uint32_t q : g.q)
-          if (measured_[q]) throw ParseError{"gate on a measured qubit (mid-circuit measurement is not supported)", line};
+          if (measured_[q])
+            throw ParseError{"gate on a measured qubit (mid-circuit measurement is not supported)",
+                             line};
         out_.push_back(std::move(g));
         out_line_.push_back(line);
       }
@@ -572,7 +575,8 @@ class Parser {
     size_t width = 1;
     for (const auto& o : ops)
       if (o.size() != 1) {
-        if (width != 1 && o.size() != width) throw ParseError{"register operands differ in size", line};
+        if (width != 1 && o.size() != width)
+          throw ParseError{"register operands differ in size", line};
         width = o.size();
       }
     std::vector<Gate> res;
@@ -623,7 +627,8 @@ class Parser {
           gs.swap(r);
         }
         std::vector<Gate> r;
-        for (long k = 0; k < std::labs(long(md.value)); ++k) r.insert(r.end(), gs.begin(), gs.end());
+        for (long k = 0; k < std::labs(long(md.value)); ++k)
+          r.insert(r.end(), gs.begin(), gs.end());
         gs.swap(r);
       } else {
         for (Gate& g : gs) {
@@ -643,9 +648,11 @@ class Parser {
     if (d != defs_.end()) {
       const GateDef& def = d->second;
       if (p.size() != def.params.size())
-        throw ParseError{"'" + name + "' takes " + std::to_string(def.params.size()) + " parameters", line};
+        throw ParseError{
+            "'" + name + "' takes " + std::to_string(def.params.size()) + " parameters", line};
       if (qs.size() != def.args.size())
-        throw ParseError{"'" + name + "' takes " + std::to_string(def.args.size()) + " qubits", line};
+        throw ParseError{"'" + name + "' takes " + std::to_string(def.args.size()) + " qubits",
+                         line};
       Scope s;
       for (size_t i = 0; i < p.size(); ++i) s.params[def.params[i]] = p[i];
       for (size_t i = 0; i < qs.size(); ++i) s.args[def.args[i]] = qs[i];

@@ -19,10 +19,27 @@ __device__ __forceinline__ uint32_t insert_zero_bit32(uint32_t x, uint32_t b) {
 
 __device__ __forceinline__ uint64_t insert_zero_bits(uint64_t t, const uint32_t* ts, uint32_t tb) {
   for (uint32_t j = 0; j < tb; ++j) {
-    const uint32_t b = ts[j];
+    // the bit positions are read at each use: masks derived from them and hoisted out of
+    // a kernel's tile loop would hold two scalar registers per tile bit
+    uint32_t b = ts[j];
+    asm volatile("" : "+s"(b));
     t = ((t >> b) << (b + 1)) | (t & ((1ull << b) - 1));
   }
   return t;
+}
+
+// Uniform part of a chunk's state offset: bit j of c is tile-local bit lb + j (state bit
+// ts[lb + j]).  The bit positions are read at each use (scalar work per tile): offsets
+// hoisted out of a kernel's tile loop would hold registers across all the ops.
+__device__ __forceinline__ uint64_t chunk_offset(uint32_t c, const uint32_t* ts, uint32_t lb) {
+  uint64_t off = 0;
+  for (uint32_t j = 0; (c >> j) != 0; ++j)
+    if ((c >> j) & 1u) {
+      uint32_t b = ts[lb + j];
+      asm volatile("" : "+s"(b));
+      off |= 1ull << b;
+    }
+  return off;
 }
 
 // State offset of tile-local index e: bit j of e is state bit ts[j].
@@ -193,6 +210,7 @@ __device__ __forceinline__ void with_kpos(uint32_t p, F&& f) {
   if constexpr (LK > 2) if (p == 2) return f(integral_constant<int, 2>{});
   if constexpr (LK > 3) if (p == 3) return f(integral_constant<int, 3>{});
   if constexpr (LK > 4) if (p == 4) return f(integral_constant<int, 4>{});
+  if constexpr (LK > 5) if (p == 5) return f(integral_constant<int, 5>{});
 }
 
 // Phases uniform over the workgroup: pr/pi[c] is the phase of an amplitude whose register
@@ -487,6 +505,264 @@ __device__ __forceinline__ void apply_regblock(T* sr, T* si, uint32_t N, uint64_
   }
 }
 
+// ------------------------------------------------------------------ lane runs (kind 4)
+// A lane run holds the tile in the register layout of the diagonal runs -- lane tid keeps
+// the KMAX amplitudes e = tid + k NT -- and applies its members in order: runs of diagonal
+// gates (member kind 1: TileDiags off .. off + n_diag) and 1-qubit ops (member kind 0)
+// whose target is a lane bit (tile-local bit < 6: the pair partner is in the same wave,
+// lane ^ 2^p) or a register bit (bit >= log2 NT: both amplitudes of a pair are in the
+// lane's registers).
+// Only the run's ends touch LDS, and one barrier follows it, where the LDS form of the same
+// ops takes a round trip and a barrier per op.  Member conditions: outside bits cmask/cval
+// (uniform per tile), tile-local bits insm[0] / lcval (per amplitude).
+
+// Cross-lane exchange for lane bit P < 4: the value of lane (lane ^ 2^P) by DPP (VALU
+// moves within a row of 16 lanes): quad_perm for P = 0, 1, row_ror:8 for P = 3, and for
+// P = 2 row_half_mirror (lane ^ 7 within 8) followed by quad_perm [3,2,1,0] (lane ^ 3).
+template <int P>
+__device__ __forceinline__ uint32_t dpp_xor_u32(uint32_t v) {
+  const int x = int(v);
+  if constexpr (P == 0) return uint32_t(__builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, true));
+  if constexpr (P == 1) return uint32_t(__builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, true));
+  if constexpr (P == 3) return uint32_t(__builtin_amdgcn_update_dpp(x, x, 0x128, 0xF, 0xF, true));
+  if constexpr (P == 2) {
+    const int h = __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, true);
+    return uint32_t(__builtin_amdgcn_update_dpp(h, h, 0x1B, 0xF, 0xF, true));
+  }
+  return v;
+}
+
+template <int P, typename T>
+__device__ __forceinline__ T dpp_xor(T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "float or double");
+  uint32_t w[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < int(sizeof(T) / 4); ++i) w[i] = dpp_xor_u32<P>(w[i]);
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
+// Lane bit P = 4 or 5: v_permlane16_swap / v_permlane32_swap of v with a copy of itself
+// leaves in the first result the value of lane (lane & ~2^P) and in the second that of
+// lane (lane | 2^P) -- the pair's lower and upper amplitudes, in every lane.
+template <int P, typename T>
+__device__ __forceinline__ void permlane_pair(T v, T& lo, T& hi) {
+  static_assert(P == 4 || P == 5, "permlane swaps exchange rows of 16 or halves of 32");
+  uint32_t w[sizeof(T) / 4], a[sizeof(T) / 4], b[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < int(sizeof(T) / 4); ++i) {
+    const auto sw = P == 5 ? __builtin_amdgcn_permlane32_swap(w[i], w[i], false, false)
+                           : __builtin_amdgcn_permlane16_swap(w[i], w[i], false, false);
+    a[i] = sw[0];
+    b[i] = sw[1];
+  }
+  __builtin_memcpy(&lo, a, sizeof(T));
+  __builtin_memcpy(&hi, b, sizeof(T));
+}
+
+// A value that is the same in every lane, read into scalar registers: a select between two
+// such values stays a select of values (a select between m[a] and m[b] would otherwise
+// become a per-lane load from a selected address).
+template <typename T>
+__device__ __forceinline__ T uniform(T v) {
+  uint32_t w[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < int(sizeof(T) / 4); ++i)
+    w[i] = uint32_t(__builtin_amdgcn_readfirstlane(int(w[i])));
+  __builtin_memcpy(&v, w, sizeof(T));
+  return v;
+}
+
+// Row `up` of pair_math<V> (y0 for the lower amplitude, y1 for the upper) for one lane:
+// the lane's coefficients are picked once per op (c0 multiplies x0, c1 multiplies x1;
+// anti-diagonal: c0 multiplies the partner), then every amplitude takes the operations
+// pair_math performs for that row, in its order.
+template <typename T, uint32_t V>
+struct PairRow {
+  T c0r, c0i, c1r, c1i;  // multiply x0 and x1 (anti-diagonal: c0 multiplies the partner)
+  T csr, csi, cpr, cpi;  // multiply the lane's own amplitude and its partner
+  __device__ __forceinline__ PairRow(const T* __restrict__ m, bool up) {
+    // all entries are read before the selects (reads inside a select's arms would become
+    // a divergent branch)
+    T e[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) e[i] = uniform(m[i]);
+    if constexpr (V == kPairAnti) {  // y0 = g01 x1, y1 = g10 x0
+      c0r = up ? e[4] : e[2];
+      c0i = up ? e[5] : e[3];
+      c1r = c1i = T(0);
+      csr = csi = T(0);
+      cpr = c0r;
+      cpi = c0i;
+    } else {
+      c0r = up ? e[4] : e[0];
+      c0i = up ? e[5] : e[1];
+      c1r = up ? e[6] : e[2];
+      c1i = up ? e[7] : e[3];
+      csr = up ? e[6] : e[0];  // g11 for the upper amplitude, g00 for the lower
+      csi = up ? e[7] : e[1];
+      cpr = up ? e[4] : e[2];  // g10 / g01
+      cpi = up ? e[5] : e[3];
+    }
+  }
+  // The same row as own term + partner term: pair_math adds the x0 term to the x1 term,
+  // and IEEE addition is commutative, so for either row this is its value bit for bit.
+  __device__ __forceinline__ void apply_sp(T& yr, T& yi, T xr, T xi, T pr, T pi) const {
+    if constexpr (V == kPairSwap) {
+      yr = pr;
+      yi = pi;
+    } else if constexpr (V == kPairAnti) {
+      T ar = pr, ai = pi;
+      cmul(ar, ai, cpr, cpi);
+      yr = ar;
+      yi = ai;
+    } else if constexpr (V == kPairReal) {
+      yr = csr * xr + cpr * pr;
+      yi = csr * xi + cpr * pi;
+    } else {
+      const T ar = csr * xr - csi * xi, ai = csr * xi + csi * xr;
+      const T br = cpr * pr - cpi * pi, bi = cpr * pi + cpi * pr;
+      yr = ar + br;
+      yi = ai + bi;
+    }
+  }
+  // The row from the pair's lower and upper amplitudes (x0, x1), held in every lane.
+  __device__ __forceinline__ void apply_x01(T& yr, T& yi, T x0r, T x0i, T x1r, T x1i,
+                                            bool up) const {
+    if constexpr (V == kPairSwap || V == kPairAnti) {  // only the partner enters
+      apply_sp(yr, yi, T(0), T(0), up ? x0r : x1r, up ? x0i : x1i);
+    } else if constexpr (V == kPairReal) {
+      yr = c0r * x0r + c1r * x1r;
+      yi = c0r * x0i + c1r * x1i;
+    } else {
+      const T ar = c0r * x0r - c0i * x0i, ai = c0r * x0i + c0i * x0r;
+      const T br = c1r * x1r - c1i * x1i, bi = c1r * x1i + c1i * x1r;
+      yr = ar + br;
+      yi = ai + bi;
+    }
+  }
+};
+
+// 1-qubit member on lane bit P.
+template <typename T, uint32_t V, int P, int NT, int KMAX>
+__device__ __forceinline__ void run_1q_lane(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N,
+                                            const T* __restrict__ m, uint32_t cm, uint32_t cv) {
+  // recomputed per member (one VALU op): a lane mask per bit position hoisted to the kernel
+  // entry would hold scalar registers for the whole kernel
+  uint32_t tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const bool up = (tid >> P) & 1u;
+  const PairRow<T, V> row(m, up);
+  static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
+    constexpr int k = kc.value;
+    if (uint32_t(k) * NT < N) {  // uniform: whole waves are in or out for N >= 64
+      T yr, yi;
+      if constexpr (P < 4) {
+        const T pr = dpp_xor<P>(xr[k]), pi = dpp_xor<P>(xi[k]);
+        row.apply_sp(yr, yi, xr[k], xi[k], pr, pi);
+      } else {
+        T x0r, x1r, x0i, x1i;
+        permlane_pair<P>(xr[k], x0r, x1r);
+        permlane_pair<P>(xi[k], x0i, x1i);
+        row.apply_x01(yr, yi, x0r, x0i, x1r, x1i, up);
+      }
+      if (cm == 0) {
+        xr[k] = yr;
+        xi[k] = yi;
+      } else {
+        const bool c = ((tid + uint32_t(k) * NT) & cm) == cv;
+        xr[k] = c ? yr : xr[k];
+        xi[k] = c ? yi : xi[k];
+      }
+    }
+  });
+}
+
+// 1-qubit member on register bit J (tile-local bit log2(NT) + J).
+template <typename T, uint32_t V, int J, int NT, int KMAX>
+__device__ __forceinline__ void run_1q_reg(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N,
+                                           const T* __restrict__ m, uint32_t cm, uint32_t cv) {
+  static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
+    constexpr int k = kc.value;
+    if constexpr (((k >> J) & 1) == 0) {
+      constexpr int k1 = k | (1 << J);
+      if (uint32_t(k1) * NT < N) {
+        if (cm == 0) {
+          pair_math<T, V>(xr[k], xi[k], xr[k1], xi[k1], m);
+        } else {
+          T a0r = xr[k], a0i = xi[k], a1r = xr[k1], a1i = xi[k1];
+          pair_math<T, V>(a0r, a0i, a1r, a1i, m);
+          const bool c = ((threadIdx.x + uint32_t(k) * NT) & cm) == cv;
+          xr[k] = c ? a0r : xr[k];
+          xi[k] = c ? a0i : xi[k];
+          xr[k1] = c ? a1r : xr[k1];
+          xi[k1] = c ? a1i : xi[k1];
+        }
+      }
+    }
+  });
+}
+
+template <typename T, int NT, int KMAX>
+__device__ __forceinline__ void apply_lane_run(T* sr, T* si, uint32_t N, uint64_t base,
+                                               const TileOp& op,
+                                               const TileOp* __restrict__ mem,
+                                               const TileDiag* __restrict__ diags,
+                                               const T* __restrict__ pool) {
+  constexpr int LB = int(ilog2c(NT));    // lane + wave bits
+  constexpr int LK = int(ilog2c(KMAX));  // register bits
+  T xr[KMAX], xi[KMAX];
+  static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
+    const uint32_t e = threadIdx.x + uint32_t(kc.value) * NT;
+    xr[kc.value] = T(0);
+    xi[kc.value] = T(0);
+    if (e < N) {
+      xr[kc.value] = sr[e];
+      xi[kc.value] = si[e];
+    }
+  });
+  const uint32_t nmem = op.n_diag;
+  for (uint32_t u = 0; u < nmem; ++u) {
+    const TileOp& g = mem[u];
+    if ((base & g.cmask) != g.cval) continue;
+    if (g.kind == 1) {  // consecutive diagonal gates diags[off .. off + n_diag)
+      for (uint32_t d = 0; d < g.n_diag; ++d) {
+        const TileDiag* dg = diags + g.off + d;
+        if (dg->m <= 2)
+          diag_gate_small<T, NT, KMAX>(xr, xi, N, base, dg, pool);
+        else
+          diag_gate_wide<T, NT, KMAX>(xr, xi, N, base, dg, pool);
+      }
+      continue;
+    }
+    const T* __restrict__ mat = pool + 2 * g.off;
+    const uint32_t p = g.lq[0], cm = g.insm[0], cv = g.lcval;
+    with_pair_variant(g.real, [&](auto vc) __attribute__((always_inline)) {
+      constexpr uint32_t V = vc.value;
+      if (p < 6u) {
+        with_kpos<6>(p, [&](auto pc) __attribute__((always_inline)) {
+          run_1q_lane<T, V, pc.value, NT, KMAX>(xr, xi, N, mat, cm, cv);
+        });
+      } else if constexpr (LK > 0) {
+        with_kpos<LK>(p - uint32_t(LB), [&](auto jc) __attribute__((always_inline)) {
+          run_1q_reg<T, V, jc.value, NT, KMAX>(xr, xi, N, mat, cm, cv);
+        });
+      }
+    });
+  }
+  static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
+    const uint32_t e = threadIdx.x + uint32_t(kc.value) * NT;
+    if (e < N) {
+      sr[e] = xr[kc.value];
+      si[e] = xi[kc.value];
+    }
+  });
+}
+
 // Interpret ops[0..n_ops) on the tile (LDS); ends with a workgroup barrier.
 template <typename T, int NT, int KMAX>
 __device__ __forceinline__ void run_ops(T* sr, T* si, uint32_t N, uint64_t base,
@@ -495,6 +771,12 @@ __device__ __forceinline__ void run_ops(T* sr, T* si, uint32_t N, uint64_t base,
                                         const T* __restrict__ pool) {
   for (uint32_t o = 0; o < n_ops; ++o) {
     const TileOp op = ops[o];
+    if (op.kind == 4) {  // lane run; its op.n_diag members follow it
+      apply_lane_run<T, NT, KMAX>(sr, si, N, base, op, ops + o + 1, diags, pool);
+      o += op.n_diag;
+      __syncthreads();
+      continue;
+    }
     if (op.kind == 3) {  // register block; its op.n_diag members follow it
       if ((base & op.cmask) != op.cval) {  // the members' common outside condition
         o += op.n_diag;
