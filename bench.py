@@ -303,6 +303,15 @@ def main():
                                    per_size=150, seed=seed)
     model, train_loss = planner.train_cost_model(b, X, y, epochs=2000)
     t_model = time.perf_counter() - t0
+    # the tile-pass cost model of the form planner (fused groups vs exact tile passes)
+    t0 = time.perf_counter()
+    XT, yT = planner.collect_tile_timings(b, sizes=(24,) if args.quick else (26,),
+                                          per_size=300, seed=seed)
+    tile_model, tile_loss = planner.train_tile_model(b, XT, yT, epochs=2000)
+    t_tile_model = time.perf_counter() - t0
+
+    def form_name(tq, chosen):
+        return f"tiles_t{tq}" if tq else ("dp" if chosen == 0 else f"greedy_k{chosen}")
     nq = 22 if args.quick else 28
     mixc = rs.mix(nq, 40 * nq, seed)
     ref = b.alloc(nq, rs.C128)
@@ -318,9 +327,12 @@ def main():
     t0 = time.perf_counter()
     planned, pred_ms, chosen = planner.plan(b, model, mixc, 6)
     t_plan = time.perf_counter() - t0
-    cands = [("greedy_k%d" % k, rs.fuse(b, mixc, k)) for k in (4, 5, 6)] + [("learned", planned)]
-    for name, circ in cands:
-        p = rs.Program(st64, circ, precision=rs.PREC_MFMA_BF16X3)
+    formc, form_tq, form_pred, form_chosen = planner.plan_form(b, model, tile_model, mixc, rs.C64)
+    cands = [("greedy_k%d" % k, rs.fuse(b, mixc, k), 0) for k in (4, 5, 6)]
+    cands += [("learned", planned, 0), ("learned_form", formc, form_tq)]
+    for name, circ, tq in cands:
+        p = rs.Program(st64, circ, precision=rs.PREC_EXACT if tq else rs.PREC_MFMA_BF16X3,
+                       tile_qubits=tq)
         s = timed_steps(b, st64, p, 2, 1, reset64)
         reset64()
         p.launch()
@@ -335,7 +347,18 @@ def main():
                    "chosen": "dp" if chosen == 0 else f"greedy_k{chosen}",
                    "plain_c128_s": round(mix_plain_s, 4), "legs": legs,
                    "gain_vs_greedy_k4": round(legs["greedy_k4"]["circuit_s"]
-                                              / legs["learned"]["circuit_s"], 3)}
+                                              / legs["learned"]["circuit_s"], 3),
+                   "form_planner": {"tile_timings": int(len(XT)),
+                                    "tile_train_mse_std": round(tile_loss, 4),
+                                    "tile_model_s": round(t_tile_model, 2),
+                                    "chosen": form_name(form_tq, form_chosen),
+                                    "predicted_ms": round(form_pred, 3),
+                                    "gain_vs_greedy_k4": round(
+                                        legs["greedy_k4"]["circuit_s"]
+                                        / legs["learned_form"]["circuit_s"], 3),
+                                    "gain_vs_greedy_k6": round(
+                                        legs["greedy_k6"]["circuit_s"]
+                                        / legs["learned_form"]["circuit_s"], 3)}}
     log(f"learned planner: {learned_rec}")
     st64.free()
     ref.free()
@@ -355,11 +378,15 @@ def main():
         st64 = b.alloc(nq2, rs.C64)
         reset64 = lambda: b.init_basis(st64, 0)  # noqa: E731
         planned, _, chosen = planner.plan(b, model, mc, 6)
+        formc, form_tq, _, form_chosen = planner.plan_form(b, model, tile_model, mc, rs.C64)
         rec = {"qubits": nq2, "gates": len(mc), "plain_c128_s": round(plain_s, 4),
-               "chosen": "dp" if chosen == 0 else f"greedy_k{chosen}"}
-        for name, circ in (("greedy_k4", rs.fuse(b, mc, 4)), ("greedy_k6", rs.fuse(b, mc, 6)),
-                           ("learned", planned)):
-            p = rs.Program(st64, circ, precision=rs.PREC_MFMA_BF16X3)
+               "chosen": "dp" if chosen == 0 else f"greedy_k{chosen}",
+               "form_chosen": form_name(form_tq, form_chosen)}
+        for name, circ, tq in (("greedy_k4", rs.fuse(b, mc, 4), 0),
+                               ("greedy_k6", rs.fuse(b, mc, 6), 0), ("learned", planned, 0),
+                               ("learned_form", formc, form_tq)):
+            p = rs.Program(st64, circ, precision=rs.PREC_EXACT if tq else rs.PREC_MFMA_BF16X3,
+                           tile_qubits=tq)
             s_ = timed_steps(b, st64, p, 1, 1, reset64)
             reset64()
             p.launch()
@@ -383,9 +410,12 @@ def main():
     st64 = b.alloc(n_layered, rs.C64)
     reset64 = lambda: b.init_basis(st64, 0)  # noqa: E731
     planned, _, chosen = planner.plan(b, model, lay, 6)
+    formc, form_tq, _, form_chosen = planner.plan_form(b, model, tile_model, lay, rs.C64)
     lay_legs = {}
     cands = [("greedy_k%d" % k, rs.fuse(b, lay, k), rs.PREC_MFMA_BF16X3, 0) for k in (4, 5, 6)]
     cands += [("learned", planned, rs.PREC_MFMA_BF16X3, 0),
+              ("learned_form", formc, rs.PREC_EXACT if form_tq else rs.PREC_MFMA_BF16X3,
+               form_tq),
               ("tiled_c64_exact", lay, rs.PREC_EXACT, 13)]
     for name, circ, prec, tq in cands:
         p = rs.Program(st64, circ, precision=prec, tile_qubits=tq)
@@ -398,6 +428,7 @@ def main():
                           "max_abs_err": err, "one_minus_fidelity": 1 - fid}
         del p
     lay_legs["learned"]["chosen"] = "dp" if chosen == 0 else f"greedy_k{chosen}"
+    lay_legs["learned_form"]["chosen"] = form_name(form_tq, form_chosen)
     log(f"layered({n_layered}) fused / learned / tiled c64: {lay_legs}")
     st64.free()
     ref.free()
@@ -410,8 +441,22 @@ def main():
     b.synchronize(st)
     ghz_ms = min(prog.time() for _ in range(20))
     del prog
+    # the same chain as tile passes (a few launches instead of 20), checked on its output
+    ghz_tiled = {}
+    for tq in (10, 12):
+        b.init_basis(st, 0)
+        prog = rs.Program(st, rs.ghz_chain(20), use_graph=True, tile_qubits=tq)
+        prog.launch()
+        b.synchronize(st)
+        h = 2 ** -0.5
+        ok = (abs(b.amplitude(st, 0) - h) < 1e-15 and abs(b.amplitude(st, (1 << 20) - 1) - h) < 1e-15
+              and abs(b.norm_sqr(st) - 1.0) < 1e-14)
+        us = min(prog.time() for _ in range(20)) * 1e3
+        if ok and ("us" not in ghz_tiled or us < ghz_tiled["us"]):
+            ghz_tiled = {"us": round(us, 2), "tile_qubits": tq, "passes": prog.num_kernels()}
+        del prog
     st.free()
-    log(f"GHZ(20) graph replay: {ghz_ms * 1e3:.1f} us")
+    log(f"GHZ(20) graph replay: {ghz_ms * 1e3:.1f} us per-gate, tiled {ghz_tiled}")
 
     # ------------------------------------------------------------ CPU baseline
     cpu = None
@@ -457,6 +502,7 @@ def main():
         "batched_hea": batch_rec,
         "learned_planner": learned_rec,
         "ghz20_us": round(ghz_ms * 1e3, 2),
+        "ghz20_tiled": ghz_tiled,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

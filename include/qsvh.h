@@ -325,6 +325,29 @@ int qsvh_group_features(uint32_t n, uint32_t k, const uint32_t* qubits, int diag
 int qsvh_plan_fusion(qsvh_ctx* ctx, qsvh_mlp* model, const qsvh_circuit* in,
                      uint32_t max_qubits, uint32_t window, qsvh_circuit** out,
                      double* predicted_ms, int* chosen);
+/* Cost-model features of every tile pass the tile scheduler builds for `c` on a state of
+ * `dtype` with `tile_qubits` tile bits (10..12 for QSVH_C128, 11..13 for QSVH_C64):
+ * QSVH_TILE_FEATURES floats per pass into out (at most max_passes rows); *n_passes: passes
+ * built, *n_other: launches left outside passes (gates a tile cannot hold).  Features:
+ * dtype, tile bits/13, n/32, then counts /16 of diagonal gates (phase uniform per tile,
+ * lane-bit, register-bit, m > 2), lane-run 1q members (lane / register target x general /
+ * cheap pair variant), register blocks, their members (general / cheap), LDS ops, permutation
+ * ops, lane runs, conditioned lane-run members.  No reference counterpart: the tile passes
+ * and their cost model are this engine's. */
+#define QSVH_TILE_FEATURES 18
+int qsvh_tile_pass_features(const qsvh_circuit* c, int dtype, uint32_t tile_qubits, float* out,
+                            size_t max_passes, size_t* n_passes, size_t* n_other);
+/* Learned choice of the execution form for `in` on a `dtype` state: the fused plan of
+ * qsvh_plan_fusion (group_model, 12 inputs) against tile passes at every allowed tile size
+ * (tile_model, QSVH_TILE_FEATURES inputs, trained on log2 of pass milliseconds at 2^30
+ * amplitudes; gates outside passes priced by group_model), the cheapest predicted kept.
+ * Fused: *out is the fused circuit (run with an MFMA precision), *tile_qubits = 0, *chosen as
+ * qsvh_plan_fusion.  Tiled: *out is `in` unchanged (run exact with program option
+ * tile_qubits), *tile_qubits > 0, *chosen = -tile_qubits.  Extends the fusion planner
+ * (fusion.rs:39-77 is the greedy fallback) to the choice between the two forms. */
+int qsvh_plan_form(qsvh_ctx* ctx, qsvh_mlp* group_model, qsvh_mlp* tile_model,
+                   const qsvh_circuit* in, int dtype, uint32_t max_qubits, uint32_t window,
+                   qsvh_circuit** out, int* tile_qubits, double* predicted_ms, int* chosen);
 
 /* Apply one gate `reps` times back to back and return the mean device time per launch
  * (HIP events on the context stream).  Measurement helper for single-kernel rooflines. */

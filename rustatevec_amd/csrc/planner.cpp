@@ -20,10 +20,78 @@
 #include "../../include/qsvh.h"
 #include "handles.hpp"
 #include "mlp.hpp"
+#include "program.hpp"
 
 namespace qsvh {
 
+Launch classify(const Op& op, int dtype);  // api.cpp
+
 constexpr int kFeatures = 12;
+constexpr int kTileFeatures = 18;
+
+// Feature vector of one tile pass (the ops schedule_tiles lowered for it): the state's
+// dtype, tile bits and size, then counts (/16) of diagonal gates by lowering variant
+// (phase uniform per tile, indexed by lane bits, by register bits only, m > 2), of lane-run
+// 1-qubit members (lane or register target; general or cheap pair variant), register
+// blocks and their members, LDS ops (standalone 1-qubit and dense 2-3 qubit), permutation
+// ops, lane runs, and members with per-amplitude conditions.
+void tile_pass_features(const TileBuilder& b, const TilePassArg& pa, uint32_t n, int dtype,
+                        float* f) {
+  float cnt[kTileFeatures] = {0};
+  auto diag = [&](const TileDiag& d) {
+    const uint32_t k0 = d.variant & 15u;
+    cnt[d.m > 2 ? 6 : k0 == 0 ? 3 : k0 == 1 ? 4 : 5] += 1.f;
+  };
+  const uint32_t end = pa.op_begin + pa.n_ops;
+  for (uint32_t o = pa.op_begin; o < end; ++o) {
+    const TileOp& op = b.ops[o];
+    if (op.kind == 4 || op.kind == 3) {
+      cnt[op.kind == 4 ? 16 : 11] += 1.f;
+      for (uint32_t u = 1; u <= op.n_diag; ++u) {
+        const TileOp& g = b.ops[o + u];
+        if (g.kind == 1) {
+          for (uint32_t d = 0; d < g.n_diag; ++d) diag(b.diags[g.off + d]);
+          continue;
+        }
+        const bool cheap = g.real != 0;
+        if (op.kind == 3)
+          cnt[cheap ? 13 : 12] += 1.f;
+        else
+          cnt[g.lq[0] >= kTileLaneBits ? (cheap ? 10 : 9) : (cheap ? 8 : 7)] += 1.f;
+        if (op.kind == 4 && g.insm[0] != 0) cnt[17] += 1.f;
+      }
+      o += op.n_diag;
+    } else if (op.kind == 1) {
+      for (uint32_t d = 0; d < op.n_diag; ++d) diag(b.diags[op.off + d]);
+    } else {
+      cnt[op.kind == 0 ? 14 : 15] += 1.f;
+    }
+  }
+  f[0] = float(dtype);
+  f[1] = float(pa.tb) / 13.f;
+  f[2] = float(n) / 32.f;
+  for (int i = 3; i < kTileFeatures; ++i) f[i] = cnt[i] / 16.f;
+}
+
+// The tile schedule of circuit c (dtype) at tb tile bits: the feature rows of its passes
+// and the launches it leaves outside passes.
+void tile_schedule_features(const Circuit& c, int dtype, uint32_t tb, std::vector<float>* rows,
+                            std::vector<Launch>* other) {
+  std::vector<Launch> in;
+  in.reserve(c.ops.size());
+  for (const Op& op : c.ops) in.push_back(classify(op, dtype));
+  TileBuilder bld;
+  std::vector<TilePassArg> passes;
+  std::vector<Launch> out;
+  std::vector<uint64_t> order;
+  schedule_tiles(in, c.n, tb, bld, passes, out, order);
+  rows->assign(passes.size() * kTileFeatures, 0.f);
+  for (size_t i = 0; i < passes.size(); ++i)
+    tile_pass_features(bld, passes[i], c.n, dtype, rows->data() + i * kTileFeatures);
+  other->clear();
+  for (Launch& l : out)
+    if (l.kind != Launch::kTile) other->push_back(std::move(l));
+}
 
 // Feature vector of a group: k qubits (ascending), diagonal flag, density of the composed
 // matrix (nonzeros / 4^k) and the state size n.
@@ -274,6 +342,57 @@ int plan_fusion(Ctx* ctx, Mlp& mlp, const Circuit& c, uint32_t max_qubits, uint3
   return build_partition(ctx, c, parts, out);
 }
 
+// Learned choice of the execution form: the fused plan (plan_fusion, matrix-core groups)
+// against tile passes at every allowed tile size, each priced by its cost model -- pass
+// times by the tile model, launches a tile cannot hold by the group model -- keeping the
+// lowest predicted total.  Tiled: *out is the circuit unchanged and *tile_qubits > 0.
+int plan_form(Ctx* ctx, Mlp& group_mlp, Mlp& tile_mlp, const Circuit& c, int dtype,
+              uint32_t max_qubits, uint32_t window, Circuit* out, int* tile_qubits,
+              double* predicted_ms, int* chosen) {
+  double fused_ms = 0;
+  int fused_pick = 0;
+  int rc = plan_fusion(ctx, group_mlp, c, max_qubits, window, out, &fused_ms, &fused_pick);
+  if (rc != kOk) return rc;
+  double best = fused_ms;
+  int best_tb = 0;
+  const uint32_t lo = dtype == 0 ? 10 : 11, hi = dtype == 0 ? 12 : 13;
+  for (uint32_t tb = lo; tb <= hi && tb <= c.n; ++tb) {
+    std::vector<float> rows, y;
+    std::vector<Launch> other;
+    tile_schedule_features(c, dtype, tb, &rows, &other);
+    const size_t np = rows.size() / kTileFeatures;
+    y.resize(np);
+    if (np) {
+      rc = tile_mlp.predict(rows.data(), np, y.data(), false);
+      if (rc != kOk) return rc;
+    }
+    double t = 0;
+    for (size_t i = 0; i < np; ++i) t += std::exp2(double(y[i])) * std::exp2(double(c.n) - 30.0);
+    std::vector<Candidate> cands;
+    for (const Launch& l : other) {
+      Candidate cd;
+      uint32_t q[kMaxGateQubits];
+      std::copy(l.q, l.q + l.m, q);
+      std::sort(q, q + l.m);
+      group_features(c.n, l.m, q, l.kind == Launch::kDiag, 1.0, cd.f);
+      cands.push_back(cd);
+    }
+    std::vector<double> oc;
+    rc = predict_ms(group_mlp, c.n, cands, &oc);
+    if (rc != kOk) return rc;
+    for (double x : oc) t += x;
+    if (t < best) {
+      best = t;
+      best_tb = int(tb);
+    }
+  }
+  if (best_tb > 0) *out = c;
+  if (tile_qubits) *tile_qubits = best_tb;
+  if (predicted_ms) *predicted_ms = best;
+  if (chosen) *chosen = best_tb > 0 ? -best_tb : fused_pick;
+  return kOk;
+}
+
 }  // namespace qsvh
 
 using namespace qsvh;
@@ -301,6 +420,44 @@ int qsvh_plan_fusion(qsvh_ctx* h, qsvh_mlp* model, const qsvh_circuit* in, uint3
   if (!res) return kOom;
   const int rc = plan_fusion(&h->c, model->m, in->c, max_qubits, window ? window : 48, &res->c,
                              predicted_ms, chosen);
+  if (rc != kOk) {
+    delete res;
+    return rc;
+  }
+  *out = res;
+  return kOk;
+}
+
+int qsvh_tile_pass_features(const qsvh_circuit* c, int dtype, uint32_t tile_qubits, float* out,
+                            size_t max_passes, size_t* n_passes, size_t* n_other) {
+  if (!c || (dtype != 0 && dtype != 1) || (max_passes && !out)) return kInvalidArg;
+  const uint32_t lo = dtype == 0 ? 10 : 11, hi = dtype == 0 ? 12 : 13;
+  if (tile_qubits < lo || tile_qubits > hi || c->c.n < tile_qubits) return kInvalidArg;
+  std::vector<float> rows;
+  std::vector<Launch> other;
+  tile_schedule_features(c->c, dtype, tile_qubits, &rows, &other);
+  const size_t np = rows.size() / kTileFeatures;
+  std::copy(rows.begin(), rows.begin() + std::min(np, max_passes) * kTileFeatures, out);
+  if (n_passes) *n_passes = np;
+  if (n_other) *n_other = other.size();
+  return kOk;
+}
+
+int qsvh_plan_form(qsvh_ctx* h, qsvh_mlp* group_model, qsvh_mlp* tile_model,
+                   const qsvh_circuit* in, int dtype, uint32_t max_qubits, uint32_t window,
+                   qsvh_circuit** out, int* tile_qubits, double* predicted_ms, int* chosen) {
+  if (!h || !group_model || !tile_model || !in || !out || (dtype != 0 && dtype != 1))
+    return kInvalidArg;
+  *out = nullptr;
+  if (max_qubits < 1 || max_qubits > kMaxGateQubits)
+    return set_err(&h->c, kTooWide, "plan: max_qubits must be 1..6");
+  if (group_model->m.F != kFeatures) return set_err(&h->c, kInvalidArg, "plan: group model has 12 inputs");
+  if (tile_model->m.F != kTileFeatures)
+    return set_err(&h->c, kInvalidArg, "plan: tile model has 18 inputs");
+  qsvh_circuit* res = new (std::nothrow) qsvh_circuit;
+  if (!res) return kOom;
+  const int rc = plan_form(&h->c, group_model->m, tile_model->m, in->c, dtype, max_qubits,
+                           window ? window : 48, &res->c, tile_qubits, predicted_ms, chosen);
   if (rc != kOk) {
     delete res;
     return rc;

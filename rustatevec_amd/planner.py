@@ -18,6 +18,7 @@ from . import (C64, PREC_MFMA_BF16X3, Circuit, HipBackend, Program, QsvhError, _
                lib)
 
 N_FEATURES = 12
+N_TILE_FEATURES = 18
 HIDDEN = 64
 
 
@@ -40,6 +41,10 @@ def _bind():
         "qsvh_group_features": (i32, [u32, u32, P, i32, C.c_double, P]),
         "qsvh_plan_fusion": (i32, [P, P, P, u32, u32, PP, C.POINTER(C.c_double),
                                    C.POINTER(C.c_int)]),
+        "qsvh_tile_pass_features": (i32, [P, i32, u32, P, C.c_size_t,
+                                          C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+        "qsvh_plan_form": (i32, [P, P, P, P, i32, u32, u32, PP, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -179,5 +184,101 @@ def plan(backend: HipBackend, model: Mlp, circuit: Circuit, max_qubits=6, window
     return Circuit(_handle=h), ms.value, chosen.value
 
 
+def tile_pass_features(circuit: Circuit, dtype: int, tile_qubits: int):
+    """(rows, n_other): the cost-model features of every tile pass the scheduler builds for
+    `circuit` (one row per pass) and the number of launches left outside passes."""
+    L = _bind()
+    n_p, n_o = C.c_size_t(0), C.c_size_t(0)
+    _check(L.qsvh_tile_pass_features(circuit.handle, dtype, tile_qubits, None, 0,
+                                     C.byref(n_p), C.byref(n_o)))
+    out = np.zeros((n_p.value, N_TILE_FEATURES), dtype=np.float32)
+    if n_p.value:
+        _check(L.qsvh_tile_pass_features(circuit.handle, dtype, tile_qubits, _ptr(out),
+                                         n_p.value, C.byref(n_p), C.byref(n_o)))
+    return out, n_o.value
+
+
+def _random_pass(rng, n, tb):
+    """A random circuit that the tile scheduler runs as one pass of tb tile bits: gates of
+    the mixed workloads with their action qubits in one random tile set (controls and the
+    qubits of diagonal gates anywhere)."""
+    tile = [0, 1, 2, 3] + sorted(int(x) for x in rng.choice(np.arange(4, n), size=tb - 4,
+                                                           replace=False))
+    c = Circuit(n)
+    for _ in range(int(rng.integers(1, 65))):
+        r = rng.random()
+        t = int(rng.choice(tile))
+        o = int(rng.choice([q for q in range(n) if q != t]))
+        a = lambda: float(rng.uniform(0, 2 * np.pi))  # noqa: E731
+        if r < 0.25:
+            c.u3(t, a(), a(), a())
+        elif r < 0.40:
+            c.ry(t, a()) if rng.random() < 0.5 else c.h(t)
+        elif r < 0.45:
+            c.x(t)
+        elif r < 0.60:
+            c.rz(int(rng.integers(n)), a())
+        elif r < 0.80:
+            k = rng.integers(3)
+            c.cz(t, o) if k == 0 else (c.cphase(o, t, a()) if k == 1 else c.rzz(o, t, a()))
+        elif r < 0.95:
+            c.cx(o, t)
+        elif r < 0.98:
+            u = int(rng.choice([q for q in tile if q != t]))
+            c.swap(t, u)
+        else:
+            p = int(rng.choice([q for q in range(n) if q not in (t, o)]))
+            c.gate("ccx", [o, p, t])
+    return c
+
+
+def collect_tile_timings(backend: HipBackend, sizes=(26,), per_size=300, seed=7, dtype=C64,
+                         reps=3):
+    """Time random single tile passes (tile bits 11..13 for complex64, 10..12 for
+    complex128) on HBM-resident states; returns pass features X and y = log2 of the
+    device milliseconds scaled to 2^30 amplitudes."""
+    rng = np.random.default_rng(seed)
+    lo = 11 if dtype == C64 else 10
+    X, y = [], []
+    for n in sizes:
+        st = backend.alloc(n, dtype)
+        backend.init_random_normal(st, seed)
+        for _ in range(per_size):
+            tb = int(rng.integers(lo, lo + 3))
+            c = _random_pass(rng, n, tb)
+            rows, other = tile_pass_features(c, dtype, tb)
+            if len(rows) != 1 or other:
+                continue
+            p = Program(st, c, tile_qubits=tb)
+            p.launch()
+            ms = min(p.time() for _ in range(reps))
+            X.append(rows[0])
+            y.append(np.log2(ms * 2.0 ** (30 - n)))
+            del p
+        st.free()
+    return np.array(X, dtype=np.float32), np.array(y, dtype=np.float32)
+
+
+def train_tile_model(backend: HipBackend, X, y, epochs=1500, lr=3e-3, seed=2):
+    m = Mlp(backend, n_features=N_TILE_FEATURES, seed=seed)
+    loss = m.train(X, y, epochs=epochs, lr=lr)
+    return m, loss
+
+
+def plan_form(backend: HipBackend, group_model: Mlp, tile_model: Mlp, circuit: Circuit,
+              dtype=C64, max_qubits=6, window=48):
+    """(circuit, tile_qubits, predicted ms, chosen): the fused plan (tile_qubits 0; run it
+    with an MFMA precision) or the circuit as tile passes of tile_qubits bits (run exact),
+    whichever the cost models predict faster; chosen as in ``plan``, or -tile_qubits."""
+    h = C.c_void_p()
+    tq, chosen = C.c_int(0), C.c_int(0)
+    ms = C.c_double(0)
+    _check(_bind().qsvh_plan_form(backend.ctx, group_model.handle, tile_model.handle,
+                                  circuit.handle, dtype, max_qubits, window, C.byref(h),
+                                  C.byref(tq), C.byref(ms), C.byref(chosen)), backend.ctx)
+    return Circuit(_handle=h), tq.value, ms.value, chosen.value
+
+
 __all__ = ["Mlp", "group_features", "collect_timings", "train_cost_model", "plan",
+           "tile_pass_features", "collect_tile_timings", "train_tile_model", "plan_form",
            "QsvhError"]

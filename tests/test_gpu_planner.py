@@ -99,3 +99,43 @@ def test_planner_gain_over_greedy(backend, cost_model):
           f"vs best greedy k={best_k}: {greedy[best_k] / t_plan:.2f}x")
     assert t_plan <= 1.15 * greedy[best_k]
     st.free()
+
+
+@pytest.fixture(scope="module")
+def tile_model(backend):
+    X, y = planner.collect_tile_timings(backend, sizes=(24,), per_size=160, seed=12)
+    m, loss = planner.train_tile_model(backend, X, y, epochs=1500)
+    print(f"tile model: {len(X)} pass timings, final standardised MSE {loss:.4f}")
+    assert len(X) >= 150
+    return m
+
+
+def test_form_planner_picks_the_faster_form(backend, cost_model, tile_model):
+    """The form planner (fused groups on the matrix cores vs exact tile passes) returns a
+    runnable plan, correct to the fused tolerance against the complex128 plain run, and no
+    slower than the best greedy fused plan or tiles at a fixed size."""
+    n = 22
+    circ = rs.mix(n, 40 * n, 23)
+    ref = backend.execute(circ, rs.C128)
+    st = backend.alloc(n, rs.C64)
+
+    def run(c, tq=0, prec=rs.PREC_MFMA_BF16X3):
+        p = rs.Program(st, c, precision=prec if tq == 0 else rs.PREC_EXACT, tile_qubits=tq)
+        backend.init_basis(st, 0)
+        p.launch()
+        err, fid = backend.compare(ref, st)
+        t = min(p.time() for _ in range(3))
+        return t, err, fid
+
+    planned, tq, pred, chosen = planner.plan_form(backend, cost_model, tile_model, circ, rs.C64)
+    t_plan, err, fid = run(planned, tq)
+    assert err < 1e-5 and 1 - fid < 1e-6, (err, fid)
+    assert (tq == 0) == (chosen >= 0) and (tq == 0 or chosen == -tq)
+    greedy = {k: run(rs.fuse(backend, circ, k))[0] for k in (4, 6)}
+    tiled = {tb: run(circ, tb)[0] for tb in (11, 13)}
+    best = min(min(greedy.values()), min(tiled.values()))
+    print(f"form plan: tile_qubits {tq}, {t_plan:.3f} ms (predicted {pred:.3f}); greedy "
+          f"{greedy}; tiled {tiled}")
+    assert t_plan <= 1.15 * best
+    st.free()
+    ref.free()

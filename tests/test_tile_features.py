@@ -1,0 +1,50 @@
+"""Tile-pass cost-model features (qsvh_tile_pass_features) on the CPU: one pass whose
+lowering is known op by op, so every count is pinned (a miscounted or misclassified op
+moves one of them)."""
+import numpy as np
+
+import rustatevec_amd as rs
+from rustatevec_amd import planner
+
+
+def test_features_count_every_lowered_op():
+    # n = 16, 12 tile bits: the scheduler takes bits 0..3, the action qubits in order and
+    # fills up with the lowest bits, so the tile is qubits 0..11 (e-bit = qubit): lane bits
+    # 0..5, wave bits 6..8, register bits 9..11; qubits 12..15 are outside.
+    c = rs.Circuit(16)
+    c.h(0)                      # lane-run member, lane bit, real (cheap)
+    c.u3(1, 0.1, 0.2, 0.3)      # lane-run member, lane bit, general
+    c.u3(10, 0.4, 0.5, 0.6)     # lane-run member, register bit, general
+    c.h(11)                     # lane-run member, register bit, cheap
+    c.h(7)                      # wave bit: a standalone LDS op (closes the run)
+    c.rz(3, 0.7)                # diagonal, lane bit
+    c.rz(10, 0.8)               # diagonal, register bit
+    c.rz(14, 0.9)               # diagonal, outside: phase uniform per tile
+    c.gate("ccz", [0, 1, 2])    # diagonal on 3 qubits
+    c.swap(4, 5)                # permutation op (closes the run)
+    c.cx(13, 2)                 # lane-run member, lane bit, swap variant, outside control
+    c.cx(2, 1)                  # lane-run member, lane bit, swap variant, conditioned on bit 2
+    rows, other = planner.tile_pass_features(c, rs.C128, 12)
+    assert rows.shape == (1, planner.N_TILE_FEATURES) and other == 0
+    f = rows[0]
+    assert f[0] == 0.0 and np.isclose(f[1], 12 / 13) and np.isclose(f[2], 16 / 32)
+    counts = np.rint(f[3:] * 16).astype(int)
+    want = {3: 1, 4: 1, 5: 1, 6: 1, 7: 1, 8: 3, 9: 1, 10: 1, 11: 0, 12: 0, 13: 0, 14: 1,
+            15: 1, 16: 3, 17: 1}
+    assert {i + 3: int(v) for i, v in enumerate(counts)} == want
+
+
+def test_features_of_the_workloads():
+    # QFT(30) complex128 at 12 tile bits: 6 passes holding its 435 CP, 30 H and 15 SWAP
+    rows, other = planner.tile_pass_features(rs.qft(30), rs.C128, 12)
+    assert rows.shape == (6, planner.N_TILE_FEATURES) and other == 0
+    tot = np.rint(rows[:, 3:] * 16).sum(axis=0).astype(int)
+    assert tot[0:4].sum() == 435            # diagonal gates, all lowering variants
+    assert tot[4:8].sum() + tot[9:11].sum() + tot[11] == 30  # H: run members, block members, LDS
+    assert tot[12] == 15                    # SWAP permutations
+    # a gate a tile cannot hold (4-qubit dense) stays outside the passes
+    c = rs.Circuit(14)
+    q = np.linalg.qr(np.random.default_rng(1).normal(size=(16, 16)))[0]
+    c.h(0).push(q, [0, 5, 9, 13]).h(1)
+    rows, other = planner.tile_pass_features(c, rs.C64, 11)
+    assert other == 1 and len(rows) == 2
