@@ -10,6 +10,8 @@
 // a final kernel adds up per circuit in a fixed order.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "amp.hpp"
 #include "engine.hpp"
 #include "tile_ops.hpp"
@@ -56,64 +58,74 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict_
         hi = mid - 1;
     }
     const BatchPass* bp = passes + lo;
-    const uint32_t tb = bp->tb;
-    const uint32_t N = 1u << tb;
-    const uint32_t* ts = bp->ts;  // uniform: read through the scalar cache
-    const uint64_t t = w - bp->tile_begin;
-    const uint64_t base = tile::insert_zero_bits(t, ts, tb);
-    T* amp = amp_pool + 2 * bp->state_off;
-    const uint32_t flags = bp->flags;
-    // chunk c of lane tid (e = (tid + c NT) A): offset = per-lane part | uniform part
-    constexpr uint32_t LA = tile::ilog2c(kThreads * A);
-    const uint32_t la = tb < LA ? tb : LA;
-    const uint64_t off_lane = tile::offset((threadIdx.x * A) & (N - 1), ts, la);
-    for (uint32_t e = threadIdx.x * A, c = 0; e < N; e += kThreads * A, ++c) {
-      const uint64_t off = off_lane | tile::chunk_offset(c, ts, LA);
-      if (flags & kBatchInit) {
-        // |0...0>: amplitude 1 at state index 0, which is tile 0, local index 0
-#pragma unroll
-        for (int v = 0; v < A; ++v) {
-          sr[e + v] = (t == 0 && e + v == 0) ? T(1) : T(0);
-          si[e + v] = T(0);
-        }
-      } else {
-        T r[A], m[A];
-        cload<T, A>(amp, base + off, r, m);
-#pragma unroll
-        for (int v = 0; v < A; ++v) {
-          sr[e + v] = r[v];
-          si[e + v] = m[v];
+    // A full tile (tile bits = MAXTB, every work item of the circuits that exceed one
+    // tile) takes a copy of the body with the tile size a compile-time constant, so the
+    // per-amplitude bounds checks and chunk loops fold away; smaller tiles take the other.
+    auto tile_body = [&](auto fullc) __attribute__((always_inline)) {
+      constexpr bool FULL = decltype(fullc)::value;
+      const uint32_t tb = FULL ? uint32_t(MAXTB) : bp->tb;
+      const uint32_t N = FULL ? NMAX : (1u << tb);
+      const uint32_t* ts = bp->ts;  // uniform: read through the scalar cache
+      const uint64_t t = w - bp->tile_begin;
+      const uint64_t base = tile::insert_zero_bits(t, ts, tb);
+      T* amp = amp_pool + 2 * bp->state_off;
+      const uint32_t flags = bp->flags;
+      // chunk c of lane tid (e = (tid + c NT) A): offset = per-lane part | uniform part
+      constexpr uint32_t LA = tile::ilog2c(kThreads * A);
+      const uint32_t la = tb < LA ? tb : LA;
+      const uint64_t off_lane = tile::offset((threadIdx.x * A) & (N - 1), ts, la);
+      for (uint32_t e = threadIdx.x * A, c = 0; e < N; e += kThreads * A, ++c) {
+        const uint64_t off = off_lane | tile::chunk_offset(c, ts, LA);
+        if (flags & kBatchInit) {
+          // |0...0>: amplitude 1 at state index 0, which is tile 0, local index 0
+  #pragma unroll
+          for (int v = 0; v < A; ++v) {
+            sr[e + v] = (t == 0 && e + v == 0) ? T(1) : T(0);
+            si[e + v] = T(0);
+          }
+        } else {
+          T r[A], m[A];
+          cload<T, A>(amp, base + off, r, m);
+  #pragma unroll
+          for (int v = 0; v < A; ++v) {
+            sr[e + v] = r[v];
+            si[e + v] = m[v];
+          }
         }
       }
-    }
-    __syncthreads();
-    tile::run_ops<T, kThreads, int(NMAX / kThreads)>(sr, si, N, base, ops + bp->op_begin,
-                                                     bp->n_ops, diags, pool);
-    if (flags & kBatchExpZ0) {
-      double acc = 0.0;
-      for (uint32_t e = threadIdx.x; e < N; e += kThreads) {
-        const double a = double(sr[e]), b = double(si[e]);
-        const double p = a * a + b * b;
-        acc += ((base | tile::offset(e, ts, tb)) & 1u) ? -p : p;
-      }
-      acc = wave_sum(acc);
-      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
       __syncthreads();
-      if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int k = 0; k < kThreads / 64; ++k) s += red[k];
-        partials[bp->part_off + t] = s;
+      tile::run_ops<T, kThreads, int(NMAX / kThreads)>(sr, si, N, base, ops + bp->op_begin,
+                                                       bp->n_ops, diags, pool);
+      if (flags & kBatchExpZ0) {
+        double acc = 0.0;
+        for (uint32_t e = threadIdx.x; e < N; e += kThreads) {
+          const double a = double(sr[e]), b = double(si[e]);
+          const double p = a * a + b * b;
+          acc += ((base | tile::offset(e, ts, tb)) & 1u) ? -p : p;
+        }
+        acc = wave_sum(acc);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          double s = 0.0;
+          for (int k = 0; k < kThreads / 64; ++k) s += red[k];
+          partials[bp->part_off + t] = s;
+        }
       }
-    }
-    for (uint32_t e = threadIdx.x * A, c = 0; e < N; e += kThreads * A, ++c) {
-      T r[A], m[A];
-#pragma unroll
-      for (int v = 0; v < A; ++v) {
-        r[v] = sr[e + v];
-        m[v] = si[e + v];
+      for (uint32_t e = threadIdx.x * A, c = 0; e < N; e += kThreads * A, ++c) {
+        T r[A], m[A];
+  #pragma unroll
+        for (int v = 0; v < A; ++v) {
+          r[v] = sr[e + v];
+          m[v] = si[e + v];
+        }
+        cstore<T, A>(amp, base + (off_lane | tile::chunk_offset(c, ts, LA)), r, m);
       }
-      cstore<T, A>(amp, base + (off_lane | tile::chunk_offset(c, ts, LA)), r, m);
-    }
+    };
+    if (bp->tb == uint32_t(MAXTB))
+      tile_body(std::true_type{});
+    else
+      tile_body(std::false_type{});
     __syncthreads();
   }
 }
