@@ -112,7 +112,7 @@ int plan(Batch* b, int tb, int blocks) {
     if (out.empty()) {  // no gates: one empty pass still builds |0> and <Z_0>
       TilePassArg pa{};
       pa.tb = tbc;
-      for (uint32_t j = 0; j < tbc; ++j) pa.ts[j] = j;
+      for (uint32_t j = 0; j < tbc; ++j) pa.ts[j] = pa.tsort[j] = j;
       pa.op_begin = uint32_t(bld.ops.size());
       pa.n_ops = 0;
       Launch l;
@@ -128,6 +128,7 @@ int plan(Batch* b, int tb, int blocks) {
       bp.n = circ.n;
       bp.tb = pa.tb;
       std::copy(pa.ts, pa.ts + 16, bp.ts);
+      std::copy(pa.tsort, pa.tsort + 16, bp.tsort);
       bp.op_begin = pa.op_begin;
       bp.n_ops = pa.n_ops;
       bp.n_tiles = uint64_t(1) << (circ.n - pa.tb);

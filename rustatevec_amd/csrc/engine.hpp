@@ -173,8 +173,9 @@ struct TileDiag {
   uint32_t loc[kMaxGateQubits];
 };
 struct TilePassArg {
-  uint32_t tb;      // tile bits
-  uint32_t ts[16];  // ascending state bits of the tile
+  uint32_t tb;         // tile bits
+  uint32_t ts[16];     // state bit of tile-local bit j: bits 0..3 first, then the scheduler's order
+  uint32_t tsort[16];  // the same state bits ascending (insertion of a tile's base)
   uint32_t op_begin, n_ops;
 };
 hipError_t launch_tile(const State& st, const TilePassArg& pa, const TileOp* ops,
@@ -190,7 +191,7 @@ struct BatchPass {
   uint64_t tile_begin;
   uint64_t n_tiles;
   uint32_t n, tb;
-  uint32_t ts[16];
+  uint32_t ts[16], tsort[16];  // as in TilePassArg
   uint32_t op_begin, n_ops;
   uint32_t flags;
   uint32_t part_off;  // first partial slot (kBatchExpZ0)

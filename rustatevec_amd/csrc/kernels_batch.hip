@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict_
       const uint32_t N = FULL ? NMAX : (1u << tb);
       const uint32_t* ts = bp->ts;  // uniform: read through the scalar cache
       const uint64_t t = w - bp->tile_begin;
-      const uint64_t base = tile::insert_zero_bits(t, ts, tb);
+      const uint64_t base = tile::insert_zero_bits(t, bp->tsort, tb);
       T* amp = amp_pool + 2 * bp->state_off;
       const uint32_t flags = bp->flags;
       // chunk c of lane tid (e = (tid + c NT) A): offset = per-lane part | uniform part

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   constexpr uint32_t LA = tile::ilog2c(kThreads * A);
   const uint64_t off_lane = tile::offset(threadIdx.x * A, pa.ts, LA);
   for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    const uint64_t base = tile::insert_zero_bits(t, pa.ts, TB);
+    const uint64_t base = tile::insert_zero_bits(t, pa.tsort, TB);
     // A consecutive tile-local indices are A consecutive amplitudes (ts starts 0, 1, 2, 3)
 #pragma unroll
     for (uint32_t c = 0; c < PER; ++c) {

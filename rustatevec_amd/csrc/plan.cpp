@@ -129,15 +129,49 @@ void schedule_tiles(const std::vector<Launch>& in, uint32_t n, uint32_t tb, Tile
         if (popcount64(tmask) == tb) break;
       }
       for (uint32_t b = 0; popcount64(tmask) < tb && b < n; ++b) tmask |= 1ull << b;
+      // Tile-local positions: the low bits first (in order: contiguous runs), then the other
+      // tile qubits by their activity in this pass, least active first.  Positions from
+      // kTileLaneBits up are register bits (a 1-qubit op there pairs amplitudes in a lane's
+      // registers) and positions 4 .. kTileLaneBits - 1 need an LDS round trip, so the
+      // qubits the pass's non-diagonal ops act on most take the register positions.
+      // Activity: 2 per non-diagonal op acting on the qubit, 1 per diagonal gate on it.
+      uint32_t busy[64] = {0};
+      {
+        uint64_t blk = 0;
+        for (uint64_t i : rem) {
+          if ((qm[i] & blk) || (act[i] & ~tmask)) {
+            blk |= qm[i];
+            continue;
+          }
+          for (uint32_t j = 0; j < in[i].m; ++j) {
+            const uint32_t q = in[i].q[j];
+            if (in[i].kind == Launch::kDiag)
+              busy[q] += (tmask >> q) & 1u;
+            else if ((act[i] >> q) & 1u)
+              busy[q] += 2;
+          }
+        }
+      }
+      std::vector<uint32_t> others;
+      for (uint32_t b = 0; b < 64; ++b)
+        if (((tmask & ~low) >> b) & 1u) others.push_back(b);
+      std::stable_sort(others.begin(), others.end(),
+                       [&](uint32_t a, uint32_t c) { return busy[a] < busy[c]; });
       TilePassArg pa{};
       pa.tb = tb;
       uint32_t pos[64] = {0};
-      uint32_t k = 0;
-      for (uint32_t b = 0; b < 64; ++b)
-        if ((tmask >> b) & 1u) {
+      uint32_t k = 0, ks = 0;
+      for (uint32_t b = 0; b < 64; ++b) {
+        if ((tmask >> b) & 1u) pa.tsort[ks++] = b;
+        if ((low >> b) & 1u) {
           pa.ts[k] = b;
           pos[b] = k++;
         }
+      }
+      for (uint32_t b : others) {
+        pa.ts[k] = b;
+        pos[b] = k++;
+      }
       pa.op_begin = uint32_t(bld.ops.size());
       bld.pass_begin = bld.ops.size();
       std::vector<uint64_t> rest;
