@@ -198,32 +198,40 @@ def tile_pass_features(circuit: Circuit, dtype: int, tile_qubits: int):
     return out, n_o.value
 
 
+# gate kinds of the training passes, in the order of the cumulative weights below
+_PASS_KINDS = ("u3", "real", "x", "rz", "diag2", "cx", "swap", "ccx")
+
+
 def _random_pass(rng, n, tb):
-    """A random circuit that the tile scheduler runs as one pass of tb tile bits: gates of
-    the mixed workloads with their action qubits in one random tile set (controls and the
-    qubits of diagonal gates anywhere)."""
+    """A random circuit that the tile scheduler runs as one pass of tb tile bits: 1 to 160
+    gates with their action qubits in one random tile set (controls and the qubits of
+    diagonal gates anywhere), drawn from a gate-kind mix that is itself random per pass
+    (Dirichlet), so passes range from the mixed workloads' to layered circuits' (U3, RZ,
+    CZ) and single-kind runs."""
     tile = [0, 1, 2, 3] + sorted(int(x) for x in rng.choice(np.arange(4, n), size=tb - 4,
                                                            replace=False))
+    w = rng.dirichlet(np.full(len(_PASS_KINDS), 0.7))
+    cum = np.cumsum(w)
     c = Circuit(n)
-    for _ in range(int(rng.integers(1, 65))):
-        r = rng.random()
+    for _ in range(int(rng.integers(1, 161))):
+        kind = _PASS_KINDS[min(int(np.searchsorted(cum, rng.random())), len(_PASS_KINDS) - 1)]
         t = int(rng.choice(tile))
         o = int(rng.choice([q for q in range(n) if q != t]))
         a = lambda: float(rng.uniform(0, 2 * np.pi))  # noqa: E731
-        if r < 0.25:
+        if kind == "u3":
             c.u3(t, a(), a(), a())
-        elif r < 0.40:
+        elif kind == "real":
             c.ry(t, a()) if rng.random() < 0.5 else c.h(t)
-        elif r < 0.45:
+        elif kind == "x":
             c.x(t)
-        elif r < 0.60:
+        elif kind == "rz":
             c.rz(int(rng.integers(n)), a())
-        elif r < 0.80:
+        elif kind == "diag2":
             k = rng.integers(3)
             c.cz(t, o) if k == 0 else (c.cphase(o, t, a()) if k == 1 else c.rzz(o, t, a()))
-        elif r < 0.95:
+        elif kind == "cx":
             c.cx(o, t)
-        elif r < 0.98:
+        elif kind == "swap":
             u = int(rng.choice([q for q in tile if q != t]))
             c.swap(t, u)
         else:
@@ -232,7 +240,7 @@ def _random_pass(rng, n, tb):
     return c
 
 
-def collect_tile_timings(backend: HipBackend, sizes=(26,), per_size=300, seed=7, dtype=C64,
+def collect_tile_timings(backend: HipBackend, sizes=(26,), per_size=400, seed=7, dtype=C64,
                          reps=3):
     """Time random single tile passes (tile bits 11..13 for complex64, 10..12 for
     complex128) on HBM-resident states; returns pass features X and y = log2 of the
