@@ -154,6 +154,22 @@ def main():
     bytes_step = passes * (1 << n) * 32
     value = bytes_step / step_s / 1e9
     log(f"QFT({n}) plain c128: {passes} passes, {step_s:.4f} s/step, {value:.1f} GB/s")
+    # the state across the ABI (SoA host arrays <-> interleaved HBM): download and upload
+    # rates, and the circuit rate with both transfers included (never the reported value)
+    t0 = time.perf_counter()
+    host_re, host_im = b.download_parts(st)
+    dl_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.upload(st, re=host_re, im=host_im)
+    b.synchronize(st)
+    ul_s = time.perf_counter() - t0
+    del host_re, host_im
+    sbytes = (1 << n) * 16
+    host_rec = {"state_bytes": sbytes, "download_s": round(dl_s, 3), "upload_s": round(ul_s, 3),
+                "download_gbs": round(sbytes / dl_s / 1e9, 2),
+                "upload_gbs": round(sbytes / ul_s / 1e9, 2),
+                "pcie_inclusive_gbs": round(bytes_step / (step_s + dl_s + ul_s) / 1e9, 1)}
+    log(f"host transfers of the {n}q state: {host_rec}")
     del prog
     reset()
     rs.Program(st, qft).launch()  # the complex128 reference result stays resident
@@ -492,6 +508,7 @@ def main():
         "copy_kernel_gbs": round(copy_gbs, 1),
         "h_gate_gbs": {str(q): round(per_launch / (t * 1e-3) / 1e9, 1) for q, t in ms_h.items()},
         "tiled": tiled,
+        "host_transfer": host_rec,
         "plain_subspace": subspace,
         "sampling": sampling,
         "fused": fused_out,

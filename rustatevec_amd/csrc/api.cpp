@@ -190,42 +190,47 @@ void program_free(Program* p) {
 }
 
 
-// Host SoA <-> device interleaved, through a host staging buffer of at most 2^22
-// amplitudes per round.
+// Host SoA <-> device interleaved: chunks of up to 2^24 amplitudes are split (joined) on the
+// device between the state and a device staging pair, and the host copies move whole re / im
+// runs straight to (from) the caller's arrays -- no per-amplitude host loop.
 template <typename T>
 static hipError_t split_t(const State& s, T* re, T* im) {
   const uint64_t dim = s.dim();
-  const uint64_t chunk = std::min<uint64_t>(dim, uint64_t(1) << 22);
-  std::vector<T> buf(2 * chunk);
-  for (uint64_t a = 0; a < dim; a += chunk) {
+  const uint64_t chunk = std::min<uint64_t>(dim, uint64_t(1) << 24);
+  hipStream_t str = s.ctx->stream;
+  T* d = nullptr;
+  hipError_t e = hipMalloc(&d, 2 * chunk * sizeof(T));
+  if (e != hipSuccess) return e;
+  for (uint64_t a = 0; a < dim && e == hipSuccess; a += chunk) {
     const uint64_t c = std::min(chunk, dim - a);
-    const hipError_t e = hipMemcpy(buf.data(), static_cast<const T*>(s.amp) + 2 * a,
-                                   2 * c * sizeof(T), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return e;
-    for (uint64_t i = 0; i < c; ++i) {
-      re[a + i] = buf[2 * i];
-      im[a + i] = buf[2 * i + 1];
-    }
+    e = launch_split(s, a, c, d, d + chunk, str);
+    if (e == hipSuccess) e = hipMemcpyAsync(re + a, d, c * sizeof(T), hipMemcpyDeviceToHost, str);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(im + a, d + chunk, c * sizeof(T), hipMemcpyDeviceToHost, str);
   }
-  return hipSuccess;
+  const hipError_t f = hipStreamSynchronize(str);
+  hipFree(d);
+  return e != hipSuccess ? e : f;
 }
 
 template <typename T>
 static hipError_t join_t(const State& s, const T* re, const T* im) {
   const uint64_t dim = s.dim();
-  const uint64_t chunk = std::min<uint64_t>(dim, uint64_t(1) << 22);
-  std::vector<T> buf(2 * chunk);
-  for (uint64_t a = 0; a < dim; a += chunk) {
+  const uint64_t chunk = std::min<uint64_t>(dim, uint64_t(1) << 24);
+  hipStream_t str = s.ctx->stream;
+  T* d = nullptr;
+  hipError_t e = hipMalloc(&d, 2 * chunk * sizeof(T));
+  if (e != hipSuccess) return e;
+  for (uint64_t a = 0; a < dim && e == hipSuccess; a += chunk) {
     const uint64_t c = std::min(chunk, dim - a);
-    for (uint64_t i = 0; i < c; ++i) {
-      buf[2 * i] = re[a + i];
-      buf[2 * i + 1] = im[a + i];
-    }
-    const hipError_t e = hipMemcpy(static_cast<T*>(s.amp) + 2 * a, buf.data(),
-                                   2 * c * sizeof(T), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(d, re + a, c * sizeof(T), hipMemcpyHostToDevice, str);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d + chunk, im + a, c * sizeof(T), hipMemcpyHostToDevice, str);
+    if (e == hipSuccess) e = launch_join(s, a, c, d, d + chunk, str);
   }
-  return hipSuccess;
+  const hipError_t f = hipStreamSynchronize(str);
+  hipFree(d);
+  return e != hipSuccess ? e : f;
 }
 
 hipError_t download_split(const State& s, void* re, void* im) {

@@ -112,6 +112,12 @@ hipError_t launch_copy(const State& src, void* dst, const Tuning& t, hipStream_t
 // precision (api.cpp).
 hipError_t download_split(const State& s, void* re, void* im);
 hipError_t upload_join(const State& s, const void* re, const void* im);
+// kernels_util.hip: amplitudes [first, first + count) of the state to / from separate device
+// re / im arrays.
+hipError_t launch_split(const State& st, uint64_t first, uint64_t count, void* re, void* im,
+                        hipStream_t s);
+hipError_t launch_join(const State& st, uint64_t first, uint64_t count, const void* re,
+                       const void* im, hipStream_t s);
 
 // ---- kernels_mfma.hip ------------------------------------------------------------------
 // Fused k-qubit group (k = 4..6, qubits `q` ascending) on a complex64 state of >= 11

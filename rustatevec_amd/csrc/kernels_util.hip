@@ -5,6 +5,8 @@
 //   output of SplitMix64 is mix(seed + (k + 1) * gamma), so every amplitude is generated
 //   independently (counter-based) yet equals what one sequential stream would give.
 // * Comparison of two states in f64: max_i |a_i - b_i|, <a|b>, <a|a>, <b|b>.
+// * The ABI's layout: amplitudes [first, first + count) of the interleaved state to / from
+//   separate re and im arrays in device memory (staging of the host transfers).
 #include <hip/hip_runtime.h>
 
 #include "engine.hpp"
@@ -141,7 +143,65 @@ hipError_t compare_t(const State& a, const State& b, double* part, double* out, 
   return hipGetLastError();
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ amp, T* __restrict__ re,
+                                                  T* __restrict__ im, uint64_t count) {
+  using V = T __attribute__((ext_vector_type(2)));
+  const V* a = reinterpret_cast<const V*>(amp);
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const V v = a[i];
+    re[i] = v.x;
+    im[i] = v.y;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_join(T* __restrict__ amp, const T* __restrict__ re,
+                                                 const T* __restrict__ im, uint64_t count) {
+  using V = T __attribute__((ext_vector_type(2)));
+  V* a = reinterpret_cast<V*>(amp);
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    V v;
+    v.x = re[i];
+    v.y = im[i];
+    a[i] = v;
+  }
+}
+
+unsigned stage_grid(uint64_t count) {
+  const uint64_t g = (count + kBlock - 1) / kBlock;
+  return unsigned(g < 65536 ? (g ? g : 1) : 65536);
+}
+
 }  // namespace
+
+hipError_t launch_split(const State& st, uint64_t first, uint64_t count, void* re, void* im,
+                        hipStream_t s) {
+  if (st.dtype == 0)
+    k_split<double><<<stage_grid(count), kBlock, 0, s>>>(
+        static_cast<const double*>(st.amp) + 2 * first, static_cast<double*>(re),
+        static_cast<double*>(im), count);
+  else
+    k_split<float><<<stage_grid(count), kBlock, 0, s>>>(
+        static_cast<const float*>(st.amp) + 2 * first, static_cast<float*>(re),
+        static_cast<float*>(im), count);
+  return hipGetLastError();
+}
+
+hipError_t launch_join(const State& st, uint64_t first, uint64_t count, const void* re,
+                       const void* im, hipStream_t s) {
+  if (st.dtype == 0)
+    k_join<double><<<stage_grid(count), kBlock, 0, s>>>(
+        static_cast<double*>(st.amp) + 2 * first, static_cast<const double*>(re),
+        static_cast<const double*>(im), count);
+  else
+    k_join<float><<<stage_grid(count), kBlock, 0, s>>>(
+        static_cast<float*>(st.amp) + 2 * first, static_cast<const float*>(re),
+        static_cast<const float*>(im), count);
+  return hipGetLastError();
+}
 
 hipError_t launch_init_random(const State& st, uint64_t seed, double* part, int max_part,
                               double* d_norm, hipStream_t s) {
