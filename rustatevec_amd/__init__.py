@@ -378,8 +378,8 @@ class Program:
             self._h = None
 
 
-TUNE_KEYS = ("max_blocks", "unroll", "nontemporal", "mfma_blocks", "tile_blocks", "contig",
-             "xcd")
+TUNE_KEYS = ("max_blocks", "unroll", "nontemporal", "mfma_blocks", "mfma_layout",
+             "tile_blocks", "contig", "xcd")
 
 
 class HipBackend:

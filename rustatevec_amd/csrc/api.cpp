@@ -308,8 +308,10 @@ int qsvh_tune_set(qsvh_ctx* h, const char* key, int64_t value) {
     t.unroll = int(value);
   } else if (k == "nontemporal" && value >= 0 && value <= 2) {
     t.nontemporal = int(value);
-  } else if (k == "mfma_blocks" && value >= 1 && value <= (1ll << 20)) {
+  } else if (k == "mfma_blocks" && value >= 0 && value <= (1ll << 20)) {
     t.mfma_blocks = int(value);
+  } else if (k == "mfma_layout" && value >= 0 && value <= 2) {
+    t.mfma_layout = int(value);
   } else if (k == "tile_blocks" && value >= 1 && value <= (1ll << 20)) {
     t.tile_blocks = int(value);
   } else if (k == "contig" && (value == 0 || value == 1)) {
@@ -330,6 +332,7 @@ int qsvh_tune_get(qsvh_ctx* h, const char* key, int64_t* value) {
   else if (k == "unroll") *value = t.unroll;
   else if (k == "nontemporal") *value = t.nontemporal;
   else if (k == "mfma_blocks") *value = t.mfma_blocks;
+  else if (k == "mfma_layout") *value = t.mfma_layout;
   else if (k == "tile_blocks") *value = t.tile_blocks;
   else if (k == "contig") *value = t.contig;
   else if (k == "xcd") *value = t.xcd;

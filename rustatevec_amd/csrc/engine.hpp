@@ -19,7 +19,11 @@ struct Tuning {
   int unroll = 1;              // independent vector items per thread per iteration
   int nontemporal = 2;         // 1: non-temporal (streaming) loads and stores; 2: where
                                //    they pay (streaming kernels, states beyond the MALL)
-  int mfma_blocks = 2048;      // workgroups of the persistent MFMA group kernels
+  int mfma_blocks = 0;         // workgroups of the persistent MFMA group kernels; 0: by
+                               // precision (measured: 8 per CU for FP32, 2 per CU for
+                               // split-BF16, whose HBM rate drops with more in flight)
+  int mfma_layout = 0;         // FP32 group kernel's LDS layout: 0 by the group's qubits,
+                               // 1 row-major, 2 column-major
   int tile_blocks = 1024;      // workgroups of the persistent tile-pass kernel
   int contig = 0;              // 1: a thread's unrolled items are adjacent (else strided)
   // 1: blocks b and b + 1 of a streaming launch take neighbouring address ranges of one

@@ -74,30 +74,50 @@ __device__ __forceinline__ void tile_rc(uint32_t tl, const TileMap& m, uint32_t&
   }
 }
 
+// LDS position of tile-local index tl: row s, column n at s * rs + n * cs.
+__device__ __forceinline__ uint32_t tile_pos(uint32_t tl, const TileMap& m, uint32_t rs,
+                                             uint32_t cs) {
+  uint32_t s, n;
+  tile_rc(tl, m, s, n);
+  return s * rs + n * cs;
+}
+
 // Each lane moves four 16-byte chunks of two interleaved amplitudes per tile
 // (2048 / 2 / 256 = 4); tile-local bit 0 is state bit 0, so a chunk is consecutive.
+// Chunk c of a lane holds tile-local indices 2 (tid + 256 c) + {0, 1}: against chunk 0 it
+// sets tile-local bits 9 and 10, its second amplitude bit 0.  So a chunk's state offset and
+// LDS positions are the lane's chunk-0 values plus terms that are the same for every lane
+// (kept in scalar registers); the lane keeps one offset and one position, not eight.
 constexpr int kChunks = (1 << kTileBits) / 2 / kBlock;
+constexpr uint32_t chunk_tl(int c, int e) { return uint32_t(2 * kBlock * c + e); }
 
 // ------------------------------------------------------------------------- FP32
-// LDS layout [row][col], row stride chosen = 32 (mod 64) floats so that the two half-waves
-// of a B-fragment read (rows r and r+1) fall in disjoint banks.
-template <int K>
+// The tile sits in LDS as the real 2D x NT block operand (block rows 0..D-1 = Xr,
+// D..2D-1 = Xi), element (row, col) at row * RS + col * CS.  Row-major (RS = NT + pad,
+// CS = 1) keeps the B-fragment reads conflict-free and suits tiles whose lowest bits
+// (the ones consecutive lanes step through when staging) are column bits; column-major
+// (RS = 1, CS = 2D + 33) suits tiles whose lowest bits are group (row) bits, which
+// row-major would stage into a single bank.  CS = 2D + 33 puts the 32 columns of a
+// half-wave's B read on distinct banks and the two half-waves (rows r, r + 1) on
+// disjoint ones.
+template <int K, bool CM>
 struct F32Shape {
   static constexpr int D = 1 << K;
   static constexpr int NT = 1 << (kTileBits - K);  // columns per tile
-  static constexpr int STRIDE = NT + (NT >= 64 ? 32 : 0);
+  static constexpr int RS = CM ? 1 : NT + (NT >= 64 ? 32 : 0);
+  static constexpr int CS = CM ? 2 * D + 33 : 1;
+  static constexpr int WORDS = CM ? NT * CS : 2 * D * RS;
   static constexpr int NTILE_COLS = NT / 32;       // 32-wide output blocks per row band
 };
 
-template <int K, bool NTA>
+template <int K, bool NTA, bool CM>
 __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
                                                       uint64_t n_tiles, TileMap map,
                                                       const float* __restrict__ ablock,
                                                       int xcd) {
-  using S = F32Shape<K>;
-  constexpr int D = S::D, R = 2 * D, STRIDE = S::STRIDE;
-  __shared__ float xr[D * STRIDE];
-  __shared__ float xi[D * STRIDE];
+  using S = F32Shape<K, CM>;
+  constexpr int D = S::D, R = 2 * D, RS = S::RS, CS = S::CS;
+  __shared__ float xb[S::WORDS];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int mt = wave / S::NTILE_COLS, nt = wave % S::NTILE_COLS;  // this wave's 32x32 block
@@ -108,20 +128,11 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
   for (int kk = 0; kk < D; ++kk)
     a[kk] = ablock[(32 * mt + (lane & 31)) * R + 2 * kk + (lane >> 5)];
 
-  // Per-lane chunk offsets and LDS positions (the same for every tile).
-  uint64_t off[kChunks];
-  uint32_t pos[kChunks][2];
-#pragma unroll
-  for (int c = 0; c < kChunks; ++c) {
-    const uint32_t tl = 2u * (threadIdx.x + kBlock * c);
-    off[c] = tile_offset(tl, map);
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      uint32_t s, n;
-      tile_rc(tl + e, map, s, n);
-      pos[c][e] = s * STRIDE + n;
-    }
-  }
+  // The lane's chunk-0 offset and LDS position (re; im D rows further).
+  uint64_t off0 = tile_offset(2u * threadIdx.x, map);
+  uint32_t pos0 = tile_pos(2u * threadIdx.x, map, RS, CS);
+  auto off = [&](int c) { return off0 | tile_offset(chunk_tl(c, 0), map); };
+  auto pos = [&](int c, int e) { return pos0 + tile_pos(chunk_tl(c, e), map, RS, CS); };
 
   floatx4 v[kChunks];
   uint64_t t = xcd_block(xcd);  // consecutive tiles stay on one XCD
@@ -129,18 +140,22 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
     const uint64_t base = tile_base(t, map);
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
-      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (base + off[c])));
+      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (base + off(c))));
     }
   }
   for (; t < n_tiles; t += gridDim.x) {
+    // opaque to the compiler, so that the chunk offsets are formed here from the one
+    // offset and position instead of being held in registers across the loop
+    asm volatile("" : "+v"(off0), "+v"(pos0));
     const uint64_t base = tile_base(t, map);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < kChunks; ++c)
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
-        xr[pos[c][e]] = v[c][2 * e];
-        xi[pos[c][e]] = v[c][2 * e + 1];
+        const uint32_t q = pos(c, e);
+        xb[q] = v[c][2 * e];
+        xb[q + D * RS] = v[c][2 * e + 1];
       }
     __syncthreads();
     const uint64_t tn = t + gridDim.x;
@@ -148,26 +163,23 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
       const uint64_t nb = tile_base(tn, map);
 #pragma unroll
       for (int c = 0; c < kChunks; ++c) {
-        v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (nb + off[c])));
+        v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (nb + off(c))));
       }
     }
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
                     0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int col = 32 * nt + (lane & 31);
+    const int bcol = (lane >> 5) * RS + col * CS;
 #pragma unroll
     for (int kk = 0; kk < D; ++kk) {
-      const int row = 2 * kk + (lane >> 5);  // block row of the B operand
-      const float b = row < D ? xr[row * STRIDE + col] : xi[(row - D) * STRIDE + col];
+      const float b = xb[bcol + 2 * kk * RS];  // B[2 kk + lane / 32][col]
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[kk], b, acc, 0, 0, 0);
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = 32 * mt + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-      if (row < D)
-        xr[row * STRIDE + col] = acc[r];
-      else
-        xi[(row - D) * STRIDE + col] = acc[r];
+      xb[row * RS + col * CS] = acc[r];
     }
     __syncthreads();
 #pragma unroll
@@ -175,25 +187,33 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
       floatx4 y;
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
-        y[2 * e] = xr[pos[c][e]];
-        y[2 * e + 1] = xi[pos[c][e]];
+        const uint32_t q = pos(c, e);
+        y[2 * e] = xb[q];
+        y[2 * e + 1] = xb[q + D * RS];
       }
-      vstore<NTA>(reinterpret_cast<floatx4*>(amp + 2 * (base + off[c])), y);
+      vstore<NTA>(reinterpret_cast<floatx4*>(amp + 2 * (base + off(c))), y);
     }
   }
 }
 
 // ------------------------------------------------------------------------- BF16x3
-// LDS layout [col][block row] (block rows 0..D-1 = Xr, D..2D-1 = Xi), so that the eight
-// consecutive K values of a bf16 B fragment are contiguous.
+// The tile is split into bf16 hi and lo parts once, as it is staged: LDS holds the hi and
+// lo B operands [col][block row] (block rows 0..D-1 = Xr, D..2D-1 = Xi), so that the eight
+// consecutive K values of a B fragment are one 16-byte read and the MFMA loop converts
+// nothing.  The FP32 result goes back through the same LDS as [col][block row] floats.
 template <int K>
 struct B16Shape {
   static constexpr int D = 1 << K;
   static constexpr int R = 2 * D;
   static constexpr int NT = 1 << (kTileBits - K);
-  static constexpr int CSTRIDE = R + 4;  // floats per column (padding against conflicts)
+  static constexpr int HSTRIDE = R + 8;  // bf16 per column of a B operand (16-byte rows,
+                                         // 4-dword skew: conflict-free fragment reads)
+  static constexpr int CSTRIDE = R + 4;  // floats per column of the result
   static constexpr int NTILE_COLS = NT / 32;
   static constexpr int KSTEPS = R / 16;
+  static constexpr int B_BYTES = 2 * NT * HSTRIDE * 2;
+  static constexpr int Y_BYTES = NT * CSTRIDE * 4;
+  static constexpr int LDS_BYTES = B_BYTES > Y_BYTES ? B_BYTES : Y_BYTES;
 };
 
 __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
@@ -208,8 +228,11 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
                                                          const __bf16* __restrict__ alo,
                                                          int xcd) {
   using S = B16Shape<K>;
-  constexpr int D = S::D, R = S::R, CS = S::CSTRIDE, NT = S::NT;
-  __shared__ float xb[NT * CS];
+  constexpr int D = S::D, R = S::R, HS = S::HSTRIDE, CS = S::CSTRIDE, NT = S::NT;
+  __shared__ __align__(16) unsigned char lds[S::LDS_BYTES];
+  __bf16* bhs = reinterpret_cast<__bf16*>(lds);  // hi B operand, NT x HS
+  __bf16* bls = bhs + NT * HS;                    // lo B operand
+  float* yb = reinterpret_cast<float*>(lds);      // result, NT x CS
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int mt = wave / S::NTILE_COLS, nt = wave % S::NTILE_COLS;
@@ -226,19 +249,14 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
     }
   }
 
-  uint64_t off[kChunks];
-  uint32_t pos[kChunks][2];
-#pragma unroll
-  for (int c = 0; c < kChunks; ++c) {
-    const uint32_t tl = 2u * (threadIdx.x + kBlock * c);
-    off[c] = tile_offset(tl, map);
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      uint32_t s, n;
-      tile_rc(tl + e, map, s, n);
-      pos[c][e] = n * CS + s;  // re at +0, im at +D
-    }
-  }
+  // The lane's chunk-0 offset and LDS positions (column-major, re at +0, im at +D) in
+  // bf16 units for the B operands and in floats for the result.
+  uint64_t off0 = tile_offset(2u * threadIdx.x, map);
+  uint32_t hpos0 = tile_pos(2u * threadIdx.x, map, 1, HS);
+  uint32_t ypos0 = tile_pos(2u * threadIdx.x, map, 1, CS);
+  auto off = [&](int c) { return off0 | tile_offset(chunk_tl(c, 0), map); };
+  auto hpos = [&](int c, int e) { return hpos0 + tile_pos(chunk_tl(c, e), map, 1, HS); };
+  auto ypos = [&](int c, int e) { return ypos0 + tile_pos(chunk_tl(c, e), map, 1, CS); };
 
   floatx4 v[kChunks];
   uint64_t t = xcd_block(xcd);  // consecutive tiles stay on one XCD
@@ -246,18 +264,27 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
     const uint64_t base = tile_base(t, map);
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
-      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (base + off[c])));
+      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (base + off(c))));
     }
   }
   for (; t < n_tiles; t += gridDim.x) {
+    // opaque to the compiler, so that the chunk offsets are formed here from the one
+    // offset and position instead of being held in registers across the loop
+    asm volatile("" : "+v"(off0), "+v"(hpos0), "+v"(ypos0));
     const uint64_t base = tile_base(t, map);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < kChunks; ++c)
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
-        xb[pos[c][e]] = v[c][2 * e];
-        xb[pos[c][e] + D] = v[c][2 * e + 1];
+        const uint32_t h = hpos(c, e);
+        __bf16 hr, lr, hi_, li;
+        split_bf16(v[c][2 * e], hr, lr);
+        split_bf16(v[c][2 * e + 1], hi_, li);
+        bhs[h] = hr;
+        bls[h] = lr;
+        bhs[h + D] = hi_;
+        bls[h + D] = li;
       }
     __syncthreads();
     const uint64_t tn = t + gridDim.x;
@@ -265,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
       const uint64_t nb = tile_base(tn, map);
 #pragma unroll
       for (int c = 0; c < kChunks; ++c) {
-        v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (nb + off[c])));
+        v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (nb + off(c))));
       }
     }
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
@@ -273,17 +300,9 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
     const int col = 32 * nt + (lane & 31);
 #pragma unroll
     for (int ks = 0; ks < S::KSTEPS; ++ks) {
-      const float* src = xb + col * CS + 16 * ks + 8 * (lane >> 5);
-      const floatx4 b0 = *reinterpret_cast<const floatx4*>(src);
-      const floatx4 b1 = *reinterpret_cast<const floatx4*>(src + 4);
-      bf16x8 bh, bl;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        __bf16 h, l;
-        split_bf16(j < 4 ? b0[j] : b1[j - 4], h, l);
-        bh[j] = h;
-        bl[j] = l;
-      }
+      const int at = col * HS + 16 * ks + 8 * (lane >> 5);
+      const bf16x8 bh = *reinterpret_cast<const bf16x8*>(bhs + at);
+      const bf16x8 bl = *reinterpret_cast<const bf16x8*>(bls + at);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bh, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bl, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[ks], bh, acc, 0, 0, 0);
@@ -292,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = 32 * mt + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-      xb[col * CS + row] = acc[r];
+      yb[col * CS + row] = acc[r];
     }
     __syncthreads();
 #pragma unroll
@@ -300,10 +319,11 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
       floatx4 y;
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
-        y[2 * e] = xb[pos[c][e]];
-        y[2 * e + 1] = xb[pos[c][e] + D];
+        const uint32_t q = ypos(c, e);
+        y[2 * e] = yb[q];
+        y[2 * e + 1] = yb[q + D];
       }
-      vstore<NTA>(reinterpret_cast<floatx4*>(amp + 2 * (base + off[c])), y);
+      vstore<NTA>(reinterpret_cast<floatx4*>(amp + 2 * (base + off(c))), y);
     }
   }
 }
@@ -312,12 +332,22 @@ template <int K, bool NT>
 hipError_t launch_k(const State& st, const TileMap& map, const void* mats, int precision,
                     const Tuning& t, hipStream_t s) {
   const uint64_t n_tiles = st.dim() >> kTileBits;
-  uint64_t g = n_tiles;
-  if (g > uint64_t(t.mfma_blocks)) g = t.mfma_blocks;
+  const int cus = st.ctx && st.ctx->cus > 0 ? st.ctx->cus : 256;
+  const uint64_t cap = t.mfma_blocks > 0 ? uint64_t(t.mfma_blocks)
+                                         : uint64_t(precision == 1 ? 8 : 2) * cus;
+  const uint64_t g = n_tiles < cap ? n_tiles : cap;
   float* amp = static_cast<float*>(st.amp);
   if (precision == 1) {
-    k_fused_f32<K, NT><<<unsigned(g), kBlock, 0, s>>>(amp, n_tiles, map,
-                                                      static_cast<const float*>(mats), t.xcd);
+    // Column-major staging when most of tile-local bits 1..5 (stepped through by the 32
+    // lanes of a half-wave) are group bits; knob 1 / 2 forces row- / column-major.
+    int lane_rows = 0;
+    for (int j = 1; j <= 5; ++j) lane_rows += (map.role[j] & 0x80u) ? 1 : 0;
+    const bool cm = t.mfma_layout == 2 || (t.mfma_layout == 0 && lane_rows >= 3);
+    const float* a = static_cast<const float*>(mats);
+    if (cm)
+      k_fused_f32<K, NT, true><<<unsigned(g), kBlock, 0, s>>>(amp, n_tiles, map, a, t.xcd);
+    else
+      k_fused_f32<K, NT, false><<<unsigned(g), kBlock, 0, s>>>(amp, n_tiles, map, a, t.xcd);
   } else {
     constexpr size_t n = size_t(4) << (2 * K);  // (2D)^2 entries
     const __bf16* hi = static_cast<const __bf16*>(mats);

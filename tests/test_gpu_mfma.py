@@ -130,3 +130,30 @@ def test_mfma_precision_needs_c64(backend):
         rs.Program(st, rs.qft(12), precision=rs.PREC_MFMA_F32)
     assert e.value.status == "DTYPE"
     st.free()
+
+
+def test_fp32_lds_layouts_and_grid_sizes_agree_bitwise(backend):
+    # The FP32 group kernel stages a tile row-major or column-major in LDS (chosen from the
+    # group's qubits, or forced by the "mfma_layout" knob) and runs on any persistent grid:
+    # each output is the same chain of MFMA products either way, so every layout and grid
+    # gives bit-identical amplitudes, and the automatic choice matches the forced ones.
+    n = 16
+    rng = np.random.default_rng(17)
+    psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+    psi /= np.linalg.norm(psi)
+    old = {k: v for k, v in backend.tune().items() if k in ("mfma_layout", "mfma_blocks")}
+    try:
+        for i, qs in enumerate([(0, 1, 2, 3, 4, 5), (1, 2, 3, 4), (3, 9, 14, 15, 10, 12),
+                                (0, 2, 4, 13, 11)]):
+            U = random_unitary(1 << len(qs), 40 + i)
+            circ = rs.Circuit(n).push(U, qs)
+            want = oracle.apply(oracle.CPU_SERIAL, n, psi.astype(np.complex128), U, qs)
+            outs = []
+            for lay, blocks in ((0, 0), (1, 0), (2, 0), (1, 3), (2, 512), (0, 2048)):
+                backend.tune(mfma_layout=lay, mfma_blocks=blocks)
+                outs.append(run_c64(backend, n, circ, psi, rs.PREC_MFMA_F32))
+            for o in outs[1:]:
+                assert np.array_equal(o, outs[0]), qs
+            assert max_abs_diff(outs[0], want) / np.abs(want).max() < 2e-6, qs
+    finally:
+        backend.tune(**old)
