@@ -93,7 +93,7 @@ int qsvh_abi_version(void);
  * are a grid apart; 1: adjacent), "xcd" (1: consecutive blocks of a streaming kernel take
  * neighbouring address ranges within one XCD; 0: launch order); "mfma_blocks" and
  * "tile_blocks" (workgroups of the persistent matrix-core group and tile-pass kernels;
- * "mfma_blocks" 0, the default: 8 per CU in FP32, 2 per CU in split-BF16);
+ * "mfma_blocks" 0, the default: 1 per CU for FP32 4-qubit groups, else 2 per CU);
  * "mfma_layout" (LDS layout of the FP32 group kernel: 0 chosen from the group's qubits,
  * 1 row-major, 2 column-major).
  * No reference counterpart: the

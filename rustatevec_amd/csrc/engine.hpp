@@ -20,8 +20,8 @@ struct Tuning {
   int nontemporal = 2;         // 1: non-temporal (streaming) loads and stores; 2: where
                                //    they pay (streaming kernels, states beyond the MALL)
   int mfma_blocks = 0;         // workgroups of the persistent MFMA group kernels; 0: by
-                               // precision (measured: 8 per CU for FP32, 2 per CU for
-                               // split-BF16, whose HBM rate drops with more in flight)
+                               // form (measured: 1 per CU for FP32 at k = 4, else 2 per
+                               // CU; the HBM rate drops with more tiles in flight)
   int mfma_layout = 0;         // FP32 group kernel's LDS layout: 0 by the group's qubits,
                                // 1 row-major, 2 column-major
   int tile_blocks = 1024;      // workgroups of the persistent tile-pass kernel

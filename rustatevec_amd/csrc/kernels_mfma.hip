@@ -127,6 +127,10 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
 #pragma unroll
   for (int kk = 0; kk < D; ++kk)
     a[kk] = ablock[(32 * mt + (lane & 31)) * R + 2 * kk + (lane >> 5)];
+  // Complete these loads here: left pending, their waits land inside the tile loop, where
+  // (memory operations completing in order) they also wait for the next tile's prefetch.
+#pragma unroll
+  for (int kk = 0; kk < D; ++kk) asm volatile("" : "+v"(a[kk]));
 
   // The lane's chunk-0 offset and LDS position (re; im D rows further).
   uint64_t off0 = tile_offset(2u * threadIdx.x, map);
@@ -134,21 +138,19 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
   auto off = [&](int c) { return off0 | tile_offset(chunk_tl(c, 0), map); };
   auto pos = [&](int c, int e) { return pos0 + tile_pos(chunk_tl(c, e), map, RS, CS); };
 
+  // Per tile: the next tile's loads are issued, the matrix cores work on this tile (staged
+  // in LDS), the result is stored, and the next tile is staged.  Staging the first tile
+  // ahead of the loop keeps one memory-wait pattern in the loop, so that each staging
+  // waits for its own loads only, not for the stores issued after them.
   floatx4 v[kChunks];
-  uint64_t t = xcd_block(xcd);  // consecutive tiles stay on one XCD
-  if (t < n_tiles) {
-    const uint64_t base = tile_base(t, map);
+  const uint64_t G = gridDim.x;
+  auto load = [&](uint64_t tt) {
+    const uint64_t b = tile_base(tt, map);
 #pragma unroll
-    for (int c = 0; c < kChunks; ++c) {
-      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (base + off(c))));
-    }
-  }
-  for (; t < n_tiles; t += gridDim.x) {
-    // opaque to the compiler, so that the chunk offsets are formed here from the one
-    // offset and position instead of being held in registers across the loop
-    asm volatile("" : "+v"(off0), "+v"(pos0));
-    const uint64_t base = tile_base(t, map);
-    __syncthreads();
+    for (int c = 0; c < kChunks; ++c)
+      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (b + off(c))));
+  };
+  auto stage = [&]() {
 #pragma unroll
     for (int c = 0; c < kChunks; ++c)
 #pragma unroll
@@ -157,15 +159,20 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
         xb[q] = v[c][2 * e];
         xb[q + D * RS] = v[c][2 * e + 1];
       }
-    __syncthreads();
-    const uint64_t tn = t + gridDim.x;
-    if (tn < n_tiles) {  // prefetch the next tile while the matrix cores work
-      const uint64_t nb = tile_base(tn, map);
-#pragma unroll
-      for (int c = 0; c < kChunks; ++c) {
-        v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (nb + off(c))));
-      }
-    }
+  };
+  uint64_t t = xcd_block(xcd);  // consecutive tiles stay on one XCD
+  if (t < n_tiles) {
+    load(t);
+    stage();
+  }
+  __syncthreads();
+  for (; t < n_tiles; t += G) {
+    // opaque to the compiler, so that the chunk offsets are formed here from the one
+    // offset and position instead of being held in registers across the loop
+    asm volatile("" : "+v"(off0), "+v"(pos0));
+    const uint64_t base = tile_base(t, map);
+    const uint64_t tn = t + G;
+    if (tn < n_tiles) load(tn);  // in flight while the matrix cores work
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
                     0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int col = 32 * nt + (lane & 31);
@@ -192,6 +199,11 @@ __global__ __launch_bounds__(kBlock) void k_fused_f32(float* __restrict__ amp,
         y[2 * e + 1] = xb[q + D * RS];
       }
       vstore<NTA>(reinterpret_cast<floatx4*>(amp + 2 * (base + off(c))), y);
+    }
+    if (tn < n_tiles) {
+      __syncthreads();
+      stage();
+      __syncthreads();
     }
   }
 }
@@ -248,6 +260,10 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
       al[ks][j] = alo[idx + j];
     }
   }
+  // Complete these loads here: left pending, their waits land inside the tile loop, where
+  // (memory operations completing in order) they also wait for the next tile's prefetch.
+#pragma unroll
+  for (int ks = 0; ks < S::KSTEPS; ++ks) asm volatile("" : "+v"(ah[ks]), "+v"(al[ks]));
 
   // The lane's chunk-0 offset and LDS positions (column-major, re at +0, im at +D) in
   // bf16 units for the B operands and in floats for the result.
@@ -258,21 +274,17 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
   auto hpos = [&](int c, int e) { return hpos0 + tile_pos(chunk_tl(c, e), map, 1, HS); };
   auto ypos = [&](int c, int e) { return ypos0 + tile_pos(chunk_tl(c, e), map, 1, CS); };
 
+  // Per tile as in the FP32 kernel: next tile's loads, matrix-core work on this tile, and
+  // the next tile staged (split into hi and lo as it goes to LDS) before this one's stores.
   floatx4 v[kChunks];
-  uint64_t t = xcd_block(xcd);  // consecutive tiles stay on one XCD
-  if (t < n_tiles) {
-    const uint64_t base = tile_base(t, map);
+  const uint64_t G = gridDim.x;
+  auto load = [&](uint64_t tt) {
+    const uint64_t b = tile_base(tt, map);
 #pragma unroll
-    for (int c = 0; c < kChunks; ++c) {
-      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (base + off(c))));
-    }
-  }
-  for (; t < n_tiles; t += gridDim.x) {
-    // opaque to the compiler, so that the chunk offsets are formed here from the one
-    // offset and position instead of being held in registers across the loop
-    asm volatile("" : "+v"(off0), "+v"(hpos0), "+v"(ypos0));
-    const uint64_t base = tile_base(t, map);
-    __syncthreads();
+    for (int c = 0; c < kChunks; ++c)
+      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (b + off(c))));
+  };
+  auto stage = [&]() {
 #pragma unroll
     for (int c = 0; c < kChunks; ++c)
 #pragma unroll
@@ -286,15 +298,20 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
         bhs[h + D] = hi_;
         bls[h + D] = li;
       }
-    __syncthreads();
-    const uint64_t tn = t + gridDim.x;
-    if (tn < n_tiles) {
-      const uint64_t nb = tile_base(tn, map);
-#pragma unroll
-      for (int c = 0; c < kChunks; ++c) {
-        v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(amp + 2 * (nb + off(c))));
-      }
-    }
+  };
+  uint64_t t = xcd_block(xcd);  // consecutive tiles stay on one XCD
+  if (t < n_tiles) {
+    load(t);
+    stage();
+  }
+  __syncthreads();
+  for (; t < n_tiles; t += G) {
+    // opaque to the compiler, so that the chunk offsets are formed here from the one
+    // offset and position instead of being held in registers across the loop
+    asm volatile("" : "+v"(off0), "+v"(hpos0), "+v"(ypos0));
+    const uint64_t base = tile_base(t, map);
+    const uint64_t tn = t + G;
+    if (tn < n_tiles) load(tn);  // in flight while the matrix cores work
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
                     0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int col = 32 * nt + (lane & 31);
@@ -314,17 +331,25 @@ __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp
       yb[col * CS + row] = acc[r];
     }
     __syncthreads();
+    // this tile's result into registers, the next tile staged (its loads complete), and
+    // only then this tile's stores: measured a little faster than storing first
+    floatx4 y[kChunks];
 #pragma unroll
-    for (int c = 0; c < kChunks; ++c) {
-      floatx4 y;
+    for (int c = 0; c < kChunks; ++c)
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
         const uint32_t q = ypos(c, e);
-        y[2 * e] = yb[q];
-        y[2 * e + 1] = yb[q + D];
+        y[c][2 * e] = yb[q];
+        y[c][2 * e + 1] = yb[q + D];
       }
-      vstore<NTA>(reinterpret_cast<floatx4*>(amp + 2 * (base + off(c))), y);
+    if (tn < n_tiles) {
+      __syncthreads();
+      stage();
     }
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c)
+      vstore<NTA>(reinterpret_cast<floatx4*>(amp + 2 * (base + off(c))), y[c]);
+    if (tn < n_tiles) __syncthreads();
   }
 }
 
@@ -334,7 +359,7 @@ hipError_t launch_k(const State& st, const TileMap& map, const void* mats, int p
   const uint64_t n_tiles = st.dim() >> kTileBits;
   const int cus = st.ctx && st.ctx->cus > 0 ? st.ctx->cus : 256;
   const uint64_t cap = t.mfma_blocks > 0 ? uint64_t(t.mfma_blocks)
-                                         : uint64_t(precision == 1 ? 8 : 2) * cus;
+                                         : uint64_t(precision == 1 && K == 4 ? 1 : 2) * cus;
   const uint64_t g = n_tiles < cap ? n_tiles : cap;
   float* amp = static_cast<float*>(st.amp);
   if (precision == 1) {
