@@ -44,7 +44,7 @@ class QsvhError(RuntimeError):
 
 class _ProgramOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("use_graph", C.c_int), ("tile_qubits", C.c_int),
-                ("subspace", C.c_int), ("reserved", C.c_int * 4)]
+                ("subspace", C.c_int), ("mfma_tiles", C.c_int), ("reserved", C.c_int * 3)]
 
 
 _lib = None
@@ -348,11 +348,11 @@ class Program:
     """A circuit bound to a state (matrices resident on the device, optional graph)."""
 
     def __init__(self, state: State, circuit: Circuit, precision=PREC_EXACT,
-                 use_graph=False, tile_qubits=0, subspace=False):
+                 use_graph=False, tile_qubits=0, subspace=False, mfma_tiles=0):
         self.state = state
         self.circuit = circuit
         opts = _ProgramOpts(precision, int(bool(use_graph)), int(tile_qubits),
-                            int(bool(subspace)))
+                            int(bool(subspace)), int(mfma_tiles))
         h = C.c_void_p()
         _check(lib().qsvh_program_create(state.handle, circuit.handle, C.byref(opts),
                                          C.byref(h)), state.backend.ctx)

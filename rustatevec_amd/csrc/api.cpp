@@ -94,6 +94,8 @@ hipError_t issue(const Program& p, const Launch& l, const Tuning& t, hipStream_t
     case Launch::kTile:
       return launch_tile(st, p.tile_passes[l.arena_off], p.tile_ops, p.tile_diags, p.tile_pool,
                          t.tile_blocks, s);
+    case Launch::kMfmaTile:
+      return launch_mfma_tiles(st, p.mt_passes[l.arena_off], p.mt_ops, p.mt_pool, t, s);
     case Launch::kCtrl1q:
       return launch_ctrl1q(st, l.m, l.q, l.fix, l.mat.data(), t, s);
     case Launch::kDiagSub:
@@ -168,9 +170,14 @@ int program_build(State* st, const Circuit& c, const qsvh_program_opts* opts, Pr
     return set_err(st->ctx, kDtype, "MFMA precisions need a complex64 state");
   p->launches.reserve(c.ops.size());
   for (const Op& op : c.ops) p->launches.push_back(classify(op, st->dtype));
+  const int mt = opts ? opts->mfma_tiles : 0;
+  if (mt > 0 && opts->tile_qubits > 0)
+    return set_err(st->ctx, kInvalidArg, "tile_qubits and mfma_tiles exclude each other");
+  if (mt > 0 && opts->subspace)
+    return set_err(st->ctx, kInvalidArg, "subspace launches do not combine with mfma_tiles");
   int rc = plan_tiles(p, opts ? opts->tile_qubits : 0);
   if (rc != kOk) return rc;
-  rc = plan_mfma(p);
+  rc = mt > 0 ? plan_mfma_tiles(p, mt) : plan_mfma(p);
   if (rc != kOk) return rc;
   if (opts && opts->subspace) plan_subspace(p);
   return build_arena(p);

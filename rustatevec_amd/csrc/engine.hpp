@@ -81,7 +81,7 @@ struct MqIndex {
 // subspace); kDiagSub: the phase mat[0..1] on the amplitudes whose bits q[0..m) equal
 // `fix`'s (the one non-unit entry of a diagonal gate).
 struct Launch {
-  enum Kind { k1q, kDiag, kMq, kFusedMfma, kTile, kCtrl1q, kDiagSub } kind;
+  enum Kind { k1q, kDiag, kMq, kFusedMfma, kTile, kCtrl1q, kDiagSub, kMfmaTile } kind;
   uint32_t m = 0;
   uint32_t q[kMaxGateQubits] = {0, 0, 0, 0, 0, 0};
   // matrix in f64 (interleaved re, im), row-major; for diagonal launches the 2^m phases.
@@ -129,6 +129,28 @@ hipError_t launch_join(const State& st, uint64_t first, uint64_t count, const vo
 // (precision 2).
 hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, const void* mats,
                              int precision, const Tuning& t, hipStream_t s);
+
+// MFMA tile passes: a run of fused groups applied to a tile of 2^tb amplitudes (tb = 12 or
+// 13, complex64) held in LDS as f32, one HBM round trip for the run.  Dense groups
+// (k = 4..6 qubits, all in the tile) are split-BF16x3 MFMA products; diagonal groups (up to
+// 6 qubits anywhere: the qubits outside the tile are fixed per tile) scale each amplitude.
+struct MtOp {
+  uint32_t kind;     // 0 dense group, 1 diagonal
+  uint32_t k;        // dense: group qubits (4..6); diagonal: m (1..6)
+  uint32_t in_mask;  // diagonal: bit j set when qubit j is in the tile
+  uint8_t bit[6];    // dense: tile-local bits ascending; diagonal: tile-local bit of qubit j
+                     // (in the tile) or its state bit (outside)
+  uint8_t obit[10];  // dense: the tb - k other tile-local bits, ascending
+  uint64_t mat;      // byte offset in the pass pool: dense: bf16 block matrix hi, then lo
+                     // (2D x 2D each); diagonal: 2^m complex f32 phases (re, im)
+};
+struct MtPassArg {
+  uint32_t tb;
+  uint32_t ts[13];   // state bits of the tile, ascending (tile-local bit j = state bit ts[j])
+  uint32_t op_begin, n_ops;
+};
+hipError_t launch_mfma_tiles(const State& st, const MtPassArg& pa, const MtOp* ops,
+                             const void* pool, const Tuning& t, hipStream_t s);
 
 // ---- kernels_tile.hip ------------------------------------------------------------------
 // One op of a tile pass, applied to the tile held in LDS; it runs on a tile only when

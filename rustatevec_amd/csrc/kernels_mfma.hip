@@ -430,4 +430,277 @@ hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, con
   }
 }
 
+// ======================================================================= MFMA tile passes
+namespace {
+
+// LDS word of tile-local index e: the 5-bit groups of e above bit 4 are XOR-folded into its
+// low 5 bits.  The map is linear over GF(2), so the word of a | b (disjoint bits) is the
+// XOR of the words of a and b, and 32 lanes stepping through 5 tile bits whose positions
+// differ mod 5 hit 32 distinct banks, wherever in the tile those bits are.
+__device__ __forceinline__ uint32_t swz(uint32_t e) {
+  return e ^ (((e >> 5) ^ (e >> 10)) & 31u);
+}
+
+// State offset of tile-local index tl (tile-local bit j = state bit ts[j]).
+__device__ __forceinline__ uint64_t mt_offset(uint32_t tl, const MtPassArg& pa) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int j = 0; j < 13; ++j)
+    if (j < int(pa.tb)) o |= uint64_t((tl >> j) & 1u) << pa.ts[j];
+  return o;
+}
+
+__device__ __forceinline__ uint64_t mt_base(uint64_t t, const MtPassArg& pa) {
+#pragma unroll
+  for (int j = 0; j < 13; ++j)
+    if (j < int(pa.tb)) {
+      const uint32_t b = pa.ts[j];
+      t = ((t >> b) << (b + 1)) | (t & ((1ull << b) - 1));
+    }
+  return t;
+}
+
+// XOR of the words U[i] of the set bits i of v (v < 2^K).
+template <int K>
+__device__ __forceinline__ uint32_t uxor(const uint32_t (&U)[K], uint32_t v) {
+  uint32_t x = 0;
+#pragma unroll
+  for (int i = 0; i < K; ++i)
+    if ((v >> i) & 1u) x ^= U[i];
+  return x;
+}
+
+// One dense k-qubit group on the tile in LDS: Y = U X as the real block product, split-BF16
+// x3 on v_mfma_f32_32x32x16_bf16 (as k_fused_bf16x3).  Row s, column n of X is tile-local
+// index dep(s over the group bits) | dep(n over the other bits); the 2^(TB-9) output blocks
+// of 32 x 32 are dealt 2^(TB-11) per wave, all of one row band, so the band's A fragments
+// serve them all.  Every wave reads before any writes: the product is in place.
+template <int K, int TB>
+__device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
+                                         const unsigned char* pool) {
+  constexpr int D = 1 << K, R = 2 * D, KSTEPS = R / 16;
+  constexpr int NBC = 1 << (TB - K - 5);  // 32-column blocks per row band
+  constexpr int BPW = 1 << (TB - 11);     // output blocks per wave
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, l = lane & 31;
+  const int b0 = wave * BPW, mt = b0 / NBC, nt0 = b0 % NBC;
+
+  const __bf16* ahi = reinterpret_cast<const __bf16*>(pool + op.mat);
+  const __bf16* alo = ahi + R * R;
+  bf16x8 ah[KSTEPS], al[KSTEPS];
+#pragma unroll
+  for (int ks = 0; ks < KSTEPS; ++ks) {
+    const int idx = (32 * mt + l) * R + 16 * ks + 8 * h;
+    ah[ks] = *reinterpret_cast<const bf16x8*>(ahi + idx);
+    al[ks] = *reinterpret_cast<const bf16x8*>(alo + idx);
+  }
+  uint32_t U[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) U[i] = swz(1u << op.bit[i]);
+  uint32_t wl = 0;  // the lane's column l over the other bits
+#pragma unroll
+  for (int i = 0; i < 5; ++i) wl |= uint32_t((l >> i) & 1) << op.obit[i];
+  wl = swz(wl);
+  auto colw = [&](int nt) {  // block nt's first column over the other bits (uniform)
+    uint32_t u = 0;
+#pragma unroll
+    for (int i = 5; i < TB - K; ++i) u |= uint32_t((nt >> (i - 5)) & 1) << op.obit[i];
+    return swz(u);
+  };
+
+  floatx16 acc[BPW];
+#pragma unroll
+  for (int bi = 0; bi < BPW; ++bi) {
+    acc[bi] = floatx16{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                       0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    uint32_t wb = wl ^ colw(nt0 + bi) ^ (h ? U[3] : 0u);  // B rows 16 ks + 8 h + j
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ++ks) {
+      // opaque per step: the step's read addresses are formed here, not all held at once
+      asm volatile("" : "+v"(wb));
+      const float* src = ((16 * ks) >> K) ? xi : xr;  // block rows >= D: the imaginary part
+      bf16x8 bh, bl;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float x = src[wb ^ uxor<K>(U, uint32_t((16 * ks + j) & (D - 1)))];
+        __bf16 hb, lb;
+        split_bf16(x, hb, lb);
+        bh[j] = hb;
+        bl[j] = lb;
+      }
+      acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bh, acc[bi], 0, 0, 0);
+      acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bl, acc[bi], 0, 0, 0);
+      acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[ks], bh, acc[bi], 0, 0, 0);
+      // the next step's LDS reads are issued behind these products: hoisted, all of a
+      // block's reads would be live at once
+      asm volatile("" ::: "memory");
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int bi = 0; bi < BPW; ++bi) {
+    uint32_t wb = wl ^ colw(nt0 + bi) ^ (h ? U[2] : 0u);  // output rows ... + 4 h + ...
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      if ((r & 3) == 0) asm volatile("" : "+v"(wb));
+      const uint32_t rowu = uint32_t(32 * mt + 8 * (r >> 2) + (r & 3));
+      float* dst = (rowu >> K) ? xi : xr;
+      dst[wb ^ uxor<K>(U, rowu & (D - 1))] = acc[bi][r];
+    }
+  }
+  __syncthreads();
+}
+
+// A run of diagonal groups [i0, i1): the lane's amplitudes e = tid + 256 j stay in
+// registers across the run.  A group's phase index is its qubits' bits of the state index:
+// those outside the tile come from the tile's base (the same for the whole tile), those in
+// it from e, split into the lane's part and a per-j part; the phase table (2^m <= 64
+// entries) sits one entry per lane and is read across lanes.
+template <int TB>
+__device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const MtOp* ops, uint32_t i0,
+                                            uint32_t i1, uint64_t base,
+                                            const unsigned char* pool) {
+  constexpr int E = (1 << TB) / kBlock;
+  const int lane = threadIdx.x & 63;
+  float vr[E], vi[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const uint32_t w = swz(threadIdx.x + kBlock * j);
+    vr[j] = xr[w];
+    vi[j] = xi[w];
+  }
+  for (uint32_t i = i0; i < i1; ++i) {
+    const MtOp& op = ops[i];
+    const float2 tab = lane < (1 << op.k) ? reinterpret_cast<const float2*>(pool + op.mat)[lane]
+                                          : float2{0.f, 0.f};
+    uint32_t fix = 0, lp = 0;  // out-of-tile bits; the lane's bits (tile-local bits < 8)
+    uint32_t g8[TB - 8];       // index bits of tile-local bit 8 + b
+#pragma unroll
+    for (int b = 0; b < TB - 8; ++b) g8[b] = 0;
+    for (uint32_t j = 0; j < op.k; ++j) {
+      const uint32_t b = op.bit[j];
+      if (!((op.in_mask >> j) & 1u)) {
+        fix |= uint32_t((base >> b) & 1u) << j;
+      } else if (b < 8) {
+        lp |= ((threadIdx.x >> b) & 1u) << j;
+      } else {
+#pragma unroll
+        for (int c = 0; c < TB - 8; ++c)
+          if (int(b) == 8 + c) g8[c] = 1u << j;
+      }
+    }
+    lp |= fix;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      if ((j & 3) == 0) asm volatile("" : "+v"(lp));  // indices formed 4 at a time
+      uint32_t idx = lp;
+#pragma unroll
+      for (int c = 0; c < TB - 8; ++c)
+        if ((j >> c) & 1) idx ^= g8[c];
+      const float pr = __shfl(tab.x, int(idx));
+      const float pi = __shfl(tab.y, int(idx));
+      const float a = vr[j], b = vi[j];
+      vr[j] = a * pr - b * pi;
+      vi[j] = a * pi + b * pr;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const uint32_t w = swz(threadIdx.x + kBlock * j);
+    xr[w] = vr[j];
+    xi[w] = vi[j];
+  }
+}
+
+template <int TB, bool NTA>
+__global__ __launch_bounds__(kBlock) void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
+                                                       MtPassArg pa,
+                                                       const MtOp* __restrict__ ops,
+                                                       const unsigned char* __restrict__ pool,
+                                                       int xcd) {
+  constexpr int N = 1 << TB;
+  constexpr int CH = N / 2 / kBlock;  // 16-byte chunks (2 amplitudes) per lane
+  __shared__ float xr[N], xi[N];
+  // chunk c of a lane holds tile-local indices 2 (tid + 256 c) + {0, 1}: against chunk 0 it
+  // sets tile-local bits 9.., the same for every lane (tile-local bit 0 is state bit 0)
+  uint64_t off0 = mt_offset(2u * threadIdx.x, pa);
+  const MtOp* pops = ops + pa.op_begin;
+  for (uint64_t t = xcd_block(xcd); t < n_tiles; t += gridDim.x) {
+    asm volatile("" : "+v"(off0));
+    const uint64_t base = mt_base(t, pa);
+    floatx4 v[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(
+          amp + 2 * (base + (off0 | mt_offset(uint32_t(2 * kBlock * c), pa)))));
+    __syncthreads();  // the previous tile's stores have read LDS
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const uint32_t w = swz(2u * threadIdx.x + uint32_t(2 * kBlock * c + e));
+        xr[w] = v[c][2 * e];
+        xi[w] = v[c][2 * e + 1];
+      }
+    for (uint32_t i = 0; i < pa.n_ops;) {
+      const MtOp& op = pops[i];
+      if (op.kind == 0) {
+        __syncthreads();
+        if (op.k == 4) mt_dense<4, TB>(xr, xi, op, pool);
+        else if (op.k == 5) mt_dense<5, TB>(xr, xi, op, pool);
+        else mt_dense<6, TB>(xr, xi, op, pool);
+        ++i;
+      } else {
+        uint32_t j = i + 1;
+        while (j < pa.n_ops && pops[j].kind == 1) ++j;
+        __syncthreads();  // the diagonal run's amplitudes were staged by other lanes
+        mt_diag_run<TB>(xr, xi, pops, i, j, base, pool);
+        i = j;
+      }
+    }
+    __syncthreads();
+    asm volatile("" : "+v"(off0));  // chunk offsets re-formed here, not held across the ops
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      floatx4 y;
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const uint32_t w = swz(2u * threadIdx.x + uint32_t(2 * kBlock * c + e));
+        y[2 * e] = xr[w];
+        y[2 * e + 1] = xi[w];
+      }
+      vstore<NTA>(reinterpret_cast<floatx4*>(
+                      amp + 2 * (base + (off0 | mt_offset(uint32_t(2 * kBlock * c), pa)))),
+                  y);
+    }
+  }
+}
+
+template <int TB, bool NT>
+hipError_t launch_mt(const State& st, const MtPassArg& pa, const MtOp* ops, const void* pool,
+                     const Tuning& t, hipStream_t s) {
+  const uint64_t n_tiles = st.dim() >> TB;
+  const int cus = st.ctx && st.ctx->cus > 0 ? st.ctx->cus : 256;
+  const uint64_t cap = t.mfma_blocks > 0 ? uint64_t(t.mfma_blocks) : uint64_t(2) * cus;
+  const uint64_t g = n_tiles < cap ? n_tiles : cap;
+  k_mfma_tiles<TB, NT><<<unsigned(g), kBlock, 0, s>>>(
+      static_cast<float*>(st.amp), n_tiles, pa, ops,
+      static_cast<const unsigned char*>(pool), t.xcd);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_mfma_tiles(const State& st, const MtPassArg& pa, const MtOp* ops,
+                             const void* pool, const Tuning& t, hipStream_t s) {
+  if (st.dtype != 1 || (pa.tb != 12 && pa.tb != 13) || st.n < pa.tb || pa.ts[0] != 0)
+    return hipErrorInvalidValue;
+  const bool nt = nontemporal_for(st, t, true);
+  if (pa.tb == 12)
+    return nt ? launch_mt<12, true>(st, pa, ops, pool, t, s)
+              : launch_mt<12, false>(st, pa, ops, pool, t, s);
+  return nt ? launch_mt<13, true>(st, pa, ops, pool, t, s)
+            : launch_mt<13, false>(st, pa, ops, pool, t, s);
+}
+
 }  // namespace qsvh

@@ -709,10 +709,162 @@ int plan_mfma(Program* p) {
   return hip_err(st.ctx, e, "mfma matrices upload");
 }
 
+// ----------------------------------------------------------------------- MFMA tile passes
+namespace {
+
+// A dense gate on m < 4 qubits widened to 4 (identity on the added qubits `extra`): matrix
+// index bit j <-> q[j], the gate's own bits lowest.
+Launch widen_to_4(const Launch& l, const uint32_t* extra) {
+  Launch w;
+  w.kind = Launch::kMq;
+  w.m = 4;
+  std::copy(l.q, l.q + l.m, w.q);
+  std::copy(extra, extra + (4 - l.m), w.q + l.m);
+  const uint32_t d = 1u << l.m, D = 16;
+  w.mat.assign(2 * D * D, 0.0);
+  for (uint32_t r = 0; r < D; ++r)
+    for (uint32_t c = 0; c < D; ++c)
+      if ((r >> l.m) == (c >> l.m)) {
+        const size_t src = 2 * (size_t(r & (d - 1)) * d + (c & (d - 1)));
+        w.mat[2 * (size_t(r) * D + c)] = l.mat[src];
+        w.mat[2 * (size_t(r) * D + c) + 1] = l.mat[src + 1];
+      }
+  return w;
+}
+
+}  // namespace
+
+// Consecutive ops go into one pass while the qubits of its dense ops (with bits 0..3) fit
+// in tb bits; diagonal ops join any pass (their qubits outside the tile are fixed per tile).
+// Op order is kept, so every pass applies its ops exactly in circuit order.
+int plan_mfma_tiles(Program* p, int tb) {
+  State& st = *p->st;
+  if (p->precision != QSVH_PREC_MFMA_BF16X3)
+    return set_err(st.ctx, kInvalidArg, "mfma_tiles runs split-BF16 groups (MFMA_BF16X3)");
+  if (tb != 12 && tb != 13) return set_err(st.ctx, kInvalidArg, "mfma_tiles must be 12 or 13");
+  if (st.n < uint32_t(tb)) return plan_mfma(p);  // smaller than one tile
+  const std::vector<Launch> in = std::move(p->launches);
+  p->launches.clear();
+  std::vector<MtOp> ops;
+  std::vector<uint8_t> pool;
+  std::vector<size_t> cur;
+  const uint64_t low = (1ull << kTileLowBits) - 1;
+  uint64_t S = low;
+  auto put = [&](const void* src, size_t bytes) {
+    pool.resize((pool.size() + 255) & ~size_t(255));
+    const uint64_t at = pool.size();
+    const uint8_t* b = static_cast<const uint8_t*>(src);
+    pool.insert(pool.end(), b, b + bytes);
+    return at;
+  };
+  auto close = [&]() {
+    if (cur.empty()) return;
+    uint64_t tm = S;
+    for (uint32_t b = 0; b < st.n && popcount64(tm) < uint32_t(tb); ++b) tm |= 1ull << b;
+    MtPassArg pa{};
+    pa.tb = uint32_t(tb);
+    uint32_t pos[64] = {0};
+    uint32_t nb = 0;
+    for (uint32_t b = 0; b < st.n; ++b)
+      if ((tm >> b) & 1) {
+        pa.ts[nb] = b;
+        pos[b] = nb++;
+      }
+    pa.op_begin = uint32_t(ops.size());
+    for (size_t li : cur) {
+      const Launch& l = in[li];
+      MtOp op{};
+      if (l.kind == Launch::kDiag) {
+        op.kind = 1;
+        op.k = l.m;
+        for (uint32_t j = 0; j < l.m; ++j) {
+          const uint32_t q = l.q[j];
+          if ((tm >> q) & 1) {
+            op.in_mask |= 1u << j;
+            op.bit[j] = uint8_t(pos[q]);
+          } else {
+            op.bit[j] = uint8_t(q);
+          }
+        }
+        std::vector<float> ph(size_t(2) << l.m);
+        for (size_t e = 0; e < ph.size(); ++e) ph[e] = float(l.mat[e]);
+        op.mat = put(ph.data(), ph.size() * sizeof(float));
+      } else {
+        Launch g = l;
+        if (l.m < 4) {
+          uint32_t extra[4], ne = 0;
+          for (uint32_t b = 0; b < st.n && ne < 4 - l.m; ++b)
+            if (((tm >> b) & 1) && std::find(l.q, l.q + l.m, b) == l.q + l.m) extra[ne++] = b;
+          g = widen_to_4(l, extra);
+        }
+        uint32_t sorted[kMaxGateQubits];
+        const std::vector<float> a = block_matrix(g, sorted);
+        op.kind = 0;
+        op.k = g.m;
+        uint32_t gm = 0;
+        for (uint32_t j = 0; j < g.m; ++j) {
+          op.bit[j] = uint8_t(pos[sorted[j]]);
+          gm |= 1u << pos[sorted[j]];
+        }
+        uint32_t o = 0;
+        for (uint32_t b = 0; b < uint32_t(tb); ++b)
+          if (!((gm >> b) & 1)) op.obit[o++] = uint8_t(b);
+        std::vector<uint16_t> hl(2 * a.size());
+        for (size_t e = 0; e < a.size(); ++e) {
+          hl[e] = bf16_bits(a[e]);
+          hl[a.size() + e] = bf16_bits(a[e] - bf16_value(hl[e]));
+        }
+        op.mat = put(hl.data(), hl.size() * sizeof(uint16_t));
+      }
+      ops.push_back(op);
+    }
+    pa.n_ops = uint32_t(ops.size()) - pa.op_begin;
+    Launch L;
+    L.kind = Launch::kMfmaTile;
+    L.arena_off = p->mt_passes.size();
+    p->mt_passes.push_back(pa);
+    p->launches.push_back(std::move(L));
+    cur.clear();
+    S = low;
+  };
+  for (size_t i = 0; i < in.size(); ++i) {
+    const Launch& l = in[i];
+    if (l.kind == Launch::kDiag && l.m <= 6) {
+      cur.push_back(i);
+    } else if ((l.kind == Launch::k1q || l.kind == Launch::kMq) && l.m <= 6) {
+      uint64_t Q = 0;
+      for (uint32_t j = 0; j < l.m; ++j) Q |= 1ull << l.q[j];
+      if (popcount64(S | Q) > uint32_t(tb)) close();
+      S |= Q;
+      cur.push_back(i);
+    } else {
+      close();
+      p->launches.push_back(l);
+    }
+  }
+  close();
+  if (ops.empty()) return kOk;
+  auto upload = [&](const void* src, size_t bytes, void** dst) {
+    hipError_t e = hipMalloc(dst, bytes ? bytes : 16);
+    if (e != hipSuccess) return hip_err(st.ctx, e, "mfma tile buffers alloc");
+    p->owned.push_back(*dst);
+    if (bytes) e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    return hip_err(st.ctx, e, "mfma tile buffers upload");
+  };
+  void* d = nullptr;
+  int rc = upload(ops.data(), ops.size() * sizeof(MtOp), &d);
+  if (rc != kOk) return rc;
+  p->mt_ops = static_cast<MtOp*>(d);
+  rc = upload(pool.data(), pool.size(), &p->mt_pool);
+  return rc;
+}
+
 void release_extra(Program* p) {
   for (void* b : p->owned) hipFree(b);
   p->owned.clear();
   p->mfma = nullptr;
+  p->mt_ops = nullptr;
+  p->mt_pool = nullptr;
   p->tile_ops = nullptr;
   p->tile_diags = nullptr;
   p->tile_pool = nullptr;

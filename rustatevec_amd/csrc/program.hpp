@@ -26,6 +26,10 @@ struct Program {
   TileOp* tile_ops = nullptr;
   TileDiag* tile_diags = nullptr;
   void* tile_pool = nullptr;
+  // MFMA tile passes: Launch::kMfmaTile launches index `mt_passes` through arena_off.
+  std::vector<MtPassArg> mt_passes;
+  MtOp* mt_ops = nullptr;
+  void* mt_pool = nullptr;
   // Original op index of each applied gate, in execution order (tile passes reorder
   // gates that act on disjoint qubits).
   std::vector<uint64_t> order;
@@ -72,6 +76,8 @@ void schedule_tiles(const std::vector<Launch>& in, uint32_t n, uint32_t tb, Tile
 int plan_tiles(Program* p, int tile_qubits);
 void plan_subspace(Program* p);
 int plan_mfma(Program* p);
+// Runs of fused groups as MFMA tile passes of 2^tb amplitudes (split-BF16 precision).
+int plan_mfma_tiles(Program* p, int tb);
 void release_extra(Program* p);
 
 }  // namespace qsvh

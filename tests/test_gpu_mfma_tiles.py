@@ -1,0 +1,153 @@
+"""MFMA tile passes: runs of fused groups (dense groups as split-BF16x3 MFMA products,
+diagonal groups as f32 phases, dense gates of 1..3 qubits widened to 4) applied to one
+2^12 / 2^13-amplitude LDS tile per HBM pass, against the complex128 oracle.
+
+Tolerance: each op adds at most the split-BF16x3 product error (the lo*lo term dropped,
+~2^-16 relative per product) or one f32 rounding; the per-group form is held to 1e-4
+relative for one group, so a circuit of N ops is held to 1e-4 * sqrt(N) relative to max|psi|
+and 1 - fidelity < 1e-8 * N (one group's 1e-4 squared, per op); and to within 3x the
+per-group form's error on the same ops."""
+import numpy as np
+import pytest
+
+import oracle
+import rustatevec_amd as rs
+from tests.helpers import max_abs_diff
+
+pytestmark = pytest.mark.gpu
+
+
+def random_unitary(d, rng):
+    z = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+    q, r = np.linalg.qr(z)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def random_ops(n, count, rng):
+    ops = []
+    for _ in range(count):
+        kind = rng.choice(["dense", "dense", "diag", "small"])
+        if kind == "dense":
+            k = int(rng.integers(4, 7))
+        elif kind == "diag":
+            k = int(rng.integers(1, 7))
+        else:
+            k = int(rng.integers(1, 4))
+        qs = [int(x) for x in rng.choice(n, size=k, replace=False)]
+        if kind == "diag":
+            U = np.diag(np.exp(1j * rng.uniform(0, 2 * np.pi, size=1 << k)))
+        else:
+            U = random_unitary(1 << k, rng)
+        ops.append((U, qs))
+    return ops
+
+
+def run(backend, n, ops, psi, **kw):
+    c = rs.Circuit(n)
+    for U, qs in ops:
+        c.push(U, qs)
+    st = backend.alloc(n, rs.C64)
+    backend.upload(st, psi)
+    p = rs.Program(st, c, precision=rs.PREC_MFMA_BF16X3, **kw)
+    p.launch()
+    out = backend.download(st)
+    nk = p.num_kernels()
+    del p
+    st.free()
+    return out, nk
+
+
+def exact(n, ops, psi):
+    want = psi.astype(np.complex128)
+    for U, qs in ops:
+        want = oracle.apply(oracle.CPU_SERIAL, n, want, U, qs)
+    return want
+
+
+def errors(got, want):
+    err = max_abs_diff(got, want) / np.abs(want).max()
+    fid = abs(np.vdot(want, got)) ** 2 / (np.vdot(want, want).real * np.vdot(got, got).real)
+    return err, 1 - fid
+
+
+def check(got, want, n_ops):
+    err, infid = errors(got, want)
+    assert err < 1e-4 * np.sqrt(n_ops), err
+    assert infid < 1e-8 * n_ops, infid
+    return err
+
+
+@pytest.mark.parametrize("tb", [12, 13])
+def test_random_group_mixes_match_the_oracle(backend, tb):
+    rng = np.random.default_rng(100 + tb)
+    for n in (tb, 15, 17):
+        psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+        psi /= np.linalg.norm(psi)
+        ops = random_ops(n, 24, rng)
+        want = exact(n, ops, psi)
+        got, nk = run(backend, n, ops, psi, mfma_tiles=tb)
+        err = check(got, want, len(ops))
+        assert nk < len(ops), (n, nk)
+        per_group, _ = run(backend, n, ops, psi)
+        assert err < 3 * errors(per_group, want)[0], (err, errors(per_group, want))
+
+
+def test_diagonal_groups_outside_the_tile_and_passes_of_only_diagonals(backend):
+    # diagonal groups on qubits far outside a pass's tile take their out-of-tile bits from
+    # each tile's base; a run of only diagonal groups is one pass
+    n = 18
+    rng = np.random.default_rng(7)
+    psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+    psi /= np.linalg.norm(psi)
+    ops = []
+    for qs in ([17, 16, 3], [0, 17], [12, 13, 14, 15, 16, 17], [5], [17, 1, 9, 14, 2, 11]):
+        ops.append((np.diag(np.exp(1j * rng.uniform(0, 2 * np.pi, size=1 << len(qs)))), qs))
+    got, nk = run(backend, n, ops, psi, mfma_tiles=12)
+    assert nk == 1
+    check(got, exact(n, ops, psi), len(ops))
+    ops2 = [(random_unitary(64, rng), [12, 13, 14, 15, 16, 17])] + ops + \
+           [(random_unitary(16, rng), [0, 5, 9, 17])]
+    got, nk = run(backend, n, ops2, psi, mfma_tiles=12)
+    check(got, exact(n, ops2, psi), len(ops2))
+
+
+def test_fused_qft_in_tile_passes(backend):
+    n = 16
+    qft = rs.qft(n)
+    fused = rs.fuse(backend, qft, 6)
+    rng = np.random.default_rng(3)
+    psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+    psi /= np.linalg.norm(psi)
+    st = backend.alloc(n, rs.C64)
+    outs = {}
+    for name, kw in (("groups", {}), ("tiles12", {"mfma_tiles": 12}), ("tiles13", {"mfma_tiles": 13})):
+        backend.upload(st, psi)
+        p = rs.Program(st, fused, precision=rs.PREC_MFMA_BF16X3, **kw)
+        p.launch()
+        outs[name] = (backend.download(st), p.num_kernels())
+        del p
+    st.free()
+    want = oracle.execute(oracle.CPU_SERIAL, n, qft.ops(), dtype=np.float64,
+                          start=psi.astype(np.complex128))
+    for name, (got, nk) in outs.items():
+        check(got, want, len(fused))
+    assert outs["tiles12"][1] < outs["groups"][1] // 3
+
+
+def test_smaller_than_a_tile_and_argument_errors(backend):
+    n = 11
+    rng = np.random.default_rng(5)
+    psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+    psi /= np.linalg.norm(psi)
+    ops = random_ops(n, 8, rng)
+    got, _ = run(backend, n, ops, psi, mfma_tiles=12)  # per-group launches
+    check(got, exact(n, ops, psi), len(ops))
+    st = backend.alloc(14, rs.C64)
+    c = rs.Circuit(14).h(0)
+    for kw in ({"mfma_tiles": 11}, {"mfma_tiles": 12, "tile_qubits": 12},
+               {"mfma_tiles": 12, "subspace": True}):
+        with pytest.raises(rs.QsvhError):
+            rs.Program(st, c, precision=rs.PREC_MFMA_BF16X3, **kw)
+    with pytest.raises(rs.QsvhError):
+        rs.Program(st, c, precision=rs.PREC_MFMA_F32, mfma_tiles=12)
+    st.free()
