@@ -105,6 +105,9 @@ def main():
     ap.add_argument("--mfma-only", action="store_true",
                     help="only 5 launches each of one dense 6-qubit group on a 30q c64 "
                          "state in FP32 and in split-BF16x3 MFMA, for counter runs")
+    ap.add_argument("--mt-only", action="store_true",
+                    help="only 3 launches of QFT(30) fused at k = 4 as split-BF16 MFMA tile "
+                         "passes (2^12-amplitude tiles), c64, for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -121,6 +124,17 @@ def main():
         ms_copy = b.time_copy(st, 4)
         print(json.dumps({"diag_cp_ms": ms_cp, "copy_ms": ms_copy,
                           "bytes_per_launch": (1 << 30) * 32}), flush=True)
+        return
+
+    if args.mt_only:
+        b = rs.HipBackend(0)
+        st = b.alloc(30, rs.C64)
+        b.init_random_normal(st, 1)
+        p = rs.Program(st, rs.fuse(b, rs.qft(30), 4), precision=rs.PREC_MFMA_BF16X3,
+                       mfma_tiles=12)
+        ms = [round(p.time(), 3) for _ in range(3)]
+        print(json.dumps({"qft30_fused_k4_mfma_tiles12_ms": ms, "passes": p.num_kernels()}),
+              flush=True)
         return
 
     if args.mfma_only:
@@ -234,10 +248,14 @@ def main():
     mfma_roof = None
     st64 = b.alloc(n, rs.C64)
     reset64 = lambda: b.init_random_normal(st64, seed)  # noqa: E731
+    # per-group launches in FP32 and split-BF16, and split-BF16 runs of groups in MFMA tile
+    # passes (one HBM round trip per 2^12-amplitude tile run)
+    forms = ((rs.PREC_MFMA_F32, "mfma_f32", {}), (rs.PREC_MFMA_BF16X3, "mfma_bf16x3", {}),
+             (rs.PREC_MFMA_BF16X3, "mfma_bf16x3_tiles12", {"mfma_tiles": 12}))
     for k in (4, 5, 6):
         fq = rs.fuse(b, qft, k)
-        for prec, name in ((rs.PREC_MFMA_F32, "mfma_f32"), (rs.PREC_MFMA_BF16X3, "mfma_bf16x3")):
-            p = rs.Program(st64, fq, precision=prec)
+        for prec, name, kw in forms:
+            p = rs.Program(st64, fq, precision=prec, **kw)
             f_s = timed_steps(b, st64, p, max(1, args.steps), 1, reset64)
             reset64()
             p.launch()

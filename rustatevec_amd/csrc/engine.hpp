@@ -134,18 +134,25 @@ hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, con
 // 13, complex64) held in LDS as f32, one HBM round trip for the run.  Dense groups
 // (k = 4..6 qubits, all in the tile) are split-BF16x3 MFMA products; diagonal groups (up to
 // 6 qubits anywhere: the qubits outside the tile are fixed per tile) scale each amplitude.
+// Bit positions are bytes packed in dword words (byte j of `bits` at bits[j / 4] >> 8 (j % 4)),
+// so that a kernel reads every field with scalar (uniform) loads.
 struct MtOp {
-  uint32_t kind;     // 0 dense group, 1 diagonal
-  uint32_t k;        // dense: group qubits (4..6); diagonal: m (1..6)
-  uint32_t in_mask;  // diagonal: bit j set when qubit j is in the tile
-  uint8_t bit[6];    // dense: tile-local bits ascending; diagonal: tile-local bit of qubit j
-                     // (in the tile) or its state bit (outside)
-  uint8_t obit[10];  // dense: the tb - k other tile-local bits, ascending
-  uint64_t mat;      // byte offset in the pass pool: dense: bf16 block matrix hi, then lo
-                     // (2D x 2D each); diagonal: 2^m complex f32 phases (re, im)
+  uint32_t kind;      // 0 dense group, 1 diagonal
+  uint32_t k;         // dense: group qubits (4..6); diagonal: m (1..6)
+  uint32_t in_mask;   // diagonal: bit j set when qubit j is in the tile
+  uint32_t run;       // diagonal: the consecutive diagonal ops from this one to the pass's
+                      // next dense op or end (this one included)
+  uint32_t bits[2];   // byte j: dense: tile-local bit j of the group, ascending; diagonal:
+                      // tile-local bit of qubit j (in the tile) or its state bit (outside)
+  uint32_t obits[3];  // byte i: dense: the tb - k other tile-local bits, ascending
+  uint64_t mat;       // byte offset in the pass pool: dense: bf16 block matrix hi, then lo
+                      // (2D x 2D each); diagonal: 2^m complex f32 phases (re, im)
+  void set_bit(uint32_t j, uint32_t b) { bits[j >> 2] |= (b & 0xffu) << (8 * (j & 3)); }
+  void set_obit(uint32_t i, uint32_t b) { obits[i >> 2] |= (b & 0xffu) << (8 * (i & 3)); }
 };
 struct MtPassArg {
   uint32_t tb;
+  uint32_t kmax;     // widest dense group of the pass (4..6; 4 when it has none)
   uint32_t ts[13];   // state bits of the tile, ascending (tile-local bit j = state bit ts[j])
   uint32_t op_begin, n_ops;
 };

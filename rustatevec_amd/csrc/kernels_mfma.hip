@@ -21,6 +21,8 @@
 // (the lo*lo term, ~2^-16 relative, is dropped).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "amp.hpp"
 #include "engine.hpp"
 
@@ -460,6 +462,11 @@ __device__ __forceinline__ uint64_t mt_base(uint64_t t, const MtPassArg& pa) {
   return t;
 }
 
+// Byte j of a packed bit-position array (compile-time j: one scalar load and shift).
+__device__ __forceinline__ uint32_t op_byte(const uint32_t* w, int j) {
+  return (w[j >> 2] >> (8 * (j & 3))) & 0xffu;
+}
+
 // XOR of the words U[i] of the set bits i of v (v < 2^K).
 template <int K>
 __device__ __forceinline__ uint32_t uxor(const uint32_t (&U)[K], uint32_t v) {
@@ -496,15 +503,15 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
   }
   uint32_t U[K];
 #pragma unroll
-  for (int i = 0; i < K; ++i) U[i] = swz(1u << op.bit[i]);
+  for (int i = 0; i < K; ++i) U[i] = swz(1u << op_byte(op.bits, i));
   uint32_t wl = 0;  // the lane's column l over the other bits
 #pragma unroll
-  for (int i = 0; i < 5; ++i) wl |= uint32_t((l >> i) & 1) << op.obit[i];
+  for (int i = 0; i < 5; ++i) wl |= uint32_t((l >> i) & 1) << op_byte(op.obits, i);
   wl = swz(wl);
   auto colw = [&](int nt) {  // block nt's first column over the other bits (uniform)
     uint32_t u = 0;
 #pragma unroll
-    for (int i = 5; i < TB - K; ++i) u |= uint32_t((nt >> (i - 5)) & 1) << op.obit[i];
+    for (int i = 5; i < TB - K; ++i) u |= uint32_t((nt >> (i - 5)) & 1) << op_byte(op.obits, i);
     return swz(u);
   };
 
@@ -562,57 +569,69 @@ __device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const MtOp* op
                                             const unsigned char* pool) {
   constexpr int E = (1 << TB) / kBlock;
   const int lane = threadIdx.x & 63;
-  float vr[E], vi[E];
-#pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const uint32_t w = swz(threadIdx.x + kBlock * j);
-    vr[j] = xr[w];
-    vi[j] = xi[w];
-  }
-  for (uint32_t i = i0; i < i1; ++i) {
+  // opaque, so that the amplitudes' LDS words are formed here, not held across the tiles
+  uint32_t tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  auto table = [&](uint32_t i) {
     const MtOp& op = ops[i];
-    const float2 tab = lane < (1 << op.k) ? reinterpret_cast<const float2*>(pool + op.mat)[lane]
-                                          : float2{0.f, 0.f};
-    uint32_t fix = 0, lp = 0;  // out-of-tile bits; the lane's bits (tile-local bits < 8)
-    uint32_t g8[TB - 8];       // index bits of tile-local bit 8 + b
-#pragma unroll
-    for (int b = 0; b < TB - 8; ++b) g8[b] = 0;
-    for (uint32_t j = 0; j < op.k; ++j) {
-      const uint32_t b = op.bit[j];
-      if (!((op.in_mask >> j) & 1u)) {
-        fix |= uint32_t((base >> b) & 1u) << j;
-      } else if (b < 8) {
-        lp |= ((threadIdx.x >> b) & 1u) << j;
-      } else {
-#pragma unroll
-        for (int c = 0; c < TB - 8; ++c)
-          if (int(b) == 8 + c) g8[c] = 1u << j;
-      }
-    }
-    lp |= fix;
+    return lane < (1 << op.k) ? reinterpret_cast<const float2*>(pool + op.mat)[lane]
+                              : float2{0.f, 0.f};
+  };
+  {
+    float vr[E], vi[E];
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      if ((j & 3) == 0) asm volatile("" : "+v"(lp));  // indices formed 4 at a time
-      uint32_t idx = lp;
-#pragma unroll
-      for (int c = 0; c < TB - 8; ++c)
-        if ((j >> c) & 1) idx ^= g8[c];
-      const float pr = __shfl(tab.x, int(idx));
-      const float pi = __shfl(tab.y, int(idx));
-      const float a = vr[j], b = vi[j];
-      vr[j] = a * pr - b * pi;
-      vi[j] = a * pi + b * pr;
+      const uint32_t w = swz(tid + kBlock * j);
+      vr[j] = xr[w];
+      vi[j] = xi[w];
     }
-  }
+    float2 tab = table(i0);
+    for (uint32_t i = i0; i < i1; ++i) {
+      const MtOp& op = ops[i];
+      const float2 cur = tab;
+      if (i + 1 < i1) tab = table(i + 1);  // the next group's entry loads meanwhile
+      uint32_t fix = 0, lp = 0;  // out-of-tile bits; the lane's bits (tile-local bits < 8)
+      uint32_t g8[TB - 8];       // index bits of tile-local bit 8 + b
 #pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const uint32_t w = swz(threadIdx.x + kBlock * j);
-    xr[w] = vr[j];
-    xi[w] = vi[j];
+      for (int b = 0; b < TB - 8; ++b) g8[b] = 0;
+      const uint64_t bw = uint64_t(op.bits[0]) | (uint64_t(op.bits[1]) << 32);
+      for (uint32_t q = 0; q < op.k; ++q) {
+        const uint32_t b = uint32_t(bw >> (8 * q)) & 0xffu;
+        if (!((op.in_mask >> q) & 1u)) {
+          fix |= uint32_t((base >> b) & 1u) << q;
+        } else if (b < 8) {
+          lp |= ((tid >> b) & 1u) << q;
+        } else {
+#pragma unroll
+          for (int c = 0; c < TB - 8; ++c)
+            if (int(b) == 8 + c) g8[c] = 1u << q;
+        }
+      }
+      lp |= fix;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        if ((j & 3) == 0) asm volatile("" : "+v"(lp));  // indices formed 4 at a time
+        uint32_t idx = lp;
+#pragma unroll
+        for (int c = 0; c < TB - 8; ++c)
+          if ((j >> c) & 1) idx ^= g8[c];
+        const float pr = __shfl(cur.x, int(idx));
+        const float pi = __shfl(cur.y, int(idx));
+        const float a = vr[j], b = vi[j];
+        vr[j] = a * pr - b * pi;
+        vi[j] = a * pi + b * pr;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const uint32_t w = swz(tid + kBlock * j);
+      xr[w] = vr[j];
+      xi[w] = vi[j];
+    }
   }
 }
 
-template <int TB, bool NTA>
+template <int TB, int KMAX, bool NTA>
 __global__ __launch_bounds__(kBlock) void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
                                                        MtPassArg pa,
                                                        const MtOp* __restrict__ ops,
@@ -628,31 +647,34 @@ __global__ __launch_bounds__(kBlock) void k_mfma_tiles(float* __restrict__ amp, 
   for (uint64_t t = xcd_block(xcd); t < n_tiles; t += gridDim.x) {
     asm volatile("" : "+v"(off0));
     const uint64_t base = mt_base(t, pa);
-    floatx4 v[CH];
-#pragma unroll
-    for (int c = 0; c < CH; ++c)
-      v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(
-          amp + 2 * (base + (off0 | mt_offset(uint32_t(2 * kBlock * c), pa)))));
     __syncthreads();  // the previous tile's stores have read LDS
+    {
+      floatx4 v[CH];
 #pragma unroll
-    for (int c = 0; c < CH; ++c)
+      for (int c = 0; c < CH; ++c)
+        v[c] = vload<NTA>(reinterpret_cast<const floatx4*>(
+            amp + 2 * (base + (off0 | mt_offset(uint32_t(2 * kBlock * c), pa)))));
 #pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const uint32_t w = swz(2u * threadIdx.x + uint32_t(2 * kBlock * c + e));
-        xr[w] = v[c][2 * e];
-        xi[w] = v[c][2 * e + 1];
-      }
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const uint32_t w = swz(2u * threadIdx.x + uint32_t(2 * kBlock * c + e));
+          xr[w] = v[c][2 * e];
+          xi[w] = v[c][2 * e + 1];
+        }
+    }
     for (uint32_t i = 0; i < pa.n_ops;) {
       const MtOp& op = pops[i];
       if (op.kind == 0) {
         __syncthreads();
-        if (op.k == 4) mt_dense<4, TB>(xr, xi, op, pool);
-        else if (op.k == 5) mt_dense<5, TB>(xr, xi, op, pool);
-        else mt_dense<6, TB>(xr, xi, op, pool);
+        // instantiated up to the pass's widest group only: a kernel holding the 6-qubit
+        // product's registers runs at lower occupancy
+        if (KMAX == 4 || op.k == 4) mt_dense<4, TB>(xr, xi, op, pool);
+        else if (KMAX == 5 || op.k == 5) mt_dense<KMAX == 4 ? 4 : 5, TB>(xr, xi, op, pool);
+        else mt_dense<KMAX, TB>(xr, xi, op, pool);
         ++i;
       } else {
-        uint32_t j = i + 1;
-        while (j < pa.n_ops && pops[j].kind == 1) ++j;
+        const uint32_t j = i + op.run;
         __syncthreads();  // the diagonal run's amplitudes were staged by other lanes
         mt_diag_run<TB>(xr, xi, pops, i, j, base, pool);
         i = j;
@@ -676,14 +698,24 @@ __global__ __launch_bounds__(kBlock) void k_mfma_tiles(float* __restrict__ amp, 
   }
 }
 
-template <int TB, bool NT>
+template <int TB, int KMAX, bool NT>
 hipError_t launch_mt(const State& st, const MtPassArg& pa, const MtOp* ops, const void* pool,
                      const Tuning& t, hipStream_t s) {
   const uint64_t n_tiles = st.dim() >> TB;
   const int cus = st.ctx && st.ctx->cus > 0 ? st.ctx->cus : 256;
-  const uint64_t cap = t.mfma_blocks > 0 ? uint64_t(t.mfma_blocks) : uint64_t(2) * cus;
+  // default grid: as many workgroups as fit on the chip at once (measured best: fewer
+  // leave LDS latency unhidden, more queue behind the resident ones)
+  static const int per_cu = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_mfma_tiles<TB, KMAX, NT>, kBlock,
+                                                     0) != hipSuccess || n < 1)
+      n = 1;
+    return n;
+  }();
+  const uint64_t cap =
+      t.mfma_blocks > 0 ? uint64_t(t.mfma_blocks) : uint64_t(per_cu) * uint64_t(cus);
   const uint64_t g = n_tiles < cap ? n_tiles : cap;
-  k_mfma_tiles<TB, NT><<<unsigned(g), kBlock, 0, s>>>(
+  k_mfma_tiles<TB, KMAX, NT><<<unsigned(g), kBlock, 0, s>>>(
       static_cast<float*>(st.amp), n_tiles, pa, ops,
       static_cast<const unsigned char*>(pool), t.xcd);
   return hipGetLastError();
@@ -693,14 +725,23 @@ hipError_t launch_mt(const State& st, const MtPassArg& pa, const MtOp* ops, cons
 
 hipError_t launch_mfma_tiles(const State& st, const MtPassArg& pa, const MtOp* ops,
                              const void* pool, const Tuning& t, hipStream_t s) {
-  if (st.dtype != 1 || (pa.tb != 12 && pa.tb != 13) || st.n < pa.tb || pa.ts[0] != 0)
+  if (st.dtype != 1 || (pa.tb != 12 && pa.tb != 13) || st.n < pa.tb || pa.ts[0] != 0 ||
+      pa.kmax < 4 || pa.kmax > 6)
     return hipErrorInvalidValue;
   const bool nt = nontemporal_for(st, t, true);
-  if (pa.tb == 12)
-    return nt ? launch_mt<12, true>(st, pa, ops, pool, t, s)
-              : launch_mt<12, false>(st, pa, ops, pool, t, s);
-  return nt ? launch_mt<13, true>(st, pa, ops, pool, t, s)
-            : launch_mt<13, false>(st, pa, ops, pool, t, s);
+  auto by_k = [&](auto tb) {
+    constexpr int TB = decltype(tb)::value;
+    switch (pa.kmax) {
+      case 4: return nt ? launch_mt<TB, 4, true>(st, pa, ops, pool, t, s)
+                        : launch_mt<TB, 4, false>(st, pa, ops, pool, t, s);
+      case 5: return nt ? launch_mt<TB, 5, true>(st, pa, ops, pool, t, s)
+                        : launch_mt<TB, 5, false>(st, pa, ops, pool, t, s);
+      default: return nt ? launch_mt<TB, 6, true>(st, pa, ops, pool, t, s)
+                         : launch_mt<TB, 6, false>(st, pa, ops, pool, t, s);
+    }
+  };
+  return pa.tb == 12 ? by_k(std::integral_constant<int, 12>{})
+                     : by_k(std::integral_constant<int, 13>{});
 }
 
 }  // namespace qsvh

@@ -763,6 +763,7 @@ int plan_mfma_tiles(Program* p, int tb) {
     for (uint32_t b = 0; b < st.n && popcount64(tm) < uint32_t(tb); ++b) tm |= 1ull << b;
     MtPassArg pa{};
     pa.tb = uint32_t(tb);
+    pa.kmax = 4;
     uint32_t pos[64] = {0};
     uint32_t nb = 0;
     for (uint32_t b = 0; b < st.n; ++b)
@@ -781,9 +782,9 @@ int plan_mfma_tiles(Program* p, int tb) {
           const uint32_t q = l.q[j];
           if ((tm >> q) & 1) {
             op.in_mask |= 1u << j;
-            op.bit[j] = uint8_t(pos[q]);
+            op.set_bit(j, pos[q]);
           } else {
-            op.bit[j] = uint8_t(q);
+            op.set_bit(j, q);
           }
         }
         std::vector<float> ph(size_t(2) << l.m);
@@ -801,14 +802,15 @@ int plan_mfma_tiles(Program* p, int tb) {
         const std::vector<float> a = block_matrix(g, sorted);
         op.kind = 0;
         op.k = g.m;
+        pa.kmax = std::max(pa.kmax, g.m);
         uint32_t gm = 0;
         for (uint32_t j = 0; j < g.m; ++j) {
-          op.bit[j] = uint8_t(pos[sorted[j]]);
+          op.set_bit(j, pos[sorted[j]]);
           gm |= 1u << pos[sorted[j]];
         }
         uint32_t o = 0;
         for (uint32_t b = 0; b < uint32_t(tb); ++b)
-          if (!((gm >> b) & 1)) op.obit[o++] = uint8_t(b);
+          if (!((gm >> b) & 1)) op.set_obit(o++, b);
         std::vector<uint16_t> hl(2 * a.size());
         for (size_t e = 0; e < a.size(); ++e) {
           hl[e] = bf16_bits(a[e]);
@@ -819,6 +821,9 @@ int plan_mfma_tiles(Program* p, int tb) {
       ops.push_back(op);
     }
     pa.n_ops = uint32_t(ops.size()) - pa.op_begin;
+    for (size_t e = ops.size(); e-- > pa.op_begin;)
+      if (ops[e].kind == 1)
+        ops[e].run = 1 + (e + 1 < ops.size() && ops[e + 1].kind == 1 ? ops[e + 1].run : 0);
     Launch L;
     L.kind = Launch::kMfmaTile;
     L.arena_off = p->mt_passes.size();
