@@ -362,11 +362,12 @@ def main():
     planned, pred_ms, chosen = planner.plan(b, model, mixc, 6)
     t_plan = time.perf_counter() - t0
     formc, form_tq, form_pred, form_chosen = planner.plan_form(b, model, tile_model, mixc, rs.C64)
-    cands = [("greedy_k%d" % k, rs.fuse(b, mixc, k), 0) for k in (4, 5, 6)]
-    cands += [("learned", planned, 0), ("learned_form", formc, form_tq)]
-    for name, circ, tq in cands:
+    cands = [("greedy_k%d" % k, rs.fuse(b, mixc, k), 0, 0) for k in (4, 5, 6)]
+    cands += [("learned", planned, 0, 0), ("learned_form", formc, form_tq, 0)]
+    cands += [("greedy_k%d_mfma_tiles12" % k, rs.fuse(b, mixc, k), 0, 12) for k in (4, 5)]
+    for name, circ, tq, mt in cands:
         p = rs.Program(st64, circ, precision=rs.PREC_EXACT if tq else rs.PREC_MFMA_BF16X3,
-                       tile_qubits=tq)
+                       tile_qubits=tq, mfma_tiles=mt)
         s = timed_steps(b, st64, p, 2, 1, reset64)
         reset64()
         p.launch()
@@ -446,13 +447,16 @@ def main():
     planned, _, chosen = planner.plan(b, model, lay, 6)
     formc, form_tq, _, form_chosen = planner.plan_form(b, model, tile_model, lay, rs.C64)
     lay_legs = {}
-    cands = [("greedy_k%d" % k, rs.fuse(b, lay, k), rs.PREC_MFMA_BF16X3, 0) for k in (4, 5, 6)]
-    cands += [("learned", planned, rs.PREC_MFMA_BF16X3, 0),
+    cands = [("greedy_k%d" % k, rs.fuse(b, lay, k), rs.PREC_MFMA_BF16X3, 0, 0)
+             for k in (4, 5, 6)]
+    cands += [("learned", planned, rs.PREC_MFMA_BF16X3, 0, 0),
               ("learned_form", formc, rs.PREC_EXACT if form_tq else rs.PREC_MFMA_BF16X3,
-               form_tq),
-              ("tiled_c64_exact", lay, rs.PREC_EXACT, 13)]
-    for name, circ, prec, tq in cands:
-        p = rs.Program(st64, circ, precision=prec, tile_qubits=tq)
+               form_tq, 0),
+              ("tiled_c64_exact", lay, rs.PREC_EXACT, 13, 0)]
+    cands += [("greedy_k%d_mfma_tiles12" % k, rs.fuse(b, lay, k), rs.PREC_MFMA_BF16X3, 0, 12)
+              for k in (4, 5)]
+    for name, circ, prec, tq, mt in cands:
+        p = rs.Program(st64, circ, precision=prec, tile_qubits=tq, mfma_tiles=mt)
         s = timed_steps(b, st64, p, 2, 1, reset64)
         reset64()
         p.launch()
