@@ -234,11 +234,12 @@ typedef struct qsvh_program_opts {
                        diagonal gate with one non-unit phase (CP, CZ, CCZ) that phase's
                        amplitudes.  Values equal the full pass (the skipped products are
                        exact zeros and ones); the bytes moved drop to 1/2 .. 1/8.        */
-  int mfma_tiles;   /* 12 or 13 (precision MFMA_BF16X3): runs of ops -- fused groups of up to
-                       6 qubits, dense (split-BF16 MFMA) or diagonal -- applied to a
-                       2^mfma_tiles-amplitude LDS tile per HBM pass instead of one pass
-                       each; consecutive ops share a pass while its dense ops' qubits and
-                       qubits 0..3 fit the tile.  Excludes tile_qubits and subspace.    */
+  int mfma_tiles;   /* 12 (precision MFMA_BF16X3): runs of ops -- fused groups of up to 6
+                       qubits, dense (split-BF16 MFMA) or diagonal -- applied to a
+                       2^12-amplitude LDS tile per HBM pass instead of one pass each;
+                       consecutive ops share a pass while its dense ops' qubits and qubits
+                       0..3 fit the tile and its diagonal ops' phases fit 2048 entries.
+                       Excludes tile_qubits and subspace.                                */
   int reserved[3];
 } qsvh_program_opts;
 

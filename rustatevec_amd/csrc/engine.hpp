@@ -130,8 +130,8 @@ hipError_t launch_join(const State& st, uint64_t first, uint64_t count, const vo
 hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, const void* mats,
                              int precision, const Tuning& t, hipStream_t s);
 
-// MFMA tile passes: a run of fused groups applied to a tile of 2^tb amplitudes (tb = 12 or
-// 13, complex64) held in LDS as f32, one HBM round trip for the run.  Dense groups
+// MFMA tile passes: a run of fused groups applied to a tile of 2^12 amplitudes (complex64)
+// held in LDS as f32, one HBM round trip for the run.  Dense groups
 // (k = 4..6 qubits, all in the tile) are split-BF16x3 MFMA products; diagonal groups (up to
 // 6 qubits anywhere: the qubits outside the tile are fixed per tile) scale each amplitude.
 // Bit positions are bytes packed in dword words (byte j of `bits` at bits[j / 4] >> 8 (j % 4)),
@@ -142,19 +142,23 @@ struct MtOp {
   uint32_t in_mask;   // diagonal: bit j set when qubit j is in the tile
   uint32_t run;       // diagonal: the consecutive diagonal ops from this one to the pass's
                       // next dense op or end (this one included)
+  uint32_t tab;       // diagonal: first entry of its phases in the pass's table
   uint32_t bits[2];   // byte j: dense: tile-local bit j of the group, ascending; diagonal:
                       // tile-local bit of qubit j (in the tile) or its state bit (outside)
   uint32_t obits[3];  // byte i: dense: the tb - k other tile-local bits, ascending
-  uint64_t mat;       // byte offset in the pass pool: dense: bf16 block matrix hi, then lo
-                      // (2D x 2D each); diagonal: 2^m complex f32 phases (re, im)
+  uint64_t mat;       // dense: byte offset in the pool of the bf16 block matrix hi, then lo
+                      // (2D x 2D each)
   void set_bit(uint32_t j, uint32_t b) { bits[j >> 2] |= (b & 0xffu) << (8 * (j & 3)); }
   void set_obit(uint32_t i, uint32_t b) { obits[i >> 2] |= (b & 0xffu) << (8 * (i & 3)); }
 };
+constexpr uint32_t kMtTabEntries = 2048;  // phase entries of a pass's diagonal ops (LDS)
 struct MtPassArg {
   uint32_t tb;
   uint32_t kmax;     // widest dense group of the pass (4..6; 4 when it has none)
   uint32_t ts[13];   // state bits of the tile, ascending (tile-local bit j = state bit ts[j])
   uint32_t op_begin, n_ops;
+  uint32_t tab_entries;  // the pass's phase table: complex f32 entries at pool + tab_off
+  uint64_t tab_off;
 };
 hipError_t launch_mfma_tiles(const State& st, const MtPassArg& pa, const MtOp* ops,
                              const void* pool, const Tuning& t, hipStream_t s);

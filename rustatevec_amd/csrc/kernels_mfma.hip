@@ -561,73 +561,62 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
 // A run of diagonal groups [i0, i1): the lane's amplitudes e = tid + 256 j stay in
 // registers across the run.  A group's phase index is its qubits' bits of the state index:
 // those outside the tile come from the tile's base (the same for the whole tile), those in
-// it from e, split into the lane's part and a per-j part; the phase table (2^m <= 64
-// entries) sits one entry per lane and is read across lanes.
+// it from e, split into the lane's part and a per-j part; the phases are read from the
+// pass's table in LDS.
 template <int TB>
-__device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const MtOp* ops, uint32_t i0,
-                                            uint32_t i1, uint64_t base,
-                                            const unsigned char* pool) {
+__device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const float2* tabs,
+                                            const MtOp* ops, uint32_t i0, uint32_t i1,
+                                            uint64_t base) {
   constexpr int E = (1 << TB) / kBlock;
-  const int lane = threadIdx.x & 63;
   // opaque, so that the amplitudes' LDS words are formed here, not held across the tiles
   uint32_t tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
-  auto table = [&](uint32_t i) {
+  float vr[E], vi[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const uint32_t w = swz(tid + kBlock * j);
+    vr[j] = xr[w];
+    vi[j] = xi[w];
+  }
+  for (uint32_t i = i0; i < i1; ++i) {
     const MtOp& op = ops[i];
-    return lane < (1 << op.k) ? reinterpret_cast<const float2*>(pool + op.mat)[lane]
-                              : float2{0.f, 0.f};
-  };
-  {
-    float vr[E], vi[E];
+    const float2* tab = tabs + op.tab;
+    uint32_t fix = 0, lp = 0;  // out-of-tile bits; the lane's bits (tile-local bits < 8)
+    uint32_t g8[TB - 8];            // index bits of tile-local bit 8 + b
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const uint32_t w = swz(tid + kBlock * j);
-      vr[j] = xr[w];
-      vi[j] = xi[w];
-    }
-    float2 tab = table(i0);
-    for (uint32_t i = i0; i < i1; ++i) {
-      const MtOp& op = ops[i];
-      const float2 cur = tab;
-      if (i + 1 < i1) tab = table(i + 1);  // the next group's entry loads meanwhile
-      uint32_t fix = 0, lp = 0;  // out-of-tile bits; the lane's bits (tile-local bits < 8)
-      uint32_t g8[TB - 8];       // index bits of tile-local bit 8 + b
-#pragma unroll
-      for (int b = 0; b < TB - 8; ++b) g8[b] = 0;
-      const uint64_t bw = uint64_t(op.bits[0]) | (uint64_t(op.bits[1]) << 32);
-      for (uint32_t q = 0; q < op.k; ++q) {
-        const uint32_t b = uint32_t(bw >> (8 * q)) & 0xffu;
-        if (!((op.in_mask >> q) & 1u)) {
-          fix |= uint32_t((base >> b) & 1u) << q;
-        } else if (b < 8) {
-          lp |= ((tid >> b) & 1u) << q;
-        } else {
-#pragma unroll
-          for (int c = 0; c < TB - 8; ++c)
-            if (int(b) == 8 + c) g8[c] = 1u << q;
-        }
-      }
-      lp |= fix;
-#pragma unroll
-      for (int j = 0; j < E; ++j) {
-        if ((j & 3) == 0) asm volatile("" : "+v"(lp));  // indices formed 4 at a time
-        uint32_t idx = lp;
+    for (int b = 0; b < TB - 8; ++b) g8[b] = 0;
+    const uint64_t bw = uint64_t(op.bits[0]) | (uint64_t(op.bits[1]) << 32);
+    for (uint32_t q = 0; q < op.k; ++q) {
+      const uint32_t b = uint32_t(bw >> (8 * q)) & 0xffu;
+      if (!((op.in_mask >> q) & 1u)) {
+        fix |= uint32_t((base >> b) & 1u) << q;
+      } else if (b < 8) {
+        lp |= ((tid >> b) & 1u) << q;
+      } else {
 #pragma unroll
         for (int c = 0; c < TB - 8; ++c)
-          if ((j >> c) & 1) idx ^= g8[c];
-        const float pr = __shfl(cur.x, int(idx));
-        const float pi = __shfl(cur.y, int(idx));
-        const float a = vr[j], b = vi[j];
-        vr[j] = a * pr - b * pi;
-        vi[j] = a * pi + b * pr;
+          if (int(b) == 8 + c) g8[c] = 1u << q;
       }
     }
+    lp |= fix;
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      const uint32_t w = swz(tid + kBlock * j);
-      xr[w] = vr[j];
-      xi[w] = vi[j];
+      if ((j & 3) == 0) asm volatile("" : "+v"(lp));  // indices formed 4 at a time
+      uint32_t idx = lp;
+#pragma unroll
+      for (int c = 0; c < TB - 8; ++c)
+        if ((j >> c) & 1) idx ^= g8[c];
+      const float2 ph = tab[idx];
+      const float a = vr[j], b = vi[j];
+      vr[j] = a * ph.x - b * ph.y;
+      vi[j] = a * ph.y + b * ph.x;
     }
+  }
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const uint32_t w = swz(tid + kBlock * j);
+    xr[w] = vr[j];
+    xi[w] = vi[j];
   }
 }
 
@@ -640,6 +629,11 @@ __global__ __launch_bounds__(kBlock) void k_mfma_tiles(float* __restrict__ amp, 
   constexpr int N = 1 << TB;
   constexpr int CH = N / 2 / kBlock;  // 16-byte chunks (2 amplitudes) per lane
   __shared__ float xr[N], xi[N];
+  __shared__ float2 tabs[kMtTabEntries];
+  {  // the pass's phase table, once per workgroup
+    const float2* src = reinterpret_cast<const float2*>(pool + pa.tab_off);
+    for (uint32_t e = threadIdx.x; e < pa.tab_entries; e += kBlock) tabs[e] = src[e];
+  }
   // chunk c of a lane holds tile-local indices 2 (tid + 256 c) + {0, 1}: against chunk 0 it
   // sets tile-local bits 9.., the same for every lane (tile-local bit 0 is state bit 0)
   uint64_t off0 = mt_offset(2u * threadIdx.x, pa);
@@ -676,7 +670,7 @@ __global__ __launch_bounds__(kBlock) void k_mfma_tiles(float* __restrict__ amp, 
       } else {
         const uint32_t j = i + op.run;
         __syncthreads();  // the diagonal run's amplitudes were staged by other lanes
-        mt_diag_run<TB>(xr, xi, pops, i, j, base, pool);
+        mt_diag_run<TB>(xr, xi, tabs, pops, i, j, base);
         i = j;
       }
     }
@@ -725,8 +719,8 @@ hipError_t launch_mt(const State& st, const MtPassArg& pa, const MtOp* ops, cons
 
 hipError_t launch_mfma_tiles(const State& st, const MtPassArg& pa, const MtOp* ops,
                              const void* pool, const Tuning& t, hipStream_t s) {
-  if (st.dtype != 1 || (pa.tb != 12 && pa.tb != 13) || st.n < pa.tb || pa.ts[0] != 0 ||
-      pa.kmax < 4 || pa.kmax > 6)
+  if (st.dtype != 1 || pa.tb != 12 || st.n < pa.tb || pa.ts[0] != 0 || pa.kmax < 4 ||
+      pa.kmax > 6 || pa.tab_entries > kMtTabEntries)
     return hipErrorInvalidValue;
   const bool nt = nontemporal_for(st, t, true);
   auto by_k = [&](auto tb) {
@@ -740,8 +734,7 @@ hipError_t launch_mfma_tiles(const State& st, const MtPassArg& pa, const MtOp* o
                          : launch_mt<TB, 6, false>(st, pa, ops, pool, t, s);
     }
   };
-  return pa.tb == 12 ? by_k(std::integral_constant<int, 12>{})
-                     : by_k(std::integral_constant<int, 13>{});
+  return by_k(std::integral_constant<int, 12>{});
 }
 
 }  // namespace qsvh

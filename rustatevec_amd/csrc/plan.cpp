@@ -741,7 +741,7 @@ int plan_mfma_tiles(Program* p, int tb) {
   State& st = *p->st;
   if (p->precision != QSVH_PREC_MFMA_BF16X3)
     return set_err(st.ctx, kInvalidArg, "mfma_tiles runs split-BF16 groups (MFMA_BF16X3)");
-  if (tb != 12 && tb != 13) return set_err(st.ctx, kInvalidArg, "mfma_tiles must be 12 or 13");
+  if (tb != 12) return set_err(st.ctx, kInvalidArg, "mfma_tiles must be 12");
   if (st.n < uint32_t(tb)) return plan_mfma(p);  // smaller than one tile
   const std::vector<Launch> in = std::move(p->launches);
   p->launches.clear();
@@ -750,6 +750,7 @@ int plan_mfma_tiles(Program* p, int tb) {
   std::vector<size_t> cur;
   const uint64_t low = (1ull << kTileLowBits) - 1;
   uint64_t S = low;
+  uint32_t cur_tab = 0;  // phase entries of the pass's diagonal ops so far
   auto put = [&](const void* src, size_t bytes) {
     pool.resize((pool.size() + 255) & ~size_t(255));
     const uint64_t at = pool.size();
@@ -772,6 +773,7 @@ int plan_mfma_tiles(Program* p, int tb) {
         pos[b] = nb++;
       }
     pa.op_begin = uint32_t(ops.size());
+    std::vector<float> tabs;  // the pass's phase table, staged in LDS once per workgroup
     for (size_t li : cur) {
       const Launch& l = in[li];
       MtOp op{};
@@ -787,9 +789,8 @@ int plan_mfma_tiles(Program* p, int tb) {
             op.set_bit(j, q);
           }
         }
-        std::vector<float> ph(size_t(2) << l.m);
-        for (size_t e = 0; e < ph.size(); ++e) ph[e] = float(l.mat[e]);
-        op.mat = put(ph.data(), ph.size() * sizeof(float));
+        op.tab = uint32_t(tabs.size() / 2);
+        for (size_t e = 0; e < (size_t(2) << l.m); ++e) tabs.push_back(float(l.mat[e]));
       } else {
         Launch g = l;
         if (l.m < 4) {
@@ -821,6 +822,8 @@ int plan_mfma_tiles(Program* p, int tb) {
       ops.push_back(op);
     }
     pa.n_ops = uint32_t(ops.size()) - pa.op_begin;
+    pa.tab_entries = uint32_t(tabs.size() / 2);
+    pa.tab_off = tabs.empty() ? 0 : put(tabs.data(), tabs.size() * sizeof(float));
     for (size_t e = ops.size(); e-- > pa.op_begin;)
       if (ops[e].kind == 1)
         ops[e].run = 1 + (e + 1 < ops.size() && ops[e + 1].kind == 1 ? ops[e + 1].run : 0);
@@ -831,10 +834,13 @@ int plan_mfma_tiles(Program* p, int tb) {
     p->launches.push_back(std::move(L));
     cur.clear();
     S = low;
+    cur_tab = 0;
   };
   for (size_t i = 0; i < in.size(); ++i) {
     const Launch& l = in[i];
     if (l.kind == Launch::kDiag && l.m <= 6) {
+      if (cur_tab + (1u << l.m) > kMtTabEntries) close();
+      cur_tab += 1u << l.m;
       cur.push_back(i);
     } else if ((l.kind == Launch::k1q || l.kind == Launch::kMq) && l.m <= 6) {
       uint64_t Q = 0;
