@@ -534,6 +534,9 @@ def main():
         "plain_subspace": subspace,
         "sampling": sampling,
         "fused": fused_out,
+        # the headline's second half: the fastest fused form of QFT(30) within the error bar
+        "fused_best": min((r for r in fused_out if r["max_abs_err"] <= 1e-6),
+                          key=lambda r: r["circuit_s"], default=None),
         "mfma_roofline": mfma_roof,
         "layered28": {"plain_s": round(lay_s, 4), "plain_passes": lay_passes,
                       "tiled_s": round(lay_t, 4), "tiled_passes": lay_tp,
