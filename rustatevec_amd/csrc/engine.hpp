@@ -136,15 +136,23 @@ hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, con
 // 6 qubits anywhere: the qubits outside the tile are fixed per tile) scale each amplitude.
 // Bit positions are bytes packed in dword words (byte j of `bits` at bits[j / 4] >> 8 (j % 4)),
 // so that a kernel reads every field with scalar (uniform) loads.
+// Kinds: 0 a dense group; 1 a diagonal group applied on its own; 2 the head of a merged run
+// of diagonal groups -- `members` kind-3 ops follow it, whose in-tile qubits span at most 8
+// tile bits (the head's `bits`, k of them): per tile, the product of their phases over those
+// bits is built as one table, then applied once.
 struct MtOp {
-  uint32_t kind;      // 0 dense group, 1 diagonal
-  uint32_t k;         // dense: group qubits (4..6); diagonal: m (1..6)
-  uint32_t in_mask;   // diagonal: bit j set when qubit j is in the tile
-  uint32_t run;       // diagonal: the consecutive diagonal ops from this one to the pass's
-                      // next dense op or end (this one included)
-  uint32_t tab;       // diagonal: first entry of its phases in the pass's table
+  uint32_t kind;
+  uint32_t k;         // dense: group qubits (4..6); diagonal / member: m (1..6); head: its
+                      // in-tile bits (<= 8)
+  uint32_t in_mask;   // diagonal / member: bit j set when qubit j is in the tile
+  uint32_t run;       // kinds 1..3: the consecutive non-dense ops from this one to the
+                      // pass's next dense op or end (this one included)
+  uint32_t tab;       // diagonal / member: first entry of its phases in the pass's table
+  uint32_t members;   // head: the member ops that follow it
   uint32_t bits[2];   // byte j: dense: tile-local bit j of the group, ascending; diagonal:
-                      // tile-local bit of qubit j (in the tile) or its state bit (outside)
+                      // tile-local bit of qubit j (in the tile) or its state bit (outside);
+                      // member: the head's index of qubit j's tile bit, or its state bit;
+                      // head: its tile-local bits, ascending
   uint32_t obits[3];  // byte i: dense: the tb - k other tile-local bits, ascending
   uint64_t mat;       // dense: byte offset in the pool of the bf16 block matrix hi, then lo
                       // (2D x 2D each)

@@ -558,15 +558,17 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
   __syncthreads();
 }
 
-// A run of diagonal groups [i0, i1): the lane's amplitudes e = tid + 256 j stay in
-// registers across the run.  A group's phase index is its qubits' bits of the state index:
-// those outside the tile come from the tile's base (the same for the whole tile), those in
-// it from e, split into the lane's part and a per-j part; the phases are read from the
-// pass's table in LDS.
+// A run of diagonal ops [i0, i1): the lane's amplitudes e = tid + 256 j stay in registers
+// across the run.  A phase index is the bits of the op's qubits in the state index: those
+// outside the tile come from the tile's base (the same for the whole tile), those in it
+// from e, split into the lane's part and a per-j part.  A single diagonal op reads its
+// phases from the pass's table; a merged run (head op, then members) first builds, per
+// tile, the product of its members' phases over its <= 8 tile bits (one entry per thread,
+// in one of two LDS tables used in turn), then applies that table once.
 template <int TB>
 __device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const float2* tabs,
-                                            const MtOp* ops, uint32_t i0, uint32_t i1,
-                                            uint64_t base) {
+                                            float2* tgroup, const MtOp* ops, uint32_t i0,
+                                            uint32_t i1, uint64_t base) {
   constexpr int E = (1 << TB) / kBlock;
   // opaque, so that the amplitudes' LDS words are formed here, not held across the tiles
   uint32_t tid = threadIdx.x;
@@ -578,17 +580,16 @@ __device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const float2* 
     vr[j] = xr[w];
     vi[j] = xi[w];
   }
-  for (uint32_t i = i0; i < i1; ++i) {
-    const MtOp& op = ops[i];
-    const float2* tab = tabs + op.tab;
+  // multiply every amplitude by tab[its bits at the k positions `bw` (tile-local bits where
+  // in_mask is set, else state bits)]
+  auto apply = [&](const float2* tab, uint32_t k, uint32_t in_mask, uint64_t bw) {
     uint32_t fix = 0, lp = 0;  // out-of-tile bits; the lane's bits (tile-local bits < 8)
-    uint32_t g8[TB - 8];            // index bits of tile-local bit 8 + b
+    uint32_t g8[TB - 8];       // index bits of tile-local bit 8 + b
 #pragma unroll
-    for (int b = 0; b < TB - 8; ++b) g8[b] = 0;
-    const uint64_t bw = uint64_t(op.bits[0]) | (uint64_t(op.bits[1]) << 32);
-    for (uint32_t q = 0; q < op.k; ++q) {
+    for (int c = 0; c < TB - 8; ++c) g8[c] = 0;
+    for (uint32_t q = 0; q < k; ++q) {
       const uint32_t b = uint32_t(bw >> (8 * q)) & 0xffu;
-      if (!((op.in_mask >> q) & 1u)) {
+      if (!((in_mask >> q) & 1u)) {
         fix |= uint32_t((base >> b) & 1u) << q;
       } else if (b < 8) {
         lp |= ((tid >> b) & 1u) << q;
@@ -611,6 +612,38 @@ __device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const float2* 
       vr[j] = a * ph.x - b * ph.y;
       vi[j] = a * ph.y + b * ph.x;
     }
+  };
+  auto words = [](const MtOp& o) { return uint64_t(o.bits[0]) | (uint64_t(o.bits[1]) << 32); };
+  uint32_t g = 0;
+  for (uint32_t i = i0; i < i1;) {
+    const MtOp& op = ops[i];
+    if (op.kind == 1) {
+      apply(tabs + op.tab, op.k, op.in_mask, words(op));
+      ++i;
+      continue;
+    }
+    float2* T = tgroup + 256 * (g++ & 1u);
+    if (tid < (1u << op.k)) {  // entry tid: the head's bits r set where bit r of tid is
+      float pr = 1.f, pi = 0.f;
+      for (uint32_t m = i + 1; m <= i + op.members; ++m) {
+        const MtOp& mo = ops[m];
+        const uint64_t bw = words(mo);
+        uint32_t idx = 0;
+        for (uint32_t q = 0; q < mo.k; ++q) {
+          const uint32_t b = uint32_t(bw >> (8 * q)) & 0xffu;
+          const uint32_t v = ((mo.in_mask >> q) & 1u) ? (tid >> b) : uint32_t(base >> b);
+          idx |= (v & 1u) << q;
+        }
+        const float2 ph = tabs[mo.tab + idx];
+        const float a = pr;
+        pr = a * ph.x - pi * ph.y;
+        pi = a * ph.y + pi * ph.x;
+      }
+      T[tid] = float2{pr, pi};
+    }
+    __syncthreads();  // the table is complete; the other one's last readers are done
+    apply(T, op.k, (1u << op.k) - 1u, words(op));
+    i += 1 + op.members;
   }
 #pragma unroll
   for (int j = 0; j < E; ++j) {
@@ -630,6 +663,7 @@ __global__ __launch_bounds__(kBlock) void k_mfma_tiles(float* __restrict__ amp, 
   constexpr int CH = N / 2 / kBlock;  // 16-byte chunks (2 amplitudes) per lane
   __shared__ float xr[N], xi[N];
   __shared__ float2 tabs[kMtTabEntries];
+  __shared__ float2 tgroup[2 * 256];  // merged diagonal runs' per-tile tables
   {  // the pass's phase table, once per workgroup
     const float2* src = reinterpret_cast<const float2*>(pool + pa.tab_off);
     for (uint32_t e = threadIdx.x; e < pa.tab_entries; e += kBlock) tabs[e] = src[e];
@@ -670,7 +704,7 @@ __global__ __launch_bounds__(kBlock) void k_mfma_tiles(float* __restrict__ amp, 
       } else {
         const uint32_t j = i + op.run;
         __syncthreads();  // the diagonal run's amplitudes were staged by other lanes
-        mt_diag_run<TB>(xr, xi, tabs, pops, i, j, base);
+        mt_diag_run<TB>(xr, xi, tabs, tgroup, pops, i, j, base);
         i = j;
       }
     }

@@ -821,12 +821,58 @@ int plan_mfma_tiles(Program* p, int tb) {
       }
       ops.push_back(op);
     }
+    // merge runs of diagonal ops: consecutive ones (they commute) whose in-tile qubits fit 8
+    // tile bits share one per-tile table (a head op, then its members)
+    {
+      std::vector<MtOp> merged(ops.begin(), ops.begin() + pa.op_begin);
+      size_t e = pa.op_begin;
+      while (e < ops.size()) {
+        if (ops[e].kind != 1) {
+          merged.push_back(ops[e++]);
+          continue;
+        }
+        uint32_t U = 0;  // tile-local bits of the group
+        size_t f = e;
+        for (; f < ops.size() && ops[f].kind == 1; ++f) {
+          uint32_t u = U;
+          for (uint32_t j = 0; j < ops[f].k; ++j)
+            if ((ops[f].in_mask >> j) & 1) u |= 1u << ((ops[f].bits[j >> 2] >> (8 * (j & 3))) & 0xff);
+          if (popcount64(u) > 8) break;
+          U = u;
+        }
+        if (f - e < 2) {
+          merged.push_back(ops[e++]);
+          continue;
+        }
+        MtOp head{};
+        head.kind = 2;
+        head.members = uint32_t(f - e);
+        uint32_t rank[32] = {0};
+        for (uint32_t b = 0; b < 32; ++b)
+          if ((U >> b) & 1) {
+            rank[b] = head.k;
+            head.set_bit(head.k++, b);
+          }
+        merged.push_back(head);
+        for (; e < f; ++e) {
+          MtOp m = ops[e];
+          m.kind = 3;
+          m.bits[0] = m.bits[1] = 0;
+          for (uint32_t j = 0; j < m.k; ++j) {
+            const uint32_t b = (ops[e].bits[j >> 2] >> (8 * (j & 3))) & 0xff;
+            m.set_bit(j, ((m.in_mask >> j) & 1) ? rank[b] : b);
+          }
+          merged.push_back(m);
+        }
+      }
+      ops.swap(merged);
+    }
     pa.n_ops = uint32_t(ops.size()) - pa.op_begin;
     pa.tab_entries = uint32_t(tabs.size() / 2);
     pa.tab_off = tabs.empty() ? 0 : put(tabs.data(), tabs.size() * sizeof(float));
     for (size_t e = ops.size(); e-- > pa.op_begin;)
-      if (ops[e].kind == 1)
-        ops[e].run = 1 + (e + 1 < ops.size() && ops[e + 1].kind == 1 ? ops[e + 1].run : 0);
+      if (ops[e].kind != 0)
+        ops[e].run = 1 + (e + 1 < ops.size() && ops[e + 1].kind != 0 ? ops[e + 1].run : 0);
     Launch L;
     L.kind = Launch::kMfmaTile;
     L.arena_off = p->mt_passes.size();
