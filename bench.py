@@ -343,9 +343,22 @@ def main():
                                           per_size=300, seed=seed)
     tile_model, tile_loss = planner.train_tile_model(b, XT, yT, epochs=2000)
     t_tile_model = time.perf_counter() - t0
+    # and its MFMA-tile-pass model (greedy fusion run as split-BF16 MFMA tile passes)
+    t0 = time.perf_counter()
+    XM, yM = planner.collect_mt_timings(b, sizes=(24,) if args.quick else (26,),
+                                        per_size=300, seed=seed)
+    mt_model, mt_loss = planner.train_mt_model(b, XM, yM, epochs=2000)
+    t_mt_model = time.perf_counter() - t0
 
     def form_name(tq, chosen):
+        if chosen >= 100:
+            return f"mfma_tiles_k{chosen - 100}"
         return f"tiles_t{tq}" if tq else ("dp" if chosen == 0 else f"greedy_k{chosen}")
+
+    def form_program(st_, circ, tq, mt):
+        """A program in the form plan_form chose: exact tiles, MFMA tiles or fused groups."""
+        return rs.Program(st_, circ, precision=rs.PREC_EXACT if tq else rs.PREC_MFMA_BF16X3,
+                          tile_qubits=tq, mfma_tiles=mt)
     nq = 22 if args.quick else 28
     mixc = rs.mix(nq, 40 * nq, seed)
     ref = b.alloc(nq, rs.C128)
@@ -361,9 +374,10 @@ def main():
     t0 = time.perf_counter()
     planned, pred_ms, chosen = planner.plan(b, model, mixc, 6)
     t_plan = time.perf_counter() - t0
-    formc, form_tq, form_pred, form_chosen = planner.plan_form(b, model, tile_model, mixc, rs.C64)
+    formc, form_tq, form_mt, form_pred, form_chosen = planner.plan_form(
+        b, model, tile_model, mixc, rs.C64, mt_model=mt_model)
     cands = [("greedy_k%d" % k, rs.fuse(b, mixc, k), 0, 0) for k in (4, 5, 6)]
-    cands += [("learned", planned, 0, 0), ("learned_form", formc, form_tq, 0)]
+    cands += [("learned", planned, 0, 0), ("learned_form", formc, form_tq, form_mt)]
     cands += [("greedy_k%d_mfma_tiles12" % k, rs.fuse(b, mixc, k), 0, 12) for k in (4, 5)]
     for name, circ, tq, mt in cands:
         p = rs.Program(st64, circ, precision=rs.PREC_EXACT if tq else rs.PREC_MFMA_BF16X3,
@@ -386,6 +400,9 @@ def main():
                    "form_planner": {"tile_timings": int(len(XT)),
                                     "tile_train_mse_std": round(tile_loss, 4),
                                     "tile_model_s": round(t_tile_model, 2),
+                                    "mt_timings": int(len(XM)),
+                                    "mt_train_mse_std": round(mt_loss, 4),
+                                    "mt_model_s": round(t_mt_model, 2),
                                     "chosen": form_name(form_tq, form_chosen),
                                     "predicted_ms": round(form_pred, 3),
                                     "gain_vs_greedy_k4": round(
@@ -413,15 +430,16 @@ def main():
         st64 = b.alloc(nq2, rs.C64)
         reset64 = lambda: b.init_basis(st64, 0)  # noqa: E731
         planned, _, chosen = planner.plan(b, model, mc, 6)
-        formc, form_tq, _, form_chosen = planner.plan_form(b, model, tile_model, mc, rs.C64)
+        formc, form_tq, form_mt, _, form_chosen = planner.plan_form(b, model, tile_model, mc,
+                                                                    rs.C64, mt_model=mt_model)
         rec = {"qubits": nq2, "gates": len(mc), "plain_c128_s": round(plain_s, 4),
                "chosen": "dp" if chosen == 0 else f"greedy_k{chosen}",
                "form_chosen": form_name(form_tq, form_chosen)}
-        for name, circ, tq in (("greedy_k4", rs.fuse(b, mc, 4), 0),
-                               ("greedy_k6", rs.fuse(b, mc, 6), 0), ("learned", planned, 0),
-                               ("learned_form", formc, form_tq)):
-            p = rs.Program(st64, circ, precision=rs.PREC_EXACT if tq else rs.PREC_MFMA_BF16X3,
-                           tile_qubits=tq)
+        for name, circ, tq, mt in (("greedy_k4", rs.fuse(b, mc, 4), 0, 0),
+                                   ("greedy_k6", rs.fuse(b, mc, 6), 0, 0),
+                                   ("learned", planned, 0, 0),
+                                   ("learned_form", formc, form_tq, form_mt)):
+            p = form_program(st64, circ, tq, mt)
             s_ = timed_steps(b, st64, p, 1, 1, reset64)
             reset64()
             p.launch()
@@ -445,13 +463,14 @@ def main():
     st64 = b.alloc(n_layered, rs.C64)
     reset64 = lambda: b.init_basis(st64, 0)  # noqa: E731
     planned, _, chosen = planner.plan(b, model, lay, 6)
-    formc, form_tq, _, form_chosen = planner.plan_form(b, model, tile_model, lay, rs.C64)
+    formc, form_tq, form_mt, _, form_chosen = planner.plan_form(b, model, tile_model, lay,
+                                                                rs.C64, mt_model=mt_model)
     lay_legs = {}
     cands = [("greedy_k%d" % k, rs.fuse(b, lay, k), rs.PREC_MFMA_BF16X3, 0, 0)
              for k in (4, 5, 6)]
     cands += [("learned", planned, rs.PREC_MFMA_BF16X3, 0, 0),
               ("learned_form", formc, rs.PREC_EXACT if form_tq else rs.PREC_MFMA_BF16X3,
-               form_tq, 0),
+               form_tq, form_mt),
               ("tiled_c64_exact", lay, rs.PREC_EXACT, 13, 0)]
     cands += [("greedy_k%d_mfma_tiles12" % k, rs.fuse(b, lay, k), rs.PREC_MFMA_BF16X3, 0, 12)
               for k in (4, 5)]

@@ -346,17 +346,31 @@ int qsvh_plan_fusion(qsvh_ctx* ctx, qsvh_mlp* model, const qsvh_circuit* in,
 #define QSVH_TILE_FEATURES 18
 int qsvh_tile_pass_features(const qsvh_circuit* c, int dtype, uint32_t tile_qubits, float* out,
                             size_t max_passes, size_t* n_passes, size_t* n_other);
+/* Cost-model features of every MFMA tile pass (program option mfma_tiles = 12, split-BF16)
+ * built for the fused circuit `c` (>= 12 qubits): QSVH_MT_FEATURES floats per pass into out
+ * (at most max_passes rows); *n_passes, *n_other as qsvh_tile_pass_features.  Features:
+ * counts of dense groups of 4, 5, 6 qubits (/16), single diagonal ops (/32), merged diagonal
+ * runs (/16) and their members (/64), per-tile table entries of the merged runs and the
+ * pass's phase-table entries (/2048), widest group 5 / 6 flags.  No reference counterpart. */
+#define QSVH_MT_FEATURES 10
+int qsvh_mt_pass_features(const qsvh_circuit* c, float* out, size_t max_passes,
+                          size_t* n_passes, size_t* n_other);
 /* Learned choice of the execution form for `in` on a `dtype` state: the fused plan of
  * qsvh_plan_fusion (group_model, 12 inputs) against tile passes at every allowed tile size
  * (tile_model, QSVH_TILE_FEATURES inputs, trained on log2 of pass milliseconds at 2^30
- * amplitudes; gates outside passes priced by group_model), the cheapest predicted kept.
- * Fused: *out is the fused circuit (run with an MFMA precision), *tile_qubits = 0, *chosen as
- * qsvh_plan_fusion.  Tiled: *out is `in` unchanged (run exact with program option
- * tile_qubits), *tile_qubits > 0, *chosen = -tile_qubits.  Extends the fusion planner
- * (fusion.rs:39-77 is the greedy fallback) to the choice between the two forms. */
+ * amplitudes; gates outside passes priced by group_model) and, when mt_model is given
+ * (QSVH_MT_FEATURES inputs, trained likewise; complex64, >= 12 qubits), against greedy
+ * fusion at max_qubits 4 and 5 run as MFMA tile passes; the cheapest predicted kept.
+ * Fused: *out is the fused circuit (run with an MFMA precision), *tile_qubits = 0,
+ * *mfma_tiles = 0, *chosen as qsvh_plan_fusion.  Tiled: *out is `in` unchanged (run exact
+ * with program option tile_qubits), *tile_qubits > 0, *chosen = -tile_qubits.  MFMA tiles:
+ * *out is the fused circuit (run MFMA_BF16X3 with program option mfma_tiles = *mfma_tiles =
+ * 12), *chosen = 100 + max_qubits.  Extends the fusion planner (fusion.rs:39-77 is the
+ * greedy fallback) to the choice between the forms. */
 int qsvh_plan_form(qsvh_ctx* ctx, qsvh_mlp* group_model, qsvh_mlp* tile_model,
-                   const qsvh_circuit* in, int dtype, uint32_t max_qubits, uint32_t window,
-                   qsvh_circuit** out, int* tile_qubits, double* predicted_ms, int* chosen);
+                   qsvh_mlp* mt_model, const qsvh_circuit* in, int dtype, uint32_t max_qubits,
+                   uint32_t window, qsvh_circuit** out, int* tile_qubits, int* mfma_tiles,
+                   double* predicted_ms, int* chosen);
 
 /* Apply one gate `reps` times back to back and return the mean device time per launch
  * (HIP events on the context stream).  Measurement helper for single-kernel rooflines. */

@@ -737,16 +737,9 @@ Launch widen_to_4(const Launch& l, const uint32_t* extra) {
 // Consecutive ops go into one pass while the qubits of its dense ops (with bits 0..3) fit
 // in tb bits; diagonal ops join any pass (their qubits outside the tile are fixed per tile).
 // Op order is kept, so every pass applies its ops exactly in circuit order.
-int plan_mfma_tiles(Program* p, int tb) {
-  State& st = *p->st;
-  if (p->precision != QSVH_PREC_MFMA_BF16X3)
-    return set_err(st.ctx, kInvalidArg, "mfma_tiles runs split-BF16 groups (MFMA_BF16X3)");
-  if (tb != 12) return set_err(st.ctx, kInvalidArg, "mfma_tiles must be 12");
-  if (st.n < uint32_t(tb)) return plan_mfma(p);  // smaller than one tile
-  const std::vector<Launch> in = std::move(p->launches);
-  p->launches.clear();
-  std::vector<MtOp> ops;
-  std::vector<uint8_t> pool;
+void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPlan* out) {
+  std::vector<MtOp>& ops = out->ops;
+  std::vector<uint8_t>& pool = out->pool;
   std::vector<size_t> cur;
   const uint64_t low = (1ull << kTileLowBits) - 1;
   uint64_t S = low;
@@ -761,13 +754,13 @@ int plan_mfma_tiles(Program* p, int tb) {
   auto close = [&]() {
     if (cur.empty()) return;
     uint64_t tm = S;
-    for (uint32_t b = 0; b < st.n && popcount64(tm) < uint32_t(tb); ++b) tm |= 1ull << b;
+    for (uint32_t b = 0; b < n && popcount64(tm) < tb; ++b) tm |= 1ull << b;
     MtPassArg pa{};
-    pa.tb = uint32_t(tb);
+    pa.tb = tb;
     pa.kmax = 4;
     uint32_t pos[64] = {0};
     uint32_t nb = 0;
-    for (uint32_t b = 0; b < st.n; ++b)
+    for (uint32_t b = 0; b < n; ++b)
       if ((tm >> b) & 1) {
         pa.ts[nb] = b;
         pos[b] = nb++;
@@ -795,7 +788,7 @@ int plan_mfma_tiles(Program* p, int tb) {
         Launch g = l;
         if (l.m < 4) {
           uint32_t extra[4], ne = 0;
-          for (uint32_t b = 0; b < st.n && ne < 4 - l.m; ++b)
+          for (uint32_t b = 0; b < n && ne < 4 - l.m; ++b)
             if (((tm >> b) & 1) && std::find(l.q, l.q + l.m, b) == l.q + l.m) extra[ne++] = b;
           g = widen_to_4(l, extra);
         }
@@ -810,7 +803,7 @@ int plan_mfma_tiles(Program* p, int tb) {
           gm |= 1u << pos[sorted[j]];
         }
         uint32_t o = 0;
-        for (uint32_t b = 0; b < uint32_t(tb); ++b)
+        for (uint32_t b = 0; b < tb; ++b)
           if (!((gm >> b) & 1)) op.set_obit(o++, b);
         std::vector<uint16_t> hl(2 * a.size());
         for (size_t e = 0; e < a.size(); ++e) {
@@ -875,9 +868,9 @@ int plan_mfma_tiles(Program* p, int tb) {
         ops[e].run = 1 + (e + 1 < ops.size() && ops[e + 1].kind != 0 ? ops[e + 1].run : 0);
     Launch L;
     L.kind = Launch::kMfmaTile;
-    L.arena_off = p->mt_passes.size();
-    p->mt_passes.push_back(pa);
-    p->launches.push_back(std::move(L));
+    L.arena_off = out->passes.size();
+    out->passes.push_back(pa);
+    out->launches.push_back(std::move(L));
     cur.clear();
     S = low;
     cur_tab = 0;
@@ -891,15 +884,29 @@ int plan_mfma_tiles(Program* p, int tb) {
     } else if ((l.kind == Launch::k1q || l.kind == Launch::kMq) && l.m <= 6) {
       uint64_t Q = 0;
       for (uint32_t j = 0; j < l.m; ++j) Q |= 1ull << l.q[j];
-      if (popcount64(S | Q) > uint32_t(tb)) close();
+      if (popcount64(S | Q) > tb) close();
       S |= Q;
       cur.push_back(i);
     } else {
       close();
-      p->launches.push_back(l);
+      out->launches.push_back(l);
     }
   }
   close();
+}
+
+int plan_mfma_tiles(Program* p, int tb) {
+  State& st = *p->st;
+  if (p->precision != QSVH_PREC_MFMA_BF16X3)
+    return set_err(st.ctx, kInvalidArg, "mfma_tiles runs split-BF16 groups (MFMA_BF16X3)");
+  if (tb != 12) return set_err(st.ctx, kInvalidArg, "mfma_tiles must be 12");
+  if (st.n < uint32_t(tb)) return plan_mfma(p);  // smaller than one tile
+  MtPlan plan;
+  build_mt_plan(p->launches, st.n, uint32_t(tb), &plan);
+  p->launches = std::move(plan.launches);
+  p->mt_passes = std::move(plan.passes);
+  const std::vector<MtOp>& ops = plan.ops;
+  const std::vector<uint8_t>& pool = plan.pool;
   if (ops.empty()) return kOk;
   auto upload = [&](const void* src, size_t bytes, void** dst) {
     hipError_t e = hipMalloc(dst, bytes ? bytes : 16);

@@ -93,6 +93,45 @@ void tile_schedule_features(const Circuit& c, int dtype, uint32_t tb, std::vecto
     if (l.kind != Launch::kTile) other->push_back(std::move(l));
 }
 
+constexpr int kMtFeatures = 10;
+
+// Feature vector of one MFMA tile pass (complex64, 2^12-amplitude tiles): counts of its
+// dense groups by width (4, 5, 6; /16), single diagonal ops (/32), merged diagonal runs
+// (/16) and their members (/64), the entries of the per-tile tables the merged runs build
+// and of the pass's phase table (/2048), and flags for a 5- or 6-qubit widest group (the
+// kernel's occupancy steps).
+void mt_schedule_features(const Circuit& c, std::vector<float>* rows, std::vector<Launch>* other) {
+  std::vector<Launch> in;
+  in.reserve(c.ops.size());
+  for (const Op& op : c.ops) in.push_back(classify(op, 1));
+  MtPlan plan;
+  build_mt_plan(in, c.n, 12, &plan);
+  rows->assign(plan.passes.size() * kMtFeatures, 0.f);
+  for (size_t i = 0; i < plan.passes.size(); ++i) {
+    const MtPassArg& pa = plan.passes[i];
+    float* f = rows->data() + i * kMtFeatures;
+    for (uint32_t j = pa.op_begin; j < pa.op_begin + pa.n_ops; ++j) {
+      const MtOp& o = plan.ops[j];
+      if (o.kind == 0) {
+        f[o.k - 4] += 1.f / 16;
+      } else if (o.kind == 1) {
+        f[3] += 1.f / 32;
+      } else if (o.kind == 2) {
+        f[4] += 1.f / 16;
+        f[6] += float(1u << o.k) / 2048;
+      } else {
+        f[5] += 1.f / 64;
+      }
+    }
+    f[7] = float(pa.tab_entries) / 2048;
+    f[8] = pa.kmax == 5 ? 1.f : 0.f;
+    f[9] = pa.kmax == 6 ? 1.f : 0.f;
+  }
+  other->clear();
+  for (Launch& l : plan.launches)
+    if (l.kind != Launch::kMfmaTile) other->push_back(std::move(l));
+}
+
 // Feature vector of a group: k qubits (ascending), diagonal flag, density of the composed
 // matrix (nonzeros / 4^k) and the state size n.
 void group_features(uint32_t n, uint32_t k, const uint32_t* q, bool diag, double density,
@@ -346,9 +385,9 @@ int plan_fusion(Ctx* ctx, Mlp& mlp, const Circuit& c, uint32_t max_qubits, uint3
 // against tile passes at every allowed tile size, each priced by its cost model -- pass
 // times by the tile model, launches a tile cannot hold by the group model -- keeping the
 // lowest predicted total.  Tiled: *out is the circuit unchanged and *tile_qubits > 0.
-int plan_form(Ctx* ctx, Mlp& group_mlp, Mlp& tile_mlp, const Circuit& c, int dtype,
-              uint32_t max_qubits, uint32_t window, Circuit* out, int* tile_qubits,
-              double* predicted_ms, int* chosen) {
+int plan_form(Ctx* ctx, Mlp& group_mlp, Mlp& tile_mlp, Mlp* mt_mlp, const Circuit& c,
+              int dtype, uint32_t max_qubits, uint32_t window, Circuit* out, int* tile_qubits,
+              int* mfma_tiles, double* predicted_ms, int* chosen) {
   double fused_ms = 0;
   int fused_pick = 0;
   int rc = plan_fusion(ctx, group_mlp, c, max_qubits, window, out, &fused_ms, &fused_pick);
@@ -386,10 +425,56 @@ int plan_form(Ctx* ctx, Mlp& group_mlp, Mlp& tile_mlp, const Circuit& c, int dty
       best_tb = int(tb);
     }
   }
-  if (best_tb > 0) *out = c;
+  // greedy fusion at k = 4, 5 run as MFMA tile passes (complex64): passes priced by the
+  // MFMA-tile model, launches no pass holds by the group model
+  int best_mt = 0;
+  Circuit mt_best;
+  if (mt_mlp && dtype == 1 && c.n >= 12) {
+    for (uint32_t k = 4; k <= std::min<uint32_t>(5, max_qubits); ++k) {
+      std::vector<std::pair<uint32_t, uint32_t>> parts;
+      for (const Group& g : greedy_groups(c, k))
+        parts.emplace_back(uint32_t(g.begin), uint32_t(g.end));
+      Circuit fused;
+      rc = build_partition(ctx, c, parts, &fused);
+      if (rc != kOk) return rc;
+      std::vector<float> rows, y;
+      std::vector<Launch> other;
+      mt_schedule_features(fused, &rows, &other);
+      const size_t np = rows.size() / kMtFeatures;
+      y.resize(np);
+      if (np) {
+        rc = mt_mlp->predict(rows.data(), np, y.data(), false);
+        if (rc != kOk) return rc;
+      }
+      double t = 0;
+      for (size_t i = 0; i < np; ++i) t += std::exp2(double(y[i])) * std::exp2(double(c.n) - 30.0);
+      std::vector<Candidate> cands;
+      for (const Launch& l : other) {
+        Candidate cd;
+        uint32_t q[kMaxGateQubits];
+        std::copy(l.q, l.q + l.m, q);
+        std::sort(q, q + l.m);
+        group_features(c.n, l.m, q, l.kind == Launch::kDiag, 1.0, cd.f);
+        cands.push_back(cd);
+      }
+      std::vector<double> oc;
+      rc = predict_ms(group_mlp, c.n, cands, &oc);
+      if (rc != kOk) return rc;
+      for (double x : oc) t += x;
+      if (t < best) {
+        best = t;
+        best_mt = int(k);
+        best_tb = 0;
+        mt_best = std::move(fused);
+      }
+    }
+  }
+  if (best_mt > 0) *out = std::move(mt_best);
+  else if (best_tb > 0) *out = c;
   if (tile_qubits) *tile_qubits = best_tb;
+  if (mfma_tiles) *mfma_tiles = best_mt > 0 ? 12 : 0;
   if (predicted_ms) *predicted_ms = best;
-  if (chosen) *chosen = best_tb > 0 ? -best_tb : fused_pick;
+  if (chosen) *chosen = best_mt > 0 ? 100 + best_mt : best_tb > 0 ? -best_tb : fused_pick;
   return kOk;
 }
 
@@ -443,9 +528,23 @@ int qsvh_tile_pass_features(const qsvh_circuit* c, int dtype, uint32_t tile_qubi
   return kOk;
 }
 
+int qsvh_mt_pass_features(const qsvh_circuit* c, float* out, size_t max_passes,
+                          size_t* n_passes, size_t* n_other) {
+  if (!c || (max_passes && !out) || c->c.n < 12) return kInvalidArg;
+  std::vector<float> rows;
+  std::vector<Launch> other;
+  mt_schedule_features(c->c, &rows, &other);
+  const size_t np = rows.size() / kMtFeatures;
+  std::copy(rows.begin(), rows.begin() + std::min(np, max_passes) * kMtFeatures, out);
+  if (n_passes) *n_passes = np;
+  if (n_other) *n_other = other.size();
+  return kOk;
+}
+
 int qsvh_plan_form(qsvh_ctx* h, qsvh_mlp* group_model, qsvh_mlp* tile_model,
-                   const qsvh_circuit* in, int dtype, uint32_t max_qubits, uint32_t window,
-                   qsvh_circuit** out, int* tile_qubits, double* predicted_ms, int* chosen) {
+                   qsvh_mlp* mt_model, const qsvh_circuit* in, int dtype, uint32_t max_qubits,
+                   uint32_t window, qsvh_circuit** out, int* tile_qubits, int* mfma_tiles,
+                   double* predicted_ms, int* chosen) {
   if (!h || !group_model || !tile_model || !in || !out || (dtype != 0 && dtype != 1))
     return kInvalidArg;
   *out = nullptr;
@@ -454,10 +553,13 @@ int qsvh_plan_form(qsvh_ctx* h, qsvh_mlp* group_model, qsvh_mlp* tile_model,
   if (group_model->m.F != kFeatures) return set_err(&h->c, kInvalidArg, "plan: group model has 12 inputs");
   if (tile_model->m.F != kTileFeatures)
     return set_err(&h->c, kInvalidArg, "plan: tile model has 18 inputs");
+  if (mt_model && mt_model->m.F != kMtFeatures)
+    return set_err(&h->c, kInvalidArg, "plan: MFMA-tile model has 10 inputs");
   qsvh_circuit* res = new (std::nothrow) qsvh_circuit;
   if (!res) return kOom;
-  const int rc = plan_form(&h->c, group_model->m, tile_model->m, in->c, dtype, max_qubits,
-                           window ? window : 48, &res->c, tile_qubits, predicted_ms, chosen);
+  const int rc = plan_form(&h->c, group_model->m, tile_model->m, mt_model ? &mt_model->m : nullptr,
+                           in->c, dtype, max_qubits, window ? window : 48, &res->c, tile_qubits,
+                           mfma_tiles, predicted_ms, chosen);
   if (rc != kOk) {
     delete res;
     return rc;

@@ -78,6 +78,15 @@ void plan_subspace(Program* p);
 int plan_mfma(Program* p);
 // Runs of fused groups as MFMA tile passes of 2^tb amplitudes (split-BF16 precision).
 int plan_mfma_tiles(Program* p, int tb);
+// The host side of plan_mfma_tiles: the passes, their ops and matrix pool, and the launch
+// sequence (kMfmaTile launches and the launches no pass holds).
+struct MtPlan {
+  std::vector<MtPassArg> passes;
+  std::vector<MtOp> ops;
+  std::vector<uint8_t> pool;
+  std::vector<Launch> launches;
+};
+void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPlan* out);
 void release_extra(Program* p);
 
 }  // namespace qsvh

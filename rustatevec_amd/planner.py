@@ -19,6 +19,7 @@ from . import (C64, PREC_MFMA_BF16X3, Circuit, HipBackend, Program, QsvhError, _
 
 N_FEATURES = 12
 N_TILE_FEATURES = 18
+N_MT_FEATURES = 10
 HIDDEN = 64
 
 
@@ -43,8 +44,11 @@ def _bind():
                                    C.POINTER(C.c_int)]),
         "qsvh_tile_pass_features": (i32, [P, i32, u32, P, C.c_size_t,
                                           C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
-        "qsvh_plan_form": (i32, [P, P, P, P, i32, u32, u32, PP, C.POINTER(C.c_int),
-                                 C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "qsvh_mt_pass_features": (i32, [P, P, C.c_size_t, C.POINTER(C.c_size_t),
+                                        C.POINTER(C.c_size_t)]),
+        "qsvh_plan_form": (i32, [P, P, P, P, P, i32, u32, u32, PP, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -273,20 +277,109 @@ def train_tile_model(backend: HipBackend, X, y, epochs=1500, lr=3e-3, seed=2):
     return m, loss
 
 
+def mt_pass_features(circuit: Circuit):
+    """(rows, n_other): the cost-model features of every MFMA tile pass (2^12-amplitude
+    tiles) built for the fused `circuit`, one row per pass, and the launches outside them."""
+    L = _bind()
+    n_p, n_o = C.c_size_t(0), C.c_size_t(0)
+    _check(L.qsvh_mt_pass_features(circuit.handle, None, 0, C.byref(n_p), C.byref(n_o)))
+    out = np.zeros((n_p.value, N_MT_FEATURES), dtype=np.float32)
+    if n_p.value:
+        _check(L.qsvh_mt_pass_features(circuit.handle, _ptr(out), n_p.value, C.byref(n_p),
+                                       C.byref(n_o)))
+    return out, n_o.value
+
+
+def _unitary(rng, d):
+    q, r = np.linalg.qr(rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def _random_mt_pass(rng, n):
+    """A random fused circuit that runs as one MFMA tile pass: 0 to 10 dense groups of 4..6
+    qubits (and a few 1..3-qubit gates) inside one random tile set (qubits 0..3 and 8 more),
+    0 to 80 diagonal groups of 1..6 qubits anywhere (phases within the pass's 2048-entry
+    table), in random order and with a random mix per pass."""
+    tile = [0, 1, 2, 3] + sorted(int(x) for x in rng.choice(np.arange(4, n), size=8,
+                                                           replace=False))
+    n_dense = int(rng.integers(0, 11))
+    n_small = int(rng.integers(0, 4))
+    n_diag = int(rng.integers(0 if n_dense + n_small else 1, 81))
+    wk = rng.dirichlet(np.ones(3))
+    wm = rng.dirichlet(np.ones(6))
+    ops, entries = [], 0
+    for _ in range(n_dense):
+        k = 4 + int(rng.choice(3, p=wk))
+        ops.append(("d", k))
+    for _ in range(n_small):
+        ops.append(("d", int(rng.integers(1, 4))))
+    for _ in range(n_diag):
+        m = 1 + int(rng.choice(6, p=wm))
+        if entries + (1 << m) > 2048:
+            break
+        entries += 1 << m
+        ops.append(("g", m))
+    rng.shuffle(ops)
+    c = Circuit(n)
+    for kind, k in ops:
+        if kind == "d":
+            qs = [int(x) for x in rng.choice(tile, size=k, replace=False)]
+            c.push(_unitary(rng, 1 << k), qs)
+        else:
+            qs = [int(x) for x in rng.choice(n, size=k, replace=False)]
+            c.push(np.diag(np.exp(1j * rng.uniform(0, 2 * np.pi, size=1 << k))), qs)
+    return c
+
+
+def collect_mt_timings(backend: HipBackend, sizes=(26,), per_size=300, seed=11, reps=3):
+    """Time random single MFMA tile passes (split-BF16, 2^12-amplitude tiles) on
+    HBM-resident complex64 states; returns pass features X and y = log2 of the device
+    milliseconds scaled to 2^30 amplitudes."""
+    rng = np.random.default_rng(seed)
+    X, y = [], []
+    for n in sizes:
+        st = backend.alloc(n, C64)
+        backend.init_random_normal(st, seed)
+        for _ in range(per_size):
+            c = _random_mt_pass(rng, n)
+            rows, other = mt_pass_features(c)
+            if len(rows) != 1 or other:
+                continue
+            p = Program(st, c, precision=PREC_MFMA_BF16X3, mfma_tiles=12)
+            p.launch()
+            ms = min(p.time() for _ in range(reps))
+            X.append(rows[0])
+            y.append(np.log2(ms * 2.0 ** (30 - n)))
+            del p
+        st.free()
+    return np.array(X, dtype=np.float32), np.array(y, dtype=np.float32)
+
+
+def train_mt_model(backend: HipBackend, X, y, epochs=2000, lr=3e-3, seed=3):
+    m = Mlp(backend, n_features=N_MT_FEATURES, seed=seed)
+    loss = m.train(X, y, epochs=epochs, lr=lr)
+    return m, loss
+
+
 def plan_form(backend: HipBackend, group_model: Mlp, tile_model: Mlp, circuit: Circuit,
-              dtype=C64, max_qubits=6, window=48):
-    """(circuit, tile_qubits, predicted ms, chosen): the fused plan (tile_qubits 0; run it
-    with an MFMA precision) or the circuit as tile passes of tile_qubits bits (run exact),
-    whichever the cost models predict faster; chosen as in ``plan``, or -tile_qubits."""
+              dtype=C64, max_qubits=6, window=48, mt_model: Mlp | None = None):
+    """(circuit, tile_qubits, mfma_tiles, predicted ms, chosen): the fused plan (run it with
+    an MFMA precision), the circuit as exact tile passes of tile_qubits bits, or (with
+    mt_model, complex64) greedy fusion at 4 or 5 qubits as MFMA tile passes (run it
+    MFMA_BF16X3 with mfma_tiles), whichever the cost models predict fastest; chosen as in
+    ``plan``, -tile_qubits, or 100 + the fusion width."""
     h = C.c_void_p()
-    tq, chosen = C.c_int(0), C.c_int(0)
+    tq, mt, chosen = C.c_int(0), C.c_int(0), C.c_int(0)
     ms = C.c_double(0)
     _check(_bind().qsvh_plan_form(backend.ctx, group_model.handle, tile_model.handle,
+                                  mt_model.handle if mt_model is not None else None,
                                   circuit.handle, dtype, max_qubits, window, C.byref(h),
-                                  C.byref(tq), C.byref(ms), C.byref(chosen)), backend.ctx)
-    return Circuit(_handle=h), tq.value, ms.value, chosen.value
+                                  C.byref(tq), C.byref(mt), C.byref(ms), C.byref(chosen)),
+           backend.ctx)
+    return Circuit(_handle=h), tq.value, mt.value, ms.value, chosen.value
 
 
 __all__ = ["Mlp", "group_features", "collect_timings", "train_cost_model", "plan",
            "tile_pass_features", "collect_tile_timings", "train_tile_model", "plan_form",
+           "N_MT_FEATURES", "mt_pass_features", "collect_mt_timings", "train_mt_model",
            "QsvhError"]

@@ -127,7 +127,9 @@ def test_form_planner_picks_the_faster_form(backend, cost_model, tile_model):
         t = min(p.time() for _ in range(3))
         return t, err, fid
 
-    planned, tq, pred, chosen = planner.plan_form(backend, cost_model, tile_model, circ, rs.C64)
+    planned, tq, mt, pred, chosen = planner.plan_form(backend, cost_model, tile_model, circ,
+                                                      rs.C64)
+    assert mt == 0  # no MFMA-tile model given
     t_plan, err, fid = run(planned, tq)
     assert err < 1e-5 and 1 - fid < 1e-6, (err, fid)
     assert (tq == 0) == (chosen >= 0) and (tq == 0 or chosen == -tq)
@@ -137,5 +139,50 @@ def test_form_planner_picks_the_faster_form(backend, cost_model, tile_model):
     print(f"form plan: tile_qubits {tq}, {t_plan:.3f} ms (predicted {pred:.3f}); greedy "
           f"{greedy}; tiled {tiled}")
     assert t_plan <= 1.15 * best
+    st.free()
+    ref.free()
+
+
+@pytest.fixture(scope="module")
+def mt_model(backend):
+    X, y = planner.collect_mt_timings(backend, sizes=(24,), per_size=160, seed=13)
+    m, loss = planner.train_mt_model(backend, X, y, epochs=1500)
+    print(f"MFMA-tile model: {len(X)} pass timings, final standardised MSE {loss:.4f}")
+    assert len(X) >= 150 and loss < 0.2
+    return m
+
+
+def test_form_planner_with_mfma_tiles(backend, cost_model, tile_model, mt_model):
+    """With the MFMA-tile model, plan_form also prices greedy fusion at 4 and 5 qubits as
+    MFMA tile passes; on a layered circuit (where those passes are the fastest form measured)
+    its choice runs correctly (fused tolerance against complex128) and within 15% of the
+    fastest of the fixed forms: exact tiles, MFMA tiles at k = 4, 5, fused groups at 4, 6."""
+    n = 24
+    circ = rs.layered(n, 20, 5)
+    ref = backend.execute(circ, rs.C128)
+    st = backend.alloc(n, rs.C64)
+
+    def run(c, tq=0, mt=0):
+        p = rs.Program(st, c, precision=rs.PREC_EXACT if tq else rs.PREC_MFMA_BF16X3,
+                       tile_qubits=tq, mfma_tiles=mt)
+        backend.init_basis(st, 0)
+        p.launch()
+        err, fid = backend.compare(ref, st)
+        t = min(p.time() for _ in range(3))
+        return t, err, fid
+
+    planned, tq, mt, pred, chosen = planner.plan_form(backend, cost_model, tile_model, circ,
+                                                      rs.C64, mt_model=mt_model)
+    assert (mt == 12) == (chosen >= 100) and (mt == 0 or tq == 0)
+    t_plan, err, fid = run(planned, tq, mt)
+    assert err < 1e-5 and 1 - fid < 1e-6, (err, fid)
+    forms = {"tiles13": run(circ, 13)[0], "tiles11": run(circ, 11)[0]}
+    for k in (4, 5):
+        forms[f"mt_k{k}"] = run(rs.fuse(backend, circ, k), mt=12)[0]
+    for k in (4, 6):
+        forms[f"fused_k{k}"] = run(rs.fuse(backend, circ, k))[0]
+    print(f"form plan with MFMA tiles: chosen {chosen}, {t_plan:.3f} ms (predicted "
+          f"{pred:.3f}); fixed forms {forms}")
+    assert t_plan <= 1.15 * min(forms.values())
     st.free()
     ref.free()

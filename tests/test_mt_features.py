@@ -1,0 +1,62 @@
+"""MFMA tile-pass cost-model features (qsvh_mt_pass_features) on the CPU: fused circuits
+whose pass split and op classification are known, so every count is pinned (a miscounted or
+misclassified op, a missed merge or a wrong table size moves one of them)."""
+import numpy as np
+import pytest
+
+import rustatevec_amd as rs
+from rustatevec_amd import planner
+
+
+def dense(rng, k):
+    q, _ = np.linalg.qr(rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k)))
+    return q
+
+
+def diag(rng, m):
+    return np.diag(np.exp(1j * rng.uniform(0, 2 * np.pi, size=1 << m)))
+
+
+def test_one_pass_counts_dense_merged_and_table_entries():
+    # n = 14: the dense groups' qubits {0..3} and {4..8} with qubits 0..3 fill 9 of the 12
+    # tile bits; the tile is qubits 0..11.  The two diagonal groups between them are
+    # consecutive with in-tile qubits {5, 4} (qubit 13 is outside): one merged run (a head
+    # over 2 tile bits, 2 members).  Phase table: 4 + 2 entries.
+    rng = np.random.default_rng(1)
+    c = rs.Circuit(14)
+    c.push(dense(rng, 4), [0, 1, 2, 3])
+    c.push(diag(rng, 2), [5, 13])
+    c.push(diag(rng, 1), [4])
+    c.push(dense(rng, 5), [4, 5, 6, 7, 8])
+    rows, other = planner.mt_pass_features(c)
+    assert other == 0 and rows.shape == (1, planner.N_MT_FEATURES)
+    want = np.zeros(planner.N_MT_FEATURES, dtype=np.float32)
+    want[0] = 1 / 16          # dense 4-qubit
+    want[1] = 1 / 16          # dense 5-qubit
+    want[4] = 1 / 16          # merged run heads
+    want[5] = 2 / 64          # their members
+    want[6] = 4 / 2048        # per-tile table of the head (2 tile bits)
+    want[7] = 6 / 2048        # the pass's phase table
+    want[8] = 1.0             # widest group: 5 qubits
+    np.testing.assert_allclose(rows[0], want, rtol=1e-6)
+
+
+def test_passes_split_at_the_tile_and_small_gates_widen():
+    # n = 20: a dense group on {12..17} cannot join the first pass (tile {0..11} after
+    # {4..11}): two passes.  A lone diagonal between two dense groups stays a single op;
+    # a 2-qubit dense gate counts as a widened 4-qubit group.
+    rng = np.random.default_rng(2)
+    c = rs.Circuit(20)
+    c.push(dense(rng, 6), [4, 5, 6, 7, 8, 9])
+    c.push(diag(rng, 3), [0, 15, 19])
+    c.push(dense(rng, 2), [10, 11])
+    c.push(dense(rng, 6), [12, 13, 14, 15, 16, 17])
+    c.push(diag(rng, 6), [0, 1, 2, 3, 18, 19])
+    rows, other = planner.mt_pass_features(c)
+    assert other == 0 and rows.shape[0] == 2
+    first, second = rows
+    assert first[2] == pytest.approx(1 / 16) and first[0] == pytest.approx(1 / 16)
+    assert first[3] == pytest.approx(1 / 32) and first[4] == 0
+    assert first[7] == pytest.approx(8 / 2048) and first[9] == 1.0
+    assert second[2] == pytest.approx(1 / 16) and second[3] == pytest.approx(1 / 32)
+    assert second[7] == pytest.approx(64 / 2048)
