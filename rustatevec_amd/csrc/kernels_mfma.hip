@@ -235,6 +235,17 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
   lo = static_cast<__bf16>(x - static_cast<float>(hi));
 }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+// split_bf16 of two values at once (one packed conversion each for hi and lo): the same
+// roundings, so the same bits.
+__device__ __forceinline__ void split_bf16x2(float a, float b, bf16x2& hi, bf16x2& lo) {
+  const floatx2 x = {a, b};
+  hi = __builtin_convertvector(x, bf16x2);
+  lo = __builtin_convertvector(x - __builtin_convertvector(hi, floatx2), bf16x2);
+}
+
 template <int K, bool NTA>
 __global__ __launch_bounds__(kBlock) void k_fused_bf16x3(float* __restrict__ amp,
                                                          uint64_t n_tiles, TileMap map,
@@ -501,18 +512,22 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
     ah[ks] = *reinterpret_cast<const bf16x8*>(ahi + idx);
     al[ks] = *reinterpret_cast<const bf16x8*>(alo + idx);
   }
+  // LDS byte offsets (4 x the words: the XOR structure carries over)
   uint32_t U[K];
 #pragma unroll
-  for (int i = 0; i < K; ++i) U[i] = swz(1u << op_byte(op.bits, i));
+  for (int i = 0; i < K; ++i) U[i] = 4u * swz(1u << op_byte(op.bits, i));
   uint32_t wl = 0;  // the lane's column l over the other bits
 #pragma unroll
   for (int i = 0; i < 5; ++i) wl |= uint32_t((l >> i) & 1) << op_byte(op.obits, i);
-  wl = swz(wl);
+  wl = 4u * swz(wl);
   auto colw = [&](int nt) {  // block nt's first column over the other bits (uniform)
     uint32_t u = 0;
 #pragma unroll
     for (int i = 5; i < TB - K; ++i) u |= uint32_t((nt >> (i - 5)) & 1) << op_byte(op.obits, i);
-    return swz(u);
+    return 4u * swz(u);
+  };
+  auto at = [](const float* base, uint32_t byte) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte);
   };
 
   floatx16 acc[BPW];
@@ -526,15 +541,18 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
       // opaque per step: the step's read addresses are formed here, not all held at once
       asm volatile("" : "+v"(wb));
       const float* src = ((16 * ks) >> K) ? xi : xr;  // block rows >= D: the imaginary part
-      bf16x8 bh, bl;
+      float x[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const float x = src[wb ^ uxor<K>(U, uint32_t((16 * ks + j) & (D - 1)))];
-        __bf16 hb, lb;
-        split_bf16(x, hb, lb);
-        bh[j] = hb;
-        bl[j] = lb;
-      }
+      for (int j = 0; j < 8; ++j) x[j] = at(src, wb ^ uxor<K>(U, uint32_t((16 * ks + j) & (D - 1))));
+      bf16x2 h[4], lo[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) split_bf16x2(x[2 * j], x[2 * j + 1], h[j], lo[j]);
+      const bf16x8 bh = __builtin_shufflevector(
+          __builtin_shufflevector(h[0], h[1], 0, 1, 2, 3),
+          __builtin_shufflevector(h[2], h[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
+      const bf16x8 bl = __builtin_shufflevector(
+          __builtin_shufflevector(lo[0], lo[1], 0, 1, 2, 3),
+          __builtin_shufflevector(lo[2], lo[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
       acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bh, acc[bi], 0, 0, 0);
       acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bl, acc[bi], 0, 0, 0);
       acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[ks], bh, acc[bi], 0, 0, 0);
@@ -552,7 +570,8 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
       if ((r & 3) == 0) asm volatile("" : "+v"(wb));
       const uint32_t rowu = uint32_t(32 * mt + 8 * (r >> 2) + (r & 3));
       float* dst = (rowu >> K) ? xi : xr;
-      dst[wb ^ uxor<K>(U, rowu & (D - 1))] = acc[bi][r];
+      *reinterpret_cast<float*>(reinterpret_cast<char*>(dst) + (wb ^ uxor<K>(U, rowu & (D - 1)))) =
+          acc[bi][r];
     }
   }
   __syncthreads();
