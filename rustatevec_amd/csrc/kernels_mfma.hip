@@ -531,36 +531,44 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
   };
 
   floatx16 acc[BPW];
+  uint32_t wb[BPW];  // B rows 16 ks + 8 h + j of block bi
 #pragma unroll
   for (int bi = 0; bi < BPW; ++bi) {
     acc[bi] = floatx16{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
                        0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    uint32_t wb = wl ^ colw(nt0 + bi) ^ (h ? U[3] : 0u);  // B rows 16 ks + 8 h + j
-#pragma unroll
-    for (int ks = 0; ks < KSTEPS; ++ks) {
-      // opaque per step: the step's read addresses are formed here, not all held at once
-      asm volatile("" : "+v"(wb));
-      const float* src = ((16 * ks) >> K) ? xi : xr;  // block rows >= D: the imaginary part
-      float x[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) x[j] = at(src, wb ^ uxor<K>(U, uint32_t((16 * ks + j) & (D - 1))));
-      bf16x2 h[4], lo[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) split_bf16x2(x[2 * j], x[2 * j + 1], h[j], lo[j]);
-      const bf16x8 bh = __builtin_shufflevector(
-          __builtin_shufflevector(h[0], h[1], 0, 1, 2, 3),
-          __builtin_shufflevector(h[2], h[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
-      const bf16x8 bl = __builtin_shufflevector(
-          __builtin_shufflevector(lo[0], lo[1], 0, 1, 2, 3),
-          __builtin_shufflevector(lo[2], lo[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
-      acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bh, acc[bi], 0, 0, 0);
-      acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bl, acc[bi], 0, 0, 0);
-      acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[ks], bh, acc[bi], 0, 0, 0);
-      // the next step's LDS reads are issued behind these products: hoisted, all of a
-      // block's reads would be live at once
-      asm volatile("" ::: "memory");
-    }
+    wb[bi] = wl ^ colw(nt0 + bi) ^ (h ? U[3] : 0u);
   }
+  // one K step of block bi: its B fragment read from LDS, split, three products
+  auto step = [&](int bi, int ks) {
+    // opaque per step: the step's read addresses are formed here, not all held at once
+    asm volatile("" : "+v"(wb[bi]));
+    const float* src = ((16 * ks) >> K) ? xi : xr;  // block rows >= D: the imaginary part
+    float x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      x[j] = at(src, wb[bi] ^ uxor<K>(U, uint32_t((16 * ks + j) & (D - 1))));
+    bf16x2 hh[4], lo[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) split_bf16x2(x[2 * j], x[2 * j + 1], hh[j], lo[j]);
+    const bf16x8 bh = __builtin_shufflevector(
+        __builtin_shufflevector(hh[0], hh[1], 0, 1, 2, 3),
+        __builtin_shufflevector(hh[2], hh[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
+    const bf16x8 bl = __builtin_shufflevector(
+        __builtin_shufflevector(lo[0], lo[1], 0, 1, 2, 3),
+        __builtin_shufflevector(lo[2], lo[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
+    acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bh, acc[bi], 0, 0, 0);
+    acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bl, acc[bi], 0, 0, 0);
+    acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[ks], bh, acc[bi], 0, 0, 0);
+    // the next step's LDS reads are issued behind these products: hoisted, all of a
+    // block's reads would be live at once
+    asm volatile("" ::: "memory");
+  };
+  // block by block (alternating the blocks' steps, to overlap their chains of dependent
+  // products, measured no faster)
+#pragma unroll
+  for (int bi = 0; bi < BPW; ++bi)
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ++ks) step(bi, ks);
   __syncthreads();
 #pragma unroll
   for (int bi = 0; bi < BPW; ++bi) {
