@@ -167,3 +167,29 @@ def test_smaller_than_a_tile_and_argument_errors(backend):
     with pytest.raises(rs.QsvhError):
         rs.Program(st, c, precision=rs.PREC_MFMA_F32, mfma_tiles=12)
     st.free()
+
+
+def test_graph_replays_match_the_eager_run(backend):
+    # a program of MFMA tile passes captured in a HIP graph: every replay (with eager work
+    # and synchronisations between the replays) gives the eager launch's amplitudes
+    n = 16
+    rng = np.random.default_rng(21)
+    psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+    psi /= np.linalg.norm(psi)
+    ops = random_ops(n, 20, rng)
+    want, _ = run(backend, n, ops, psi, mfma_tiles=12)
+    c = rs.Circuit(n)
+    for U, qs in ops:
+        c.push(U, qs)
+    st = backend.alloc(n, rs.C64)
+    other = backend.alloc(n, rs.C64)
+    p = rs.Program(st, c, precision=rs.PREC_MFMA_BF16X3, mfma_tiles=12, use_graph=True)
+    for _ in range(4):
+        backend.upload(st, psi)
+        p.launch()
+        backend.init_random_normal(other, 3)  # eager work between the replays
+        assert abs(backend.norm_sqr(other) - 1) < 1e-5
+        assert np.array_equal(backend.download(st), want)
+    del p
+    st.free()
+    other.free()
