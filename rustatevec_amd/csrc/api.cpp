@@ -689,15 +689,18 @@ int qsvh_sample(qsvh_state* st, uint64_t shots, uint64_t seed, uint64_t* out) {
   if (shots > (uint64_t(1) << 31)) return set_err(s.ctx, kInvalidArg, "at most 2^31 shots per call");
   const uint64_t nc = sample_chunks(s.dim());
   if (nc > (uint64_t(1) << 23)) return set_err(s.ctx, kInvalidArg, "sampling supports up to 34 qubits");
-  const size_t bytes = sizeof(double) * (s.dim() + 2 * nc + 1) + sizeof(uint64_t) * shots;
+  const uint64_t nb = sample_blocks(s.dim());
+  const size_t bytes =
+      sizeof(double) * (s.dim() + 2 * nc + 1 + nb) + sizeof(uint64_t) * shots;
   void* buf = nullptr;
   hipError_t e = hipMalloc(&buf, bytes);
   if (e != hipSuccess) return hip_err(s.ctx, e, "sample alloc");
   double* local = static_cast<double*>(buf);
   double* totals = local + s.dim();
   double* offset = totals + nc;
-  uint64_t* dout = reinterpret_cast<uint64_t*>(offset + nc + 1);
-  e = launch_sample(s, shots, seed, local, totals, offset, dout, s.ctx->stream);
+  double* bsum = offset + nc + 1;
+  uint64_t* dout = reinterpret_cast<uint64_t*>(bsum + nb);
+  e = launch_sample(s, shots, seed, local, totals, offset, bsum, dout, s.ctx->stream);
   if (e == hipSuccess && shots > 0)
     e = hipMemcpyAsync(out, dout, sizeof(uint64_t) * shots, hipMemcpyDeviceToHost,
                        s.ctx->stream);

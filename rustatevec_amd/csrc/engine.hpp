@@ -257,10 +257,13 @@ hipError_t launch_batch_reduce(const uint32_t* part_off, const uint32_t* n_part,
 
 // ---- kernels_sample.hip ----------------------------------------------------------------
 // Inverse-CDF sampling of `shots` outcomes with SplitMix64(seed) draws.  Buffers: local
-// (dim doubles), totals (sample_chunks(dim)), offset (sample_chunks(dim) + 1), out (shots).
+// (dim doubles), totals (sample_chunks(dim)), offset (sample_chunks(dim) + 1), bsum
+// (sample_blocks(dim)), out (shots).
 uint64_t sample_chunks(uint64_t dim);
+uint64_t sample_blocks(uint64_t dim);
 hipError_t launch_sample(const State& st, uint64_t shots, uint64_t seed, double* local,
-                         double* totals, double* offset, uint64_t* out, hipStream_t s);
+                         double* totals, double* offset, double* bsum, uint64_t* out,
+                         hipStream_t s);
 
 // ---- kernels_util.hip ------------------------------------------------------------------
 // Random normalised start state (counter-based SplitMix64 + Box-Muller); `d_norm` receives

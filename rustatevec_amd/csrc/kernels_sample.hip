@@ -5,9 +5,11 @@
 //  1. k_scan_local: p_i = |psi_i|^2 (formed in the state's precision, then widened to f64 as
 //     StateVector::probabilities + to_f64 do), inclusive f64 scan within chunks of 2048;
 //     chunk totals aside.
-//  2. k_scan_offsets: one lane adds the chunk totals in order: offset[c + 1] = offset[c] +
-//     total[c], so the global CDF cdf_i = offset[c(i)] + local_i is non-decreasing and its
-//     last value of chunk c is exactly offset[c + 1].
+//  2. k_scan_totals, k_scan_blocks, k_add_blocks: offset[c + 1] = total[0] + ... + total[c]
+//     by a three-level scan (2048 totals per workgroup, then the workgroup sums, then the
+//     sums added back), so the global CDF cdf_i = offset[c(i)] + local_i is non-decreasing
+//     within each chunk and across chunk boundaries to within an ulp (the last value of
+//     chunk c, offset[c] + total[c], and offset[c + 1] are the same sum in two orders).
 //  3. k_draw: one lane per shot: u = unit_k * cdf_last with unit_k the k-th unit() of
 //     SplitMix64(seed) (counter form: output k = mix(seed + (k + 1) gamma)), outcome =
 //     min(first i with cdf_i >= u, dim - 1) -- partition_point(|c| c < u) -- by binary search
@@ -91,16 +93,83 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_local(const T* __restrict
   }
 }
 
-// offset[0] = 0, offset[c + 1] = offset[c] + totals[c] (nc + 1 entries), in chunk order.
-__global__ void k_scan_offsets(const double* __restrict__ totals, uint64_t nc,
-                               double* __restrict__ offset) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// Inclusive scan over a workgroup of kScanThreads threads, thread t holding PER values in
+// order (elements t PER .. t PER + PER - 1): v becomes the inclusive prefixes and *start the
+// sum before the thread's first element; returns the workgroup total.  wsum:
+// kScanThreads / 64 doubles of LDS.
+template <int PER>
+__device__ __forceinline__ double block_scan(double (&v)[PER], double* wsum, double* start) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double acc = 0.0;
-  offset[0] = 0.0;
-  for (uint64_t c = 0; c < nc; ++c) {
-    acc = acc + totals[c];
-    offset[c + 1] = acc;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    acc += v[j];
+    v[j] = acc;
   }
+  double x = acc;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  double ex = __shfl_up(x, 1, 64);
+  if (lane == 0) ex = 0.0;
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  double woff = 0.0, total = 0.0;
+  for (int w = 0; w < kScanThreads / 64; ++w) {
+    if (w < wave) woff += wsum[w];
+    total += wsum[w];
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) v[j] += woff + ex;
+  *start = woff + ex;
+  __syncthreads();
+  return total;
+}
+
+// Level 1: offset[c + 1] = the inclusive scan of totals within each run of kScanChunk
+// chunks; bsum[b] = the run's sum.
+__global__ __launch_bounds__(kScanThreads) void k_scan_totals(const double* __restrict__ totals,
+                                                              uint64_t nc,
+                                                              double* __restrict__ offset,
+                                                              double* __restrict__ bsum) {
+  __shared__ double wsum[kScanThreads / 64];
+  const uint64_t c0 = uint64_t(blockIdx.x) * kScanChunk + uint64_t(threadIdx.x) * kScanPer;
+  double v[kScanPer];
+#pragma unroll
+  for (int j = 0; j < kScanPer; ++j) v[j] = c0 + j < nc ? totals[c0 + j] : 0.0;
+  double start;
+  const double total = block_scan<kScanPer>(v, wsum, &start);
+#pragma unroll
+  for (int j = 0; j < kScanPer; ++j)
+    if (c0 + j < nc) offset[c0 + j + 1] = v[j];
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// Level 2 (one workgroup): bsum becomes its exclusive prefix (nb <= kScanThreads * 16).
+__global__ __launch_bounds__(kScanThreads) void k_scan_blocks(double* __restrict__ bsum,
+                                                              uint64_t nb) {
+  constexpr int PER = 16;
+  __shared__ double wsum[kScanThreads / 64];
+  const uint64_t b0 = uint64_t(threadIdx.x) * PER;
+  double v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) v[j] = b0 + j < nb ? bsum[b0 + j] : 0.0;
+  double start;
+  block_scan<PER>(v, wsum, &start);
+#pragma unroll
+  for (int j = 0; j < PER; ++j)  // exclusive: the inclusive prefix of the element before
+    if (b0 + j < nb) bsum[b0 + j] = j == 0 ? start : v[j - 1];
+}
+
+// Level 3: offset[c + 1] += the prefix of the runs before chunk c's; offset[0] = 0.
+__global__ __launch_bounds__(kScanThreads) void k_add_blocks(const double* __restrict__ bsum,
+                                                             uint64_t nc,
+                                                             double* __restrict__ offset) {
+  const uint64_t c = uint64_t(blockIdx.x) * kScanThreads + threadIdx.x;
+  if (c == 0) offset[0] = 0.0;
+  if (c < nc) offset[c + 1] += bsum[c / kScanChunk];
 }
 
 __global__ __launch_bounds__(256) void k_draw(const double* __restrict__ local,
@@ -144,8 +213,13 @@ __global__ __launch_bounds__(256) void k_draw(const double* __restrict__ local,
 
 uint64_t sample_chunks(uint64_t dim) { return (dim + kScanChunk - 1) / kScanChunk; }
 
+uint64_t sample_blocks(uint64_t dim) {
+  return (sample_chunks(dim) + kScanChunk - 1) / kScanChunk;
+}
+
 hipError_t launch_sample(const State& st, uint64_t shots, uint64_t seed, double* local,
-                         double* totals, double* offset, uint64_t* out, hipStream_t s) {
+                         double* totals, double* offset, double* bsum, uint64_t* out,
+                         hipStream_t s) {
   const uint64_t dim = st.dim();
   const uint64_t nc = sample_chunks(dim);
   if (st.dtype == 0)
@@ -156,7 +230,11 @@ hipError_t launch_sample(const State& st, uint64_t shots, uint64_t seed, double*
         static_cast<const float*>(st.amp), dim, local, totals);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_scan_offsets<<<1, 64, 0, s>>>(totals, nc, offset);
+  const uint64_t nb = (nc + kScanChunk - 1) / kScanChunk;
+  k_scan_totals<<<unsigned(nb), kScanThreads, 0, s>>>(totals, nc, offset, bsum);
+  k_scan_blocks<<<1, kScanThreads, 0, s>>>(bsum, nb);
+  k_add_blocks<<<unsigned((nc + kScanThreads - 1) / kScanThreads), kScanThreads, 0, s>>>(
+      bsum, nc, offset);
   e = hipGetLastError();
   if (e != hipSuccess || shots == 0) return e;
   const uint64_t g = (shots + 255) / 256;
