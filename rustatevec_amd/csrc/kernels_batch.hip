@@ -74,7 +74,28 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict_
       constexpr uint32_t LA = tile::ilog2c(kThreads * A);
       const uint32_t la = tb < LA ? tb : LA;
       const uint64_t off_lane = tile::offset((threadIdx.x * A) & (N - 1), ts, la);
+      if constexpr (FULL) {
+        // a full tile: every chunk's load issued before any is written to LDS (a chunk
+        // offset's opaque bit reads keep the scheduler from grouping the loads otherwise)
+        constexpr uint32_t PER = NMAX / (kThreads * A);
+        if (!(flags & kBatchInit)) {
+          T r[PER][A], m[PER][A];
+#pragma unroll
+          for (uint32_t c = 0; c < PER; ++c)
+            cload<T, A>(amp, base + (off_lane | tile::chunk_offset(c, ts, LA)), r[c], m[c]);
+#pragma unroll
+          for (uint32_t c = 0; c < PER; ++c) {
+            const uint32_t e = (threadIdx.x + kThreads * c) * A;
+#pragma unroll
+            for (int v = 0; v < A; ++v) {
+              sr[e + v] = r[c][v];
+              si[e + v] = m[c][v];
+            }
+          }
+        }
+      }
       for (uint32_t e = threadIdx.x * A, c = 0; e < N; e += kThreads * A, ++c) {
+        if (FULL && !(flags & kBatchInit)) break;  // loaded above
         const uint64_t off = off_lane | tile::chunk_offset(c, ts, LA);
         if (flags & kBatchInit) {
           // |0...0>: amplitude 1 at state index 0, which is tile 0, local index 0

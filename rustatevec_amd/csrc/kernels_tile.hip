@@ -46,31 +46,42 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   const uint64_t off_lane = tile::offset(threadIdx.x * A, pa.ts, LA);
   for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
     const uint64_t base = tile::insert_zero_bits(t, pa.tsort, TB);
-    // A consecutive tile-local indices are A consecutive amplitudes (ts starts 0, 1, 2, 3)
+    // A consecutive tile-local indices are A consecutive amplitudes (ts starts 0, 1, 2, 3).
+    // All the lane's loads are issued before any of them is written to LDS: a chunk offset
+    // reads its bit positions through an opaque statement, which the scheduler does not move
+    // loads across, so a load-and-store per chunk would wait out each load in turn.
+    {
+      T r[PER][A], m[PER][A];
 #pragma unroll
-    for (uint32_t c = 0; c < PER; ++c) {
-      const uint32_t e = (threadIdx.x + kThreads * c) * A;
-      T r[A], m[A];
-      cload<T, A>(amp, base + (off_lane | tile::chunk_offset(c, pa.ts, LA)), r, m);
+      for (uint32_t c = 0; c < PER; ++c)
+        cload<T, A>(amp, base + (off_lane | tile::chunk_offset(c, pa.ts, LA)), r[c], m[c]);
 #pragma unroll
-      for (int w = 0; w < A; ++w) {
-        sr[e + w] = r[w];
-        si[e + w] = m[w];
+      for (uint32_t c = 0; c < PER; ++c) {
+        const uint32_t e = (threadIdx.x + kThreads * c) * A;
+#pragma unroll
+        for (int w = 0; w < A; ++w) {
+          sr[e + w] = r[c][w];
+          si[e + w] = m[c][w];
+        }
       }
     }
     __syncthreads();
     tile::run_ops<T, kThreads, int(N / kThreads)>(sr, si, N, base, ops + pa.op_begin, pa.n_ops,
                                                   diags, pool);
+    {  // likewise every LDS read before the stores
+      T r[PER][A], m[PER][A];
 #pragma unroll
-    for (uint32_t c = 0; c < PER; ++c) {
-      const uint32_t e = (threadIdx.x + kThreads * c) * A;
-      T r[A], m[A];
+      for (uint32_t c = 0; c < PER; ++c) {
+        const uint32_t e = (threadIdx.x + kThreads * c) * A;
 #pragma unroll
-      for (int w = 0; w < A; ++w) {
-        r[w] = sr[e + w];
-        m[w] = si[e + w];
+        for (int w = 0; w < A; ++w) {
+          r[c][w] = sr[e + w];
+          m[c][w] = si[e + w];
+        }
       }
-      cstore<T, A>(amp, base + (off_lane | tile::chunk_offset(c, pa.ts, LA)), r, m);
+#pragma unroll
+      for (uint32_t c = 0; c < PER; ++c)
+        cstore<T, A>(amp, base + (off_lane | tile::chunk_offset(c, pa.ts, LA)), r[c], m[c]);
     }
     __syncthreads();
   }
