@@ -512,6 +512,9 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
     ah[ks] = *reinterpret_cast<const bf16x8*>(ahi + idx);
     al[ks] = *reinterpret_cast<const bf16x8*>(alo + idx);
   }
+  // the tile's previous op done by every wave (the A loads above, issued first, are in
+  // flight meanwhile)
+  __syncthreads();
   // LDS byte offsets (4 x the words: the XOR structure carries over)
   uint32_t U[K];
 #pragma unroll
@@ -680,8 +683,11 @@ __device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const float2* 
   }
 }
 
+// Three workgroups per CU for passes of groups up to 5 qubits (their LDS allows it; the
+// register budget of 3 waves per SIMD is asked of the compiler), two with 6-qubit groups.
 template <int TB, int KMAX, bool NTA>
-__global__ __launch_bounds__(kBlock) void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KMAX == 6 ? 2 : 3)))
+void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
                                                        MtPassArg pa,
                                                        const MtOp* __restrict__ ops,
                                                        const unsigned char* __restrict__ pool,
@@ -721,7 +727,7 @@ __global__ __launch_bounds__(kBlock) void k_mfma_tiles(float* __restrict__ amp, 
     for (uint32_t i = 0; i < pa.n_ops;) {
       const MtOp& op = pops[i];
       if (op.kind == 0) {
-        __syncthreads();
+        // (mt_dense starts with the barrier after its A-fragment loads)
         // instantiated up to the pass's widest group only: a kernel holding the 6-qubit
         // product's registers runs at lower occupancy
         if (KMAX == 4 || op.k == 4) mt_dense<4, TB>(xr, xi, op, pool);
