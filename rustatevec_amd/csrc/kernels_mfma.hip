@@ -684,7 +684,8 @@ __device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const float2* 
 }
 
 // Three workgroups per CU for passes of groups up to 5 qubits (their LDS allows it; the
-// register budget of 3 waves per SIMD is asked of the compiler), two with 6-qubit groups.
+// register budget of 3 waves per SIMD is asked of the compiler), two with 6-qubit groups
+// (asked for 3, that kernel spills).
 template <int TB, int KMAX, bool NTA>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KMAX == 6 ? 2 : 3)))
 void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
