@@ -13,12 +13,17 @@
 // the loads and stores are coalesced whatever qubits the group acts on.  A tile is staged
 // in LDS as the 2D x 2^C real block operand; the four waves of a workgroup each produce
 // one 32 x 32 output block with MFMA, write it back through LDS, and the tile is stored.
-// Global loads of the next tile are issued before the current tile's MFMA work.
+// The next tile's loads are issued before the current tile's MFMA work.
 //
 // FP32 variant: v_mfma_f32_32x32x2_f32, the block matrix held in registers as A fragments.
 // BF16x3 variant: v_mfma_f32_32x32x16_bf16 with both operands split into bf16 hi + lo
 // (x = hi + lo, lo = bf16(x - hi)); Y = Ahi Bhi + Ahi Blo + Alo Bhi accumulated in FP32
 // (the lo*lo term, ~2^-16 relative, is dropped).
+//
+// MFMA tile passes (k_mfma_tiles, end of the file): a run of fused groups per HBM round
+// trip.  A 2^12-amplitude tile stays in LDS while the pass's dense groups (BF16x3 products
+// reading their operands straight from the tile) and diagonal groups (phases, the qubits
+// outside the tile fixed per tile) are applied to it in order.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>

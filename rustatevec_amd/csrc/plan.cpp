@@ -1,5 +1,6 @@
 // plan.cpp -- launch-list passes of a program: routing wide dense gates to the matrix
-// cores (MFMA precisions on complex64 states) and the LDS tile passes.
+// cores (MFMA precisions on complex64 states), the exact LDS tile passes, and the MFMA tile
+// passes (runs of fused groups per 2^12-amplitude tile).
 #include <algorithm>
 #include <cstring>
 #include <vector>

@@ -9,6 +9,8 @@
 // size), all candidates being evaluated in one batched forward pass on the device; the
 // plan minimises the predicted total.  The greedy plans for every max_qubits are scored by
 // the same model and kept if they predict lower (greedy runs can exceed the window).
+// plan_form then prices the other execution forms with their own networks -- exact tile
+// passes and greedy fusion run as MFMA tile passes -- and keeps the cheapest.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
