@@ -170,6 +170,8 @@ int qsvh_fuse_groups(const qsvh_circuit* in, uint32_t max_qubits, uint64_t* begi
 /* ------------------------------------------------------------------ state (Backend) */
 /* Backend::alloc (crates/qsv-core/src/backend/mod.rs:36-37); contents unspecified. */
 int qsvh_state_alloc(qsvh_ctx* ctx, uint32_t n_qubits, int dtype, qsvh_state** out);
+/* Destroy a state's programs (qsvh_program_destroy) before the state: a program keeps a
+ * reference to its state and must not outlive it. */
 void qsvh_state_free(qsvh_state* st);
 uint32_t qsvh_state_n_qubits(const qsvh_state* st);
 int qsvh_state_dtype(const qsvh_state* st);

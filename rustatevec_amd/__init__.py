@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from typing import Iterable, Sequence
 
 import numpy as np
@@ -324,6 +325,7 @@ class State:
         self._h = handle
         self.n_qubits = n
         self.dtype = dtype
+        self._programs = weakref.WeakSet()  # programs bound to this state
 
     @property
     def handle(self):
@@ -334,12 +336,18 @@ class State:
         return np.float64 if self.dtype == C128 else np.float32
 
     def free(self):
+        """Free the device state; programs bound to it are destroyed first (the C ABI's
+        order: a program must not outlive its state)."""
         if self._h is not None:
+            for p in list(self._programs):
+                p._destroy()
             lib().qsvh_state_free(self._h)
             self._h = None
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and _lib is not None:
+            for p in list(getattr(self, "_programs", ())):
+                p._destroy()
             _lib.qsvh_state_free(self._h)
             self._h = None
 
@@ -357,25 +365,34 @@ class Program:
         _check(lib().qsvh_program_create(state.handle, circuit.handle, C.byref(opts),
                                          C.byref(h)), state.backend.ctx)
         self._h = h
+        state._programs.add(self)
+
+    def _handle(self):
+        if self._h is None:
+            raise QsvhError(1, "program destroyed (its state was freed)")
+        return self._h
+
+    def _destroy(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.qsvh_program_destroy(self._h)
+            self._h = None
 
     def launch(self):
-        _check(lib().qsvh_program_launch(self._h), self.state.backend.ctx)
+        _check(lib().qsvh_program_launch(self._handle()), self.state.backend.ctx)
 
     def time(self) -> float:
         """Launch once; device milliseconds from HIP events on the engine's stream."""
         ms = C.c_double(0)
-        _check(lib().qsvh_program_time(self._h, C.byref(ms)), self.state.backend.ctx)
+        _check(lib().qsvh_program_time(self._handle(), C.byref(ms)), self.state.backend.ctx)
         return ms.value
 
     def num_kernels(self) -> int:
         n = C.c_uint64(0)
-        _check(lib().qsvh_program_num_kernels(self._h, C.byref(n)))
+        _check(lib().qsvh_program_num_kernels(self._handle(), C.byref(n)))
         return n.value
 
     def __del__(self):
-        if getattr(self, "_h", None) is not None and _lib is not None:
-            _lib.qsvh_program_destroy(self._h)
-            self._h = None
+        self._destroy()
 
 
 TUNE_KEYS = ("max_blocks", "unroll", "nontemporal", "mfma_blocks", "mfma_layout",

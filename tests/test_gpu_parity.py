@@ -411,3 +411,16 @@ def test_subspace_launches_match_the_full_passes(backend, dtype):
         want = oracle.execute(oracle.CPU_PARALLEL, n, circ.ops(), dtype=npdt)
         assert max_abs_diff(got, want) == 0.0, n
         assert k_sub == k_full
+
+
+def test_freeing_a_state_destroys_its_programs(backend):
+    # a program must not outlive its state: freeing the state destroys the programs bound
+    # to it first, and a destroyed program refuses to launch instead of touching freed memory
+    st = backend.alloc(12)
+    p = rs.Program(st, rs.qft(12), tile_qubits=12)
+    backend.init_basis(st, 0)
+    p.launch()
+    st.free()
+    with pytest.raises(rs.QsvhError):
+        p.launch()
+    del p
