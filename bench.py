@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--mt-only", action="store_true",
                     help="only 3 launches of QFT(30) fused at k = 4 as split-BF16 MFMA tile "
                          "passes (2^12-amplitude tiles), c64, for profiler runs")
+    ap.add_argument("--tile-only", action="store_true",
+                    help="only 3 launches of the random mix circuit (28 qubits, 1120 gates) "
+                         "as exact 2^13-amplitude tile passes, c64, for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -135,6 +138,17 @@ def main():
         ms = [round(p.time(), 3) for _ in range(3)]
         print(json.dumps({"qft30_fused_k4_mfma_tiles12_ms": ms, "passes": p.num_kernels()}),
               flush=True)
+        return
+
+    if args.tile_only:
+        b = rs.HipBackend(0)
+        st = b.alloc(28, rs.C64)
+        p = rs.Program(st, rs.mix(28, 40 * 28, 0xC0FFEE), tile_qubits=13)
+        ms = []
+        for _ in range(3):
+            b.init_basis(st, 0)
+            ms.append(round(p.time(), 3))
+        print(json.dumps({"mix28_tiles13_ms": ms, "passes": p.num_kernels()}), flush=True)
         return
 
     if args.mfma_only:
