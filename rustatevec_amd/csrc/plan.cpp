@@ -735,9 +735,13 @@ Launch widen_to_4(const Launch& l, const uint32_t* extra) {
 
 }  // namespace
 
-// Consecutive ops go into one pass while the qubits of its dense ops (with bits 0..3) fit
+// A pass takes ops in circuit order while the qubits of its dense ops (with bits 0..3) fit
 // in tb bits; diagonal ops join any pass (their qubits outside the tile are fixed per tile).
-// Op order is kept, so every pass applies its ops exactly in circuit order.
+// An op that does not fit is left for a later pass, and the scan goes on past it: a later
+// op joins the pass only if it commutes with every op left behind before it -- a dense op
+// shares no qubit with them, a diagonal op none with the dense ones among them (diagonal
+// ops commute with each other).  Moving such ops ahead leaves the product unchanged, and
+// each pass applies its ops in circuit order.
 void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPlan* out) {
   std::vector<MtOp>& ops = out->ops;
   std::vector<uint8_t>& pool = out->pool;
@@ -876,24 +880,44 @@ void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPla
     S = low;
     cur_tab = 0;
   };
-  for (size_t i = 0; i < in.size(); ++i) {
-    const Launch& l = in[i];
-    if (l.kind == Launch::kDiag && l.m <= 6) {
-      if (cur_tab + (1u << l.m) > kMtTabEntries) close();
-      cur_tab += 1u << l.m;
-      cur.push_back(i);
-    } else if ((l.kind == Launch::k1q || l.kind == Launch::kMq) && l.m <= 6) {
+  auto is_diag = [](const Launch& l) { return l.kind == Launch::kDiag && l.m <= 6; };
+  auto is_dense = [](const Launch& l) {
+    return (l.kind == Launch::k1q || l.kind == Launch::kMq) && l.m <= 6;
+  };
+  const uint64_t all = n >= 64 ? ~0ull : (1ull << n) - 1;
+  std::vector<char> taken(in.size(), 0);
+  size_t first = 0;  // first op not yet placed
+  while (first < in.size()) {
+    if (!is_diag(in[first]) && !is_dense(in[first])) {  // runs on its own, in order
+      out->launches.push_back(in[first++]);
+      continue;
+    }
+    uint64_t left_dense = 0, left_diag = 0;  // qubits of the ops left behind so far
+    for (size_t i = first; i < in.size() && left_dense != all; ++i) {
+      if (taken[i]) continue;
+      const Launch& l = in[i];
+      if (!is_diag(l) && !is_dense(l)) break;  // no op moves ahead of it
       uint64_t Q = 0;
       for (uint32_t j = 0; j < l.m; ++j) Q |= 1ull << l.q[j];
-      if (popcount64(S | Q) > tb) close();
-      S |= Q;
+      bool take;
+      if (is_diag(l))
+        take = !(Q & left_dense) && cur_tab + (1u << l.m) <= kMtTabEntries;
+      else
+        take = !(Q & (left_dense | left_diag)) && popcount64(S | Q) <= tb;
+      if (!take) {
+        (is_diag(l) ? left_diag : left_dense) |= Q;
+        continue;
+      }
+      if (is_diag(l))
+        cur_tab += 1u << l.m;
+      else
+        S |= Q;
       cur.push_back(i);
-    } else {
-      close();
-      out->launches.push_back(l);
+      taken[i] = 1;
     }
+    close();
+    while (first < in.size() && taken[first]) ++first;
   }
-  close();
 }
 
 int plan_mfma_tiles(Program* p, int tb) {

@@ -111,6 +111,31 @@ def test_diagonal_groups_outside_the_tile_and_passes_of_only_diagonals(backend):
     check(got, exact(n, ops2, psi), len(ops2))
 
 
+def test_commuting_groups_move_into_earlier_passes(backend):
+    # {12..17} does not fit the first pass's tile, the diagonal on {17, 0} and the group on
+    # {0..3} do not commute with what is left behind before them; the gate on 5 and the
+    # diagonal on {6, 18, 19} do and join the first pass: two passes, the oracle's state
+    n = 20
+    rng = np.random.default_rng(5)
+    psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+    psi /= np.linalg.norm(psi)
+    ops = []
+    for kind, qs in (("dense", [4, 5, 6, 7, 8, 9]), ("dense", [10, 11]),
+                     ("dense", [12, 13, 14, 15, 16, 17]), ("diag", [17, 0]),
+                     ("dense", [0, 1, 2, 3]), ("dense", [5]), ("diag", [6, 18, 19]),
+                     ("dense", [18, 19])):
+        d = 1 << len(qs)
+        U = (random_unitary(d, rng) if kind == "dense"
+             else np.diag(np.exp(1j * rng.uniform(0, 2 * np.pi, size=d))))
+        ops.append((U, qs))
+    want = exact(n, ops, psi)
+    got, nk = run(backend, n, ops, psi, mfma_tiles=12)
+    assert nk == 2
+    err = check(got, want, len(ops))
+    per_group, _ = run(backend, n, ops, psi)
+    assert err < 3 * errors(per_group, want)[0], (err, errors(per_group, want))
+
+
 def test_phase_table_budget_splits_long_diagonal_runs(backend):
     # a pass's diagonal phases (2048 entries, staged in LDS) bound its run: 40 six-qubit
     # diagonal groups (2560 entries) need two passes, applied in order

@@ -43,8 +43,9 @@ def test_one_pass_counts_dense_merged_and_table_entries():
 
 def test_passes_split_at_the_tile_and_small_gates_widen():
     # n = 20: a dense group on {12..17} cannot join the first pass (tile {0..11} after
-    # {4..11}): two passes.  A lone diagonal between two dense groups stays a single op;
-    # a 2-qubit dense gate counts as a widened 4-qubit group.
+    # {4..11}): two passes.  The 6-qubit diagonal after it shares no qubit with it, so it
+    # joins the first pass; a lone diagonal between two dense groups stays a single op; a
+    # 2-qubit dense gate counts as a widened 4-qubit group.
     rng = np.random.default_rng(2)
     c = rs.Circuit(20)
     c.push(dense(rng, 6), [4, 5, 6, 7, 8, 9])
@@ -56,7 +57,42 @@ def test_passes_split_at_the_tile_and_small_gates_widen():
     assert other == 0 and rows.shape[0] == 2
     first, second = rows
     assert first[2] == pytest.approx(1 / 16) and first[0] == pytest.approx(1 / 16)
-    assert first[3] == pytest.approx(1 / 32) and first[4] == 0
-    assert first[7] == pytest.approx(8 / 2048) and first[9] == 1.0
-    assert second[2] == pytest.approx(1 / 16) and second[3] == pytest.approx(1 / 32)
-    assert second[7] == pytest.approx(64 / 2048)
+    assert first[3] == pytest.approx(2 / 32) and first[4] == 0
+    assert first[7] == pytest.approx(72 / 2048) and first[9] == 1.0
+    want = np.zeros(planner.N_MT_FEATURES, dtype=np.float32)
+    want[2] = 1 / 16
+    want[9] = 1.0
+    np.testing.assert_allclose(second, want, rtol=1e-6)
+
+
+def test_ops_that_commute_with_the_ops_left_behind_move_ahead():
+    # n = 20.  {12..17} does not fit the first pass's tile; the diagonal on {17, 0} shares
+    # qubit 17 with it and the group on {0..3} qubit 0 with that diagonal: both stay behind.
+    # The 1-qubit gate on 5 and the diagonal on {6, 18, 19} commute with all three and join
+    # the first pass; the gate on {18, 19} commutes too but would widen the tile past 12 bits.
+    rng = np.random.default_rng(3)
+    c = rs.Circuit(20)
+    c.push(dense(rng, 6), [4, 5, 6, 7, 8, 9])
+    c.push(dense(rng, 2), [10, 11])
+    c.push(dense(rng, 6), [12, 13, 14, 15, 16, 17])
+    c.push(diag(rng, 2), [17, 0])
+    c.push(dense(rng, 4), [0, 1, 2, 3])
+    c.push(dense(rng, 1), [5])
+    c.push(diag(rng, 3), [6, 18, 19])
+    c.push(dense(rng, 2), [18, 19])
+    rows, other = planner.mt_pass_features(c)
+    assert other == 0 and rows.shape == (2, planner.N_MT_FEATURES)
+    first = np.zeros(planner.N_MT_FEATURES, dtype=np.float32)
+    first[0] = 2 / 16      # {10, 11} and {5} widened to 4 qubits
+    first[2] = 1 / 16      # {4..9}
+    first[3] = 1 / 32      # the diagonal on {6, 18, 19}
+    first[7] = 8 / 2048
+    first[9] = 1.0
+    second = np.zeros(planner.N_MT_FEATURES, dtype=np.float32)
+    second[0] = 2 / 16     # {0..3} and {18, 19} widened
+    second[2] = 1 / 16     # {12..17}
+    second[3] = 1 / 32     # the diagonal on {17, 0}
+    second[7] = 4 / 2048
+    second[9] = 1.0
+    np.testing.assert_allclose(rows[0], first, rtol=1e-6)
+    np.testing.assert_allclose(rows[1], second, rtol=1e-6)
