@@ -141,6 +141,7 @@ hipError_t issue_all(Program* p) {
 
 int program_launch(Program* p) {
   Ctx* ctx = p->st->ctx;
+  if (p->launches.empty()) return kOk;  // an empty circuit: nothing to run or capture
   if (!p->use_graph) return hip_err(ctx, issue_all(p), "launch");
   if (!p->graph) {
     hipGraph_t g = nullptr;

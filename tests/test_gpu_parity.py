@@ -424,3 +424,33 @@ def test_freeing_a_state_destroys_its_programs(backend):
     with pytest.raises(rs.QsvhError):
         p.launch()
     del p
+
+
+@pytest.mark.parametrize("kw", [{}, {"tile_qubits": 12}, {"subspace": True}, {"use_graph": True},
+                                {"precision": rs.PREC_MFMA_F32},
+                                {"precision": rs.PREC_MFMA_BF16X3},
+                                {"precision": rs.PREC_MFMA_BF16X3, "mfma_tiles": 12}],
+                         ids=["plain", "tiles", "subspace", "graph", "mfma_f32", "mfma_bf16x3",
+                              "mfma_tiles"])
+def test_empty_circuit_leaves_the_state_unchanged(backend, kw):
+    # Backend::execute over no ops is the initial state (backend/mod.rs:66-73); every form
+    # of an empty program runs no kernel and leaves an uploaded state bit-identical
+    n = 14
+    dtype = rs.C64 if "precision" in kw else rs.C128
+    rng = np.random.default_rng(14)
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi = psi.astype(np.complex64 if dtype == rs.C64 else np.complex128)
+    st = backend.alloc(n, dtype)
+    backend.upload(st, psi)
+    p = rs.Program(st, rs.Circuit(n), **kw)
+    assert p.num_kernels() == 0
+    p.launch()
+    p.launch()
+    assert np.array_equal(backend.download(st), psi)
+    del p
+    st.free()
+    st = backend.execute(rs.Circuit(3))
+    want = np.zeros(8)
+    want[0] = 1
+    assert np.array_equal(backend.download(st), want)
+    st.free()
