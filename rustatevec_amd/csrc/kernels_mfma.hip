@@ -497,7 +497,8 @@ __device__ __forceinline__ uint32_t uxor(const uint32_t (&U)[K], uint32_t v) {
 // x3 on v_mfma_f32_32x32x16_bf16 (as k_fused_bf16x3).  Row s, column n of X is tile-local
 // index dep(s over the group bits) | dep(n over the other bits); the 2^(TB-9) output blocks
 // of 32 x 32 are dealt 2^(TB-11) per wave, all of one row band, so the band's A fragments
-// serve them all.  Every wave reads before any writes: the product is in place.
+// serve them all.  Every wave reads before any writes: the product is in place.  The caller
+// puts a barrier between the last group's writes and any later read of the tile.
 template <int K, int TB>
 __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
                                          const unsigned char* pool) {
@@ -590,7 +591,8 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
           acc[bi][r];
     }
   }
-  __syncthreads();
+  // no barrier here: whatever reads the tile next starts with one (the next dense group
+  // after its A-fragment loads, a diagonal run, the tile's store)
 }
 
 // A run of diagonal ops [i0, i1): the lane's amplitudes e = tid + 256 j stay in registers
