@@ -807,9 +807,18 @@ void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPla
           op.set_bit(j, pos[sorted[j]]);
           gm |= 1u << pos[sorted[j]];
         }
-        uint32_t o = 0;
+        // the other bits, the first 5 (a wave's 32 columns) chosen with distinct residues
+        // mod 5 where the group leaves them: tile-local bit b lands in LDS bank bit b mod 5
+        // (the tile's XOR fold), so such columns hit 32 banks
+        uint32_t o = 0, res = 0, used = gm;
+        for (uint32_t b = 0; b < tb && o < 5; ++b)
+          if (!((used >> b) & 1) && !((res >> (b % 5)) & 1)) {
+            op.set_obit(o++, b);
+            res |= 1u << (b % 5);
+            used |= 1u << b;
+          }
         for (uint32_t b = 0; b < tb; ++b)
-          if (!((gm >> b) & 1)) op.set_obit(o++, b);
+          if (!((used >> b) & 1)) op.set_obit(o++, b);
         std::vector<uint16_t> hl(2 * a.size());
         for (size_t e = 0; e < a.size(); ++e) {
           hl[e] = bf16_bits(a[e]);
