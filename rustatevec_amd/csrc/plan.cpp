@@ -742,6 +742,8 @@ Launch widen_to_4(const Launch& l, const uint32_t* extra) {
 // shares no qubit with them, a diagonal op none with the dense ones among them (diagonal
 // ops commute with each other).  Moving such ops ahead leaves the product unchanged, and
 // each pass applies its ops in circuit order.
+constexpr size_t kMtLookahead = 1024;
+
 void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPlan* out) {
   std::vector<MtOp>& ops = out->ops;
   std::vector<uint8_t>& pool = out->pool;
@@ -902,7 +904,10 @@ void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPla
       continue;
     }
     uint64_t left_dense = 0, left_diag = 0;  // qubits of the ops left behind so far
-    for (size_t i = first; i < in.size() && left_dense != all; ++i) {
+    // the scan ends once the ops left behind block every qubit, or after a bounded number
+    // of them (a long circuit on few qubits would otherwise be rescanned whole per pass)
+    size_t left = 0;
+    for (size_t i = first; i < in.size() && left_dense != all && left < kMtLookahead; ++i) {
       if (taken[i]) continue;
       const Launch& l = in[i];
       if (!is_diag(l) && !is_dense(l)) break;  // no op moves ahead of it
@@ -915,6 +920,7 @@ void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPla
         take = !(Q & (left_dense | left_diag)) && popcount64(S | Q) <= tb;
       if (!take) {
         (is_diag(l) ? left_diag : left_dense) |= Q;
+        ++left;
         continue;
       }
       if (is_diag(l))
