@@ -1,0 +1,56 @@
+"""Parity at the workloads' full sizes, through properties that do not need the oracle to
+finish: QFT(30) from the random-normal start and layered(28) depth 20, plain complex128 per
+gate, keep the norm to 1e-12; QFT(30) in tile passes and with subspace launches equals the
+plain run bit for bit (compared on the device); layered(28) in tile passes is within the
+1e-12 complex128 bar of it (its passes move gates past gates on disjoint qubits, which
+changes the rounding; bitwise parity on the recorded order is checked against the oracle at
+smaller sizes); and the fused form the bench reports (fusion at 4 qubits, split-BF16 MFMA tile
+passes, complex64) stays within the 1e-6 max amplitude error bar of the plain complex128 run,
+with 1 - fidelity below 1e-8."""
+import pytest
+
+import rustatevec_amd as rs
+
+pytestmark = pytest.mark.gpu
+
+
+def run(backend, circ, n, dtype, seed, **kw):
+    st = backend.alloc(n, dtype)
+    if seed is None:
+        backend.init_basis(st, 0)
+    else:
+        backend.init_random_normal(st, seed)
+    p = rs.Program(st, circ, **kw)
+    p.launch()
+    del p
+    return st
+
+
+@pytest.mark.parametrize("workload", ["qft30", "layered28"])
+def test_full_size_forms_agree(backend, workload):
+    if workload == "qft30":
+        n, circ, seed = 30, rs.qft(30), 1
+    else:
+        n, circ, seed = 28, rs.layered(28, 20, 0xC0FFEE), None
+    print(f"{workload}: plain complex128", flush=True)
+    plain = run(backend, circ, n, rs.C128, seed)
+    assert abs(backend.norm_sqr(plain) - 1.0) < 1e-12
+    forms = [("tiles12", {"tile_qubits": 12})]
+    if workload == "qft30":
+        forms.append(("subspace", {"subspace": True}))
+    for name, kw in forms:
+        print(f"{workload}: {name}", flush=True)
+        st = run(backend, circ, n, rs.C128, seed, **kw)
+        err, fid = backend.compare(plain, st)
+        st.free()
+        if workload == "qft30":
+            assert err == 0.0, (name, err)
+        else:
+            assert err < 1e-12 and 1.0 - fid < 1e-12, (name, err, 1.0 - fid)
+    print(f"{workload}: fused k=4, split-BF16 MFMA tile passes", flush=True)
+    fused = rs.fuse(backend, circ, 4)
+    st = run(backend, fused, n, rs.C64, seed, precision=rs.PREC_MFMA_BF16X3, mfma_tiles=12)
+    err, fid = backend.compare(plain, st)
+    st.free()
+    plain.free()
+    assert err < 1e-6 and 1.0 - fid < 1e-8, (err, 1.0 - fid)
