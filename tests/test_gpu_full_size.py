@@ -6,7 +6,7 @@ plain run bit for bit (compared on the device); layered(28) in tile passes is wi
 changes the rounding; bitwise parity on the recorded order is checked against the oracle at
 smaller sizes); and the fused form the bench reports (fusion at 4 qubits, split-BF16 MFMA tile
 passes, complex64) stays within the 1e-6 max amplitude error bar of the plain complex128 run,
-with 1 - fidelity below 1e-8."""
+with 1 - fidelity below 1e-8.  The batched sweep runs at its full size, 512 circuits."""
 import pytest
 
 import rustatevec_amd as rs
@@ -54,3 +54,28 @@ def test_full_size_forms_agree(backend, workload):
     st.free()
     plain.free()
     assert err < 1e-6 and 1.0 - fid < 1e-8, (err, 1.0 - fid)
+
+
+def test_full_size_hea_batch(backend):
+    # the batched sweep at its full size: 512 HEA circuits of 10..22 qubits, depth 8,
+    # complex64, one batch; circuits of the smallest, largest and some middle sizes against
+    # their own plain per-gate runs (the complex64 bar, 2e-5), every <Z_0> within [-1, 1]
+    circuits = rs.hea_batch(512, 8, 0xC0FFEE)
+    sizes = [c.n_qubits for c in circuits]
+    assert len(circuits) == 512 and min(sizes) == 10 and max(sizes) == 22
+    b = rs.Batch(backend, circuits, rs.C64)
+    b.run()
+    z0 = b.expectation_z0()
+    assert z0.shape == (512,) and abs(z0).max() <= 1 + 1e-6
+    picks = {sizes.index(10), sizes.index(22), len(sizes) - 1 - sizes[::-1].index(22), 0, 255}
+    for i in sorted(picks):
+        c = circuits[i]
+        st = run(backend, c, c.n_qubits, rs.C64, None)
+        want = backend.download(st)
+        ez = backend.expectation_z(st, 1)
+        st.free()
+        got = b.download(i)
+        assert abs(got - want).max() < 2e-5, i
+        assert abs(z0[i] - ez) < 1e-5, (i, z0[i], ez)
+        p = b.probabilities(i)
+        assert abs(p.sum() - 1.0) < 1e-4, i
