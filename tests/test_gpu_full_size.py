@@ -6,7 +6,8 @@ plain run bit for bit (compared on the device); layered(28) in tile passes is wi
 changes the rounding; bitwise parity on the recorded order is checked against the oracle at
 smaller sizes); and the fused form the bench reports (fusion at 4 qubits, split-BF16 MFMA tile
 passes, complex64) stays within the 1e-6 max amplitude error bar of the plain complex128 run,
-with 1 - fidelity below 1e-8.  The batched sweep runs at its full size, 512 circuits."""
+with 1 - fidelity below 1e-8.  The batched sweep runs at its full size, 512 circuits, and
+the 28-qubit random mix in the forms the learned planner chooses among."""
 import pytest
 
 import rustatevec_amd as rs
@@ -79,3 +80,20 @@ def test_full_size_hea_batch(backend):
         assert abs(z0[i] - ez) < 1e-5, (i, z0[i], ez)
         p = b.probabilities(i)
         assert abs(p.sum() - 1.0) < 1e-4, i
+
+
+def test_full_size_random_mix(backend):
+    # the learned planner's 28-qubit random mix (1120 gates) in the forms it chooses among:
+    # 13-bit exact complex64 tiles and fusion at 4 qubits in split-BF16 MFMA tile passes,
+    # against the plain complex128 run (the 1e-6 bar)
+    n = 28
+    circ = rs.mix(n, 40 * n, 0xC0FFEE)
+    plain = run(backend, circ, n, rs.C128, None)
+    for name, c, kw in (("tiles13", circ, {"tile_qubits": 13}),
+                        ("mfma_tiles", rs.fuse(backend, circ, 4),
+                         {"precision": rs.PREC_MFMA_BF16X3, "mfma_tiles": 12})):
+        st = run(backend, c, n, rs.C64, None, **kw)
+        err, fid = backend.compare(plain, st)
+        st.free()
+        assert err < 1e-6 and 1.0 - fid < 1e-8, (name, err, 1.0 - fid)
+    plain.free()
