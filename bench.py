@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--tile-only", action="store_true",
                     help="only 3 launches of the random mix circuit (28 qubits, 1120 gates) "
                          "as exact 2^13-amplitude tile passes, c64, for profiler runs")
+    ap.add_argument("--qft-tile-only", action="store_true",
+                    help="only 2 launches of QFT(30) complex128 as exact 2^12-amplitude tile "
+                         "passes from the random-normal state, for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -138,6 +141,15 @@ def main():
         ms = [round(p.time(), 3) for _ in range(3)]
         print(json.dumps({"qft30_fused_k4_mfma_tiles12_ms": ms, "passes": p.num_kernels()}),
               flush=True)
+        return
+
+    if args.qft_tile_only:
+        b = rs.HipBackend(0)
+        st = b.alloc(30, rs.C128)
+        b.init_random_normal(st, 1)
+        p = rs.Program(st, rs.qft(30), tile_qubits=12)
+        ms = [round(p.time(), 3) for _ in range(2)]
+        print(json.dumps({"qft30_tiles12_c128_ms": ms, "passes": p.num_kernels()}), flush=True)
         return
 
     if args.tile_only:
