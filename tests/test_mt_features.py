@@ -96,3 +96,20 @@ def test_ops_that_commute_with_the_ops_left_behind_move_ahead():
     second[9] = 1.0
     np.testing.assert_allclose(rows[0], first, rtol=1e-6)
     np.testing.assert_allclose(rows[1], second, rtol=1e-6)
+
+
+def test_the_commuting_scan_is_bounded():
+    # n = 20: the tile fills with {4..11}; 1100 gates on qubit 13 are left behind one after
+    # another, and the scan gives up after 1024 of them, so the gate on qubit 5 at the end
+    # (which commutes with all of them) is not pulled into the first pass
+    rng = np.random.default_rng(4)
+    c = rs.Circuit(20)
+    c.push(dense(rng, 6), [4, 5, 6, 7, 8, 9])
+    c.push(dense(rng, 2), [10, 11])
+    for _ in range(1100):
+        c.push(dense(rng, 1), [13])
+    c.push(dense(rng, 1), [5])
+    rows, other = planner.mt_pass_features(c)
+    assert other == 0 and rows.shape[0] == 2
+    assert rows[0][2] == pytest.approx(1 / 16) and rows[0][0] == pytest.approx(1 / 16)
+    assert rows[1][0] == pytest.approx(1101 / 16)
