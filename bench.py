@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--qft-tile-only", action="store_true",
                     help="only 2 launches of QFT(30) complex128 as exact 2^12-amplitude tile "
                          "passes from the random-normal state, for profiler runs")
+    ap.add_argument("--layered-mt-only", action="store_true",
+                    help="only 3 launches of layered(28) depth 20 fused at k = 4 as split-BF16 "
+                         "MFMA tile passes (2^12-amplitude tiles), c64, from |0>, for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -140,6 +143,19 @@ def main():
                        mfma_tiles=12)
         ms = [round(p.time(), 3) for _ in range(3)]
         print(json.dumps({"qft30_fused_k4_mfma_tiles12_ms": ms, "passes": p.num_kernels()}),
+              flush=True)
+        return
+
+    if args.layered_mt_only:
+        b = rs.HipBackend(0)
+        st = b.alloc(28, rs.C64)
+        p = rs.Program(st, rs.fuse(b, rs.layered(28, 20, 0xC0FFEE), 4),
+                       precision=rs.PREC_MFMA_BF16X3, mfma_tiles=12)
+        ms = []
+        for _ in range(3):
+            b.init_basis(st, 0)
+            ms.append(round(p.time(), 3))
+        print(json.dumps({"layered28_fused_k4_mfma_tiles12_ms": ms, "passes": p.num_kernels()}),
               flush=True)
         return
 
