@@ -196,10 +196,19 @@ int qsvh_expectation_z(qsvh_state* st, uint64_t mask, double* out);
 /* Backend::sample (mod.rs:52-54, which a device backend overrides "to sample on-device (no
  * full copy back)"): `shots` outcomes in [0, 2^n) drawn from |psi_i|^2 on the device by
  * inverse CDF as in cdf_sample (sample.rs:140-153): an f64 prefix sum of the probabilities,
- * then for draw k of SplitMix64 started at state `seed` (the caller's rng.state; the stream
- * advances by `shots` draws) u = unit() * total and outcome = min(first i with
- * cdf_i >= u, 2^n - 1).  Writes `shots` u64 to host out.  Temporary device memory:
- * 8 bytes per amplitude plus 8 per shot; at most 2^31 shots per call and 34 qubits. */
+ * then for draw k (k = 1..shots) of SplitMix64::new(seed) u = unit() * total and
+ * outcome = min(first i with cdf_i >= u, 2^n - 1) -- the outcomes of
+ * cdf_sample(probabilities, shots, &mut SplitMix64::new(seed)), except where u lies within
+ * rounding of a CDF boundary (the device adds the CDF in another order).
+ * Random-stream use differs from the trait: the default Backend::sample (alias_sample,
+ * sample.rs:108-111) takes 2 draws per shot from the caller's generator and leaves it
+ * advanced by 2 * shots; this call takes a seed instead and touches no host generator.  A
+ * binding that seeds it with one rng.next_u64() (SplitMix64's state is private to qsv-core)
+ * leaves the caller's generator advanced by one draw, so a caller that goes on drawing from
+ * rng sees a different stream than after the default sampler, and the outcomes are a
+ * different sequence of the same distribution.  Writes `shots` u64 to host out.  Temporary
+ * device memory: 8 bytes per amplitude plus 8 per shot; at most 2^31 shots per call and 34
+ * qubits. */
 int qsvh_sample(qsvh_state* st, uint64_t shots, uint64_t seed, uint64_t* out);
 /* Backend::download (mod.rs:61-62): the full SoA state to host re/im (2^n each). */
 int qsvh_download(qsvh_state* st, void* re, void* im);

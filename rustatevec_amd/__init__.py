@@ -391,6 +391,14 @@ class Program:
         _check(lib().qsvh_program_num_kernels(self._handle(), C.byref(n)))
         return n.value
 
+    def order(self):
+        """The circuit's op indices in the order the program applies them."""
+        n = C.c_size_t(0)
+        _check(lib().qsvh_program_order(self._handle(), None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.uint64)
+        _check(lib().qsvh_program_order(self._handle(), _ptr(out), n.value, C.byref(n)))
+        return [int(x) for x in out[: n.value]]
+
     def __del__(self):
         self._destroy()
 

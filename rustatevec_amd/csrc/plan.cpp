@@ -776,6 +776,7 @@ void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPla
     std::vector<float> tabs;  // the pass's phase table, staged in LDS once per workgroup
     for (size_t li : cur) {
       const Launch& l = in[li];
+      out->order.push_back(li);
       MtOp op{};
       if (l.kind == Launch::kDiag) {
         op.kind = 1;
@@ -900,6 +901,7 @@ void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPla
   size_t first = 0;  // first op not yet placed
   while (first < in.size()) {
     if (!is_diag(in[first]) && !is_dense(in[first])) {  // runs on its own, in order
+      out->order.push_back(first);
       out->launches.push_back(in[first++]);
       continue;
     }
@@ -943,6 +945,8 @@ int plan_mfma_tiles(Program* p, int tb) {
   if (st.n < uint32_t(tb)) return plan_mfma(p);  // smaller than one tile
   MtPlan plan;
   build_mt_plan(p->launches, st.n, uint32_t(tb), &plan);
+  // p->order is the identity here (tile_qubits excludes mfma_tiles): launch i is op i
+  p->order = std::move(plan.order);
   p->launches = std::move(plan.launches);
   p->mt_passes = std::move(plan.passes);
   const std::vector<MtOp>& ops = plan.ops;

@@ -330,7 +330,15 @@ int plan_fusion(Ctx* ctx, Mlp& mlp, const Circuit& c, uint32_t max_qubits, uint3
       uint32_t extra = 0;
       for (uint32_t j = 0; j < op.m; ++j)
         if (pt.pos(op.q[j]) < 0) ++extra;
-      if (pt.k + extra > max_qubits) break;
+      if (pt.k + extra > max_qubits) {
+        // an op wider than the cap is a group of its own, as in fusion.rs:49-54
+        if (e == b) {
+          for (uint32_t j = 0; j < op.m; ++j) pt.grow(op.q[j]);
+          pt.apply(op);
+          cands.push_back(make_candidate(c, b, b + 1, pt, op_diag(op)));
+        }
+        break;
+      }
       for (uint32_t j = 0; j < op.m; ++j)
         if (pt.pos(op.q[j]) < 0) pt.grow(op.q[j]);
       pt.apply(op);
@@ -355,14 +363,18 @@ int plan_fusion(Ctx* ctx, Mlp& mlp, const Circuit& c, uint32_t max_qubits, uint3
         from[e] = uint32_t(i);
       }
     }
+  // Every position is reachable (each op has at least its own singleton candidate), so a
+  // non-finite total means a prediction was not finite: the greedy plans decide then.
   std::vector<std::pair<uint32_t, uint32_t>> parts;
-  for (uint32_t e = L; e > 0;) {
-    const Candidate& cd = cands[from[e]];
-    parts.emplace_back(cd.begin, cd.end);
-    e = cd.begin;
-  }
-  std::reverse(parts.begin(), parts.end());
   double total = L ? best[L] : 0.0;
+  if (std::isfinite(total)) {
+    for (uint32_t e = L; e > 0;) {
+      const Candidate& cd = cands[from[e]];
+      parts.emplace_back(cd.begin, cd.end);
+      e = cd.begin;
+    }
+    std::reverse(parts.begin(), parts.end());
+  }
   int pick = 0;  // 0: dynamic programme; m > 0: greedy at max_qubits m
   for (uint32_t m = 1; m <= max_qubits; ++m) {
     const std::vector<Candidate> g = greedy_candidates(c, m);
@@ -371,7 +383,8 @@ int plan_fusion(Ctx* ctx, Mlp& mlp, const Circuit& c, uint32_t max_qubits, uint3
     if (rc != kOk) return rc;
     double t = 0;
     for (double x : gc) t += x;
-    if (t < total) {
+    // (the last greedy plan is taken whatever it predicts when nothing finite came before)
+    if (t < total || (m == max_qubits && L && parts.empty())) {
       total = t;
       pick = int(m);
       parts.clear();

@@ -85,6 +85,7 @@ struct MtPlan {
   std::vector<MtOp> ops;
   std::vector<uint8_t> pool;
   std::vector<Launch> launches;
+  std::vector<uint64_t> order;  // indices into `in` in the order the plan applies them
 };
 void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPlan* out);
 void release_extra(Program* p);

@@ -7,10 +7,16 @@ changes the rounding; bitwise parity on the recorded order is checked against th
 smaller sizes); and the fused form the bench reports (fusion at 4 qubits, split-BF16 MFMA tile
 passes, complex64) stays within the 1e-6 max amplitude error bar of the plain complex128 run,
 with 1 - fidelity below 1e-8.  The batched sweep runs at its full size, 512 circuits, and
-the 28-qubit random mix in the forms the learned planner chooses among."""
+the 28-qubit random mix in the forms the learned planner chooses among.  QFT(30)'s fused
+per-group forms (FP32 and split-BF16x3 matrix-core groups at k = 4, 5, 6) meet the same bars,
+and QFT(20) is pinned to its closed form: numpy's FFT of the start state, in the qubit order
+of the reference's generator (circuits.rs:61-74: H on qubit 0 first and the final swaps make
+QFT|x> = 2^(-n/2) sum_y exp(2 pi i rev(x) rev(y) / 2^n) |y>, rev the n-bit reversal)."""
+import numpy as np
 import pytest
 
 import rustatevec_amd as rs
+from tests.helpers import qft_closed_form
 
 pytestmark = pytest.mark.gpu
 
@@ -97,3 +103,49 @@ def test_full_size_random_mix(backend):
         st.free()
         assert err < 1e-6 and 1.0 - fid < 1e-8, (name, err, 1.0 - fid)
     plain.free()
+
+
+@pytest.mark.parametrize("k", [4, 5, 6])
+def test_full_size_qft30_per_group_mfma(backend, k):
+    # QFT(30) fused at k, one launch per group, on the matrix cores in FP32 and split-BF16x3,
+    # from the random-normal start: the 1e-6 / 1e-8 bars against plain complex128
+    n, seed = 30, 1
+    circ = rs.qft(n)
+    plain = run(backend, circ, n, rs.C128, seed)
+    fused = rs.fuse(backend, circ, k)
+    for prec in (rs.PREC_MFMA_F32, rs.PREC_MFMA_BF16X3):
+        st = run(backend, fused, n, rs.C64, seed, precision=prec)
+        err, fid = backend.compare(plain, st)
+        st.free()
+        print(f"QFT(30) fused k={k} precision {prec}: max err {err:.3e}, 1-F {1 - fid:.3e}",
+              flush=True)
+        assert err < 1e-6 and 1.0 - fid < 1e-8, (k, prec, err, 1.0 - fid)
+    plain.free()
+
+
+def test_qft20_equals_the_discrete_fourier_transform(backend):
+    # every form on the closed form: plain per gate, tile passes and subspace launches in
+    # complex128 (the 1e-12 bar), fused MFMA tile passes in complex64 (1e-6)
+    n, seed = 20, 5
+    st = backend.alloc(n, rs.C128)
+    backend.init_random_normal(st, seed)
+    psi = backend.download(st)
+    st.free()
+    want = qft_closed_form(psi)
+    circ = rs.qft(n)
+    for name, dtype, c, kw, tol in (
+            ("plain", rs.C128, circ, {}, 1e-12),
+            ("tiles12", rs.C128, circ, {"tile_qubits": 12}, 1e-12),
+            ("subspace", rs.C128, circ, {"subspace": True}, 1e-12),
+            ("mfma_tiles_k4", rs.C64, rs.fuse(backend, circ, 4),
+             {"precision": rs.PREC_MFMA_BF16X3, "mfma_tiles": 12}, 1e-6)):
+        st = backend.alloc(n, dtype)
+        backend.upload(st, psi.astype(np.complex128 if dtype == rs.C128 else np.complex64))
+        p = rs.Program(st, c, **kw)
+        p.launch()
+        del p
+        got = backend.download(st)
+        st.free()
+        err = float(np.abs(got - want).max())
+        print(f"QFT(20) {name}: max |got - DFT| = {err:.3e}", flush=True)
+        assert err < tol, (name, err)

@@ -134,6 +134,16 @@ def test_commuting_groups_move_into_earlier_passes(backend):
     err = check(got, want, len(ops))
     per_group, _ = run(backend, n, ops, psi)
     assert err < 3 * errors(per_group, want)[0], (err, errors(per_group, want))
+    # the program reports the order it applies the ops in: the first pass takes ops 0, 1 and
+    # the two that moved ahead (5, 6), the second the rest in circuit order
+    c = rs.Circuit(n)
+    for U, qs in ops:
+        c.push(U, qs)
+    st = backend.alloc(n, rs.C64)
+    p = rs.Program(st, c, precision=rs.PREC_MFMA_BF16X3, mfma_tiles=12)
+    assert p.order() == [0, 1, 5, 6, 2, 3, 4, 7]
+    del p
+    st.free()
 
 
 def test_phase_table_budget_splits_long_diagonal_runs(backend):

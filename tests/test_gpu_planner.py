@@ -78,6 +78,28 @@ def test_planned_circuits_are_correct(backend, cost_model):
             s.free()
 
 
+def test_gates_wider_than_the_cap_stay_on_their_own(backend, cost_model):
+    # fusion::fuse emits a gate wider than max_qubits as a group of its own
+    # (fusion.rs:49-54): a CX-only circuit at max_qubits 1 and CCX gates at max_qubits 2
+    # plan to one op per gate and run to the oracle's state
+    for n, max_q, gates in ((8, 1, [("cx", [i, i + 1]) for i in range(7)]),
+                            (8, 2, [("ccx", [0, 1, 2]), ("h", [3]), ("ccx", [3, 4, 5]),
+                                    ("cx", [5, 6]), ("ccx", [6, 7, 0])])):
+        circ = rs.Circuit(n)
+        circ.h(0)
+        for name, qs in gates:
+            circ.gate(name, qs)
+        planned, pred_ms, chosen = planner.plan(backend, cost_model, circ, max_q)
+        assert np.isfinite(pred_ms)
+        ops = planned.ops()
+        assert len(ops) >= sum(1 for _, qs in gates if len(qs) > max_q)
+        assert all(len(qs) <= max(max_q, 3) for _, qs in ops)
+        st = backend.execute(planned, rs.C128)
+        want = oracle.execute(oracle.CPU_PARALLEL, n, circ.ops())
+        assert np.abs(backend.download(st) - want).max() < 1e-12
+        st.free()
+
+
 def test_planner_gain_over_greedy(backend, cost_model):
     n = 22
     circ = rs.mix(n, 40 * n, 21)
@@ -184,5 +206,39 @@ def test_form_planner_with_mfma_tiles(backend, cost_model, tile_model, mt_model)
     print(f"form plan with MFMA tiles: chosen {chosen}, {t_plan:.3f} ms (predicted "
           f"{pred:.3f}); fixed forms {forms}")
     assert t_plan <= 1.15 * min(forms.values())
+    st.free()
+    ref.free()
+
+
+@pytest.mark.parametrize("n", [24, 32])
+def test_planned_mix_at_full_size(backend, cost_model, tile_model, mt_model, n):
+    """The learned plans of the random mix at the planner workload's smallest and largest
+    sizes (40 n gates, complex64) against the plain complex128 run of the circuit: the
+    dynamic-programme fusion plan on the matrix cores and the form plan_form picks, both
+    within the 1e-6 max amplitude error and 1e-8 infidelity bars."""
+    circ = rs.mix(n, 40 * n, 0xC0FFEE + n)
+    print(f"mix({n}): plain complex128", flush=True)
+    ref = backend.execute(circ, rs.C128)
+    st = backend.alloc(n, rs.C64)
+
+    def run(c, tq=0, mt=0):
+        p = rs.Program(st, c, precision=rs.PREC_EXACT if tq else rs.PREC_MFMA_BF16X3,
+                       tile_qubits=tq, mfma_tiles=mt)
+        backend.init_basis(st, 0)
+        p.launch()
+        del p
+        return backend.compare(ref, st)
+
+    planned, pred, chosen = planner.plan(backend, cost_model, circ, 6)
+    err, fid = run(planned)
+    print(f"mix({n}) fusion plan (chosen {chosen}): max err {err:.3e}, 1-F {1 - fid:.3e}",
+          flush=True)
+    assert err < 1e-6 and 1 - fid < 1e-8, (err, 1 - fid)
+    planned, tq, mt, pred, chosen = planner.plan_form(backend, cost_model, tile_model, circ,
+                                                      rs.C64, mt_model=mt_model)
+    err, fid = run(planned, tq, mt)
+    print(f"mix({n}) form plan (chosen {chosen}): max err {err:.3e}, 1-F {1 - fid:.3e}",
+          flush=True)
+    assert err < 1e-6 and 1 - fid < 1e-8, (err, 1 - fid)
     st.free()
     ref.free()

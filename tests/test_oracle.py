@@ -305,3 +305,18 @@ def test_planted_mistake_is_caught(golden):
     bad[i] = (ops[i][0].conj(), ops[i][1])
     got = oracle.execute(oracle.REF, case["n"], bad)
     assert max_abs_diff(got, z["state_mix8"]) > 1e-3
+
+
+def test_qft_is_the_discrete_fourier_transform():
+    # the oracle's QFT (circuits.rs:61-74) on a random state is numpy's FFT between bit
+    # reversals; a wrong phase sign, a dropped swap or a reversed ladder fails this
+    from tests.helpers import qft_closed_form
+    rng = np.random.default_rng(8)
+    for n in (1, 2, 5, 9):
+        psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        psi /= np.linalg.norm(psi)
+        got = oracle.execute(oracle.CPU_SERIAL, n, oracle.gen_qft(n), start=psi)
+        assert np.abs(got - qft_closed_form(psi)).max() < 1e-14, n
+        # the plain FFT without the reversals is another map (n >= 2)
+        if n >= 2:
+            assert np.abs(got - np.fft.ifft(psi) * np.sqrt(psi.size)).max() > 1e-3
