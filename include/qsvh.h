@@ -210,6 +210,17 @@ int qsvh_expectation_z(qsvh_state* st, uint64_t mask, double* out);
  * device memory: 8 bytes per amplitude plus 8 per shot; at most 2^31 shots per call and 34
  * qubits. */
 int qsvh_sample(qsvh_state* st, uint64_t shots, uint64_t seed, uint64_t* out);
+/* qsvh_sample without allocating: the same kernels, the same outcomes for the same state
+ * and seed, written to `device_out` (caller-owned device memory of `shots` u64) instead of a
+ * host array.  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops that sample every iteration,
+ * harnesses that forbid hipMalloc).  It calls no allocator and does not synchronise: the
+ * outcomes are ready when the context's stream is (qsvh_synchronize).  Its scratch is one
+ * static device array per GPU (about 32 MiB, held until the process ends) that fits the CDF
+ * tables of at most 22 qubits: larger states are rejected with QSVH_ERR_INVALID_ARG ("at
+ * most 22 qubits"), as are more than 2^31 shots.  All calls on one GPU share that array, so
+ * calls that overlap on one GPU (two contexts or streams at once) race: serialise them. */
+int qsvh_sample_noalloc(qsvh_state* st, uint64_t shots, uint64_t seed, uint64_t* device_out);
 /* Backend::download (mod.rs:61-62): the full SoA state to host re/im (2^n each). */
 int qsvh_download(qsvh_state* st, void* re, void* im);
 /* Host -> device copy of a full state.  No reference counterpart (the trait has no

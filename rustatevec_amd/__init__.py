@@ -103,6 +103,7 @@ def lib() -> C.CDLL:
         "qsvh_expectation_z": (i32, [P, u64, C.POINTER(C.c_double)]),
         "qsvh_download": (i32, [P, P, P]),
         "qsvh_sample": (i32, [P, u64, u64, P]),
+        "qsvh_sample_noalloc": (i32, [P, u64, u64, P]),
         "qsvh_upload": (i32, [P, P, P]),
         "qsvh_synchronize": (i32, [P]),
         "qsvh_init_random_normal": (i32, [P, u64]),

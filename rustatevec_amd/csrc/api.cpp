@@ -279,6 +279,9 @@ int qsvh_ctx_create(int device, qsvh_ctx** out) {
   if (e == hipSuccess) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c.cus = prop.multiProcessorCount;
+    // the allocation-free sampler's scratch address on this device (a failure here only
+    // disables qsvh_sample_noalloc, which reports it)
+    c.sample_scratch_err = sample_noalloc_scratch(device, &c.sample_scratch);
   }
   if (e != hipSuccess) {
     qsvh_ctx_destroy(h);
@@ -708,6 +711,31 @@ int qsvh_sample(qsvh_state* st, uint64_t shots, uint64_t seed, uint64_t* out) {
   if (e == hipSuccess) e = hipStreamSynchronize(s.ctx->stream);
   hipFree(buf);
   return hip_err(s.ctx, e, "sample");
+}
+
+int qsvh_sample_noalloc(qsvh_state* st, uint64_t shots, uint64_t seed, uint64_t* device_out) {
+  if (!st || (!device_out && shots > 0)) return kInvalidArg;
+  State& s = st->s;
+  Ctx* c = s.ctx;
+  if (shots > (uint64_t(1) << 31)) return set_err(c, kInvalidArg, "at most 2^31 shots per call");
+  if (s.n > kNoAllocMaxQubits)
+    return set_err(c, kInvalidArg,
+                   "qsvh_sample_noalloc: at most 22 qubits (its scratch holds the CDF of 2^22 "
+                   "amplitudes); use qsvh_sample");
+  if (!c->sample_scratch)
+    return set_err(c, kHip, std::string("sampler scratch address: ") +
+                                hipGetErrorString(c->sample_scratch_err));
+  // the layout of qsvh_sample's buffer, inside the static scratch
+  const uint64_t nc = sample_chunks(s.dim()), nb = sample_blocks(s.dim());
+  if (s.dim() + 2 * nc + 1 + nb > sample_noalloc_doubles())
+    return set_err(c, kInvalidArg, "qsvh_sample_noalloc: state larger than the scratch");
+  double* local = c->sample_scratch;
+  double* totals = local + s.dim();
+  double* offset = totals + nc;
+  double* bsum = offset + nc + 1;
+  return hip_err(c, launch_sample(s, shots, seed, local, totals, offset, bsum, device_out,
+                                  c->stream),
+                 "sample_noalloc");
 }
 
 int qsvh_download(qsvh_state* st, void* re, void* im) {

@@ -42,6 +42,10 @@ struct Ctx {
   double* d_result = nullptr;
   double* h_result = nullptr;  // pinned
   static constexpr int kMaxPartials = 4096;
+  // Device address of the allocation-free sampler's scratch on this device (null when the
+  // lookup failed; sample_scratch_err holds why).
+  double* sample_scratch = nullptr;
+  hipError_t sample_scratch_err = hipSuccess;
 };
 
 // A state vector in HBM: 2^n complex amplitudes, interleaved (re, im) pairs of T --
@@ -261,6 +265,12 @@ hipError_t launch_batch_reduce(const uint32_t* part_off, const uint32_t* n_part,
 // (sample_blocks(dim)), out (shots).
 uint64_t sample_chunks(uint64_t dim);
 uint64_t sample_blocks(uint64_t dim);
+// The allocation-free sampler's scratch: a static device array for states of up to
+// kNoAllocMaxQubits qubits (sample_noalloc_doubles() doubles), its address on `device` (the
+// current device) looked up once per device.
+constexpr uint32_t kNoAllocMaxQubits = 22;
+uint64_t sample_noalloc_doubles();
+hipError_t sample_noalloc_scratch(int device, double** ptr);
 hipError_t launch_sample(const State& st, uint64_t shots, uint64_t seed, double* local,
                          double* totals, double* offset, double* bsum, uint64_t* out,
                          hipStream_t s);

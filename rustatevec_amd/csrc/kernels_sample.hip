@@ -19,6 +19,9 @@
 // differ from the CPU's only when u falls within rounding distance of a CDF boundary.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+
 #include "amp.hpp"
 #include "engine.hpp"
 
@@ -209,9 +212,41 @@ __global__ __launch_bounds__(256) void k_draw(const double* __restrict__ local,
   out[k] = idx < dim - 1 ? idx : dim - 1;
 }
 
+// Scratch of the allocation-free sampler (qsvh_sample_noalloc): the in-chunk CDF, the chunk
+// totals, their offsets and the run sums of a state of up to kNoAllocMaxQubits qubits, laid
+// out as launch_sample takes them.  Every kernel writes what it later reads (nothing has to
+// start at zero).  Each GPU holds one copy for the life of the process; host code reaches it
+// only through the address hipGetSymbolAddress returns for the current device.
+constexpr uint64_t kNoAllocAmps = uint64_t(1) << kNoAllocMaxQubits;
+constexpr uint64_t kNoAllocChunks = kNoAllocAmps / kScanChunk;
+constexpr uint64_t kNoAllocDoubles =
+    kNoAllocAmps + 2 * kNoAllocChunks + 1 + (kNoAllocChunks + kScanChunk - 1) / kScanChunk;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
 }  // namespace
 
 uint64_t sample_chunks(uint64_t dim) { return (dim + kScanChunk - 1) / kScanChunk; }
+
+uint64_t sample_noalloc_doubles() { return kNoAllocDoubles; }
+
+hipError_t sample_noalloc_scratch(int device, double** ptr) {
+  // looked up once per device (each device holds its own copy at its own address); the
+  // caller has made `device` current
+  static std::mutex mu;
+  static std::map<int, double*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) {
+    *ptr = it->second;
+    return hipSuccess;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  addr[device] = static_cast<double*>(p);
+  *ptr = static_cast<double*>(p);
+  return hipSuccess;
+}
 
 uint64_t sample_blocks(uint64_t dim) {
   return (sample_chunks(dim) + kScanChunk - 1) / kScanChunk;
