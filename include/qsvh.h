@@ -262,7 +262,11 @@ typedef struct qsvh_program_opts {
                        consecutive ops share a pass while its dense ops' qubits and qubits
                        0..3 fit the tile and its diagonal ops' phases fit 2048 entries.
                        Excludes tile_qubits and subspace.                                */
-  int reserved[3];
+  int planned;      /* 1: run the pass plan the circuit carries (qsvh_plan_passes): each of
+                       its passes as one exact tile pass or one MFMA tile pass (precision
+                       MFMA_BF16X3 when any pass is MFMA); complex64 states of >= 12 qubits.
+                       Excludes tile_qubits, mfma_tiles and subspace.                    */
+  int reserved[2];
 } qsvh_program_opts;
 
 int qsvh_program_create(qsvh_state* st, const qsvh_circuit* c, const qsvh_program_opts* opts,
@@ -274,6 +278,10 @@ int qsvh_program_launch(qsvh_program* p);
 int qsvh_program_time(qsvh_program* p, double* ms);
 /* Number of device kernels one launch issues (passes over the state). */
 int qsvh_program_num_kernels(const qsvh_program* p, uint64_t* out);
+/* Launch once with a HIP event between consecutive kernels: ms[j] = device milliseconds of
+ * kernel j (at most cap written; *n = the program's kernel count).  Measurement helper for
+ * per-pass timings (the pass planner's training data). */
+int qsvh_program_time_kernels(qsvh_program* p, double* ms, size_t cap, size_t* n);
 /* The circuit's op indices in the order the program applies them (tile passes move
  * gates past gates on disjoint qubits).  Writes at most cap entries; *n = total. */
 int qsvh_program_order(const qsvh_program* p, uint64_t* order, size_t cap, size_t* n);
@@ -373,8 +381,9 @@ int qsvh_tile_pass_features(const qsvh_circuit* c, int dtype, uint32_t tile_qubi
  * (at most max_passes rows); *n_passes, *n_other as qsvh_tile_pass_features.  Features:
  * counts of dense groups of 4, 5, 6 qubits (/16), single diagonal ops (/32), merged diagonal
  * runs (/16) and their members (/64), per-tile table entries of the merged runs and the
- * pass's phase-table entries (/2048), widest group 5 / 6 flags.  No reference counterpart. */
-#define QSVH_MT_FEATURES 10
+ * pass's phase-table entries (/2048), widest group 5 / 6 flags, n/32 (states that fit the
+ * 256 MiB infinity cache run faster per amplitude).  No reference counterpart. */
+#define QSVH_MT_FEATURES 11
 int qsvh_mt_pass_features(const qsvh_circuit* c, float* out, size_t max_passes,
                           size_t* n_passes, size_t* n_other);
 /* Learned choice of the execution form for `in` on a `dtype` state: the fused plan of
@@ -393,6 +402,31 @@ int qsvh_plan_form(qsvh_ctx* ctx, qsvh_mlp* group_model, qsvh_mlp* tile_model,
                    qsvh_mlp* mt_model, const qsvh_circuit* in, int dtype, uint32_t max_qubits,
                    uint32_t window, qsvh_circuit** out, int* tile_qubits, int* mfma_tiles,
                    double* predicted_ms, int* chosen);
+
+/* Learned choice of the form of every pass.  The circuit (>= 12 qubits) is cut into passes:
+ * each takes gates in circuit order while its dense gates' qubits, with qubits 0..3, fit one
+ * tile of 12 qubits (diagonal gates join on any qubits within a phase budget), a gate moving
+ * ahead only past gates it commutes with.  Each pass then runs exactly (its gates one by one
+ * on the tile, complex64 arithmetic) or as one MFMA tile pass of its gates fused greedily up
+ * to k = 3, 4 or 5 qubits (fusion.rs:39-77 within the pass; a group is diagonal or inside
+ * the tile), priced per pass by tile_model (QSVH_TILE_FEATURES inputs) and mt_model
+ * (QSVH_MT_FEATURES inputs), the cheapest predicted form kept.  form: -1 learned (both
+ * models needed), 0 every pass exact, 3..5 every pass MFMA at that width (a pass a forced
+ * form cannot run takes the cheapest other).  *out: the circuit in the order it is applied,
+ * exact passes' gates as given and MFMA passes' fused groups composed in f64 on the device,
+ * carrying the pass plan (run it with program option `planned`, precision MFMA_BF16X3).
+ * *predicted_ms: the models' total (0 without models).  No reference counterpart: the
+ * reference fuses with one fixed width (fusion.rs:24-77). */
+int qsvh_plan_passes(qsvh_ctx* ctx, qsvh_mlp* tile_model, qsvh_mlp* mt_model,
+                     const qsvh_circuit* in, int form, qsvh_circuit** out,
+                     double* predicted_ms);
+/* Passes of a planned circuit (0 for any other circuit). */
+int qsvh_circuit_plan_len(const qsvh_circuit* c, size_t* n_passes);
+/* Pass i of a planned circuit: its form (0 exact, k MFMA at width k), op range, tile bits
+ * and, when features is not NULL, its cost-model feature row for its form
+ * (QSVH_TILE_FEATURES floats for an exact pass, QSVH_MT_FEATURES for an MFMA pass). */
+int qsvh_circuit_plan_pass(const qsvh_circuit* c, size_t i, int* form, uint64_t* op_begin,
+                           uint64_t* n_ops, uint64_t* tile_mask, float* features);
 
 /* Apply one gate `reps` times back to back and return the mean device time per launch
  * (HIP events on the context stream).  Measurement helper for single-kernel rooflines. */

@@ -45,7 +45,8 @@ class QsvhError(RuntimeError):
 
 class _ProgramOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("use_graph", C.c_int), ("tile_qubits", C.c_int),
-                ("subspace", C.c_int), ("mfma_tiles", C.c_int), ("reserved", C.c_int * 3)]
+                ("subspace", C.c_int), ("mfma_tiles", C.c_int), ("planned", C.c_int),
+                ("reserved", C.c_int * 2)]
 
 
 _lib = None
@@ -113,6 +114,7 @@ def lib() -> C.CDLL:
         "qsvh_program_launch": (i32, [P]),
         "qsvh_program_time": (i32, [P, C.POINTER(C.c_double)]),
         "qsvh_program_num_kernels": (i32, [P, C.POINTER(C.c_uint64)]),
+        "qsvh_program_time_kernels": (i32, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "qsvh_program_order": (i32, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "qsvh_program_destroy": (None, [P]),
         "qsvh_circuit_hea_batch": (i32, [C.c_size_t, u32, u64, P]),
@@ -357,11 +359,11 @@ class Program:
     """A circuit bound to a state (matrices resident on the device, optional graph)."""
 
     def __init__(self, state: State, circuit: Circuit, precision=PREC_EXACT,
-                 use_graph=False, tile_qubits=0, subspace=False, mfma_tiles=0):
+                 use_graph=False, tile_qubits=0, subspace=False, mfma_tiles=0, planned=False):
         self.state = state
         self.circuit = circuit
         opts = _ProgramOpts(precision, int(bool(use_graph)), int(tile_qubits),
-                            int(bool(subspace)), int(mfma_tiles))
+                            int(bool(subspace)), int(mfma_tiles), int(bool(planned)))
         h = C.c_void_p()
         _check(lib().qsvh_program_create(state.handle, circuit.handle, C.byref(opts),
                                          C.byref(h)), state.backend.ctx)
@@ -391,6 +393,15 @@ class Program:
         n = C.c_uint64(0)
         _check(lib().qsvh_program_num_kernels(self._handle(), C.byref(n)))
         return n.value
+
+    def time_kernels(self):
+        """Launch once; device milliseconds of every kernel (HIP events between them)."""
+        n = C.c_size_t(0)
+        cap = self.num_kernels()
+        out = np.zeros(max(cap, 1), dtype=np.float64)
+        _check(lib().qsvh_program_time_kernels(self._handle(), _ptr(out), cap, C.byref(n)),
+               self.state.backend.ctx)
+        return out[: n.value]
 
     def order(self):
         """The circuit's op indices in the order the program applies them."""

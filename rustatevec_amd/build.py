@@ -48,6 +48,7 @@ SOURCES = {
     "kernels_sample.hip": ["-ffp-contract=off"],
     "mlp.cpp": [],
     "planner.cpp": ["-ffp-contract=off"],
+    "passes.cpp": ["-ffp-contract=off"],
 }
 
 

@@ -171,6 +171,14 @@ int program_build(State* st, const Circuit& c, const qsvh_program_opts* opts, Pr
     return set_err(st->ctx, kDtype, "MFMA precisions need a complex64 state");
   p->launches.reserve(c.ops.size());
   for (const Op& op : c.ops) p->launches.push_back(classify(op, st->dtype));
+  if (opts && opts->planned) {
+    if (opts->tile_qubits > 0 || opts->mfma_tiles > 0 || opts->subspace)
+      return set_err(st->ctx, kInvalidArg,
+                     "planned excludes tile_qubits, mfma_tiles and subspace");
+    const int rc = plan_planned_passes(p, c);
+    if (rc != kOk) return rc;
+    return build_arena(p);
+  }
   const int mt = opts ? opts->mfma_tiles : 0;
   if (mt > 0 && opts->tile_qubits > 0)
     return set_err(st->ctx, kInvalidArg, "tile_qubits and mfma_tiles exclude each other");
@@ -862,6 +870,31 @@ int qsvh_program_num_kernels(const qsvh_program* p, uint64_t* out) {
   if (!p || !out) return kInvalidArg;
   *out = p->p.launches.size();
   return kOk;
+}
+
+int qsvh_program_time_kernels(qsvh_program* h, double* ms, size_t cap, size_t* n) {
+  if (!h || !n || (cap && !ms)) return kInvalidArg;
+  Program* p = &h->p;
+  Ctx* c = p->st->ctx;
+  const size_t L = p->launches.size();
+  *n = L;
+  std::vector<hipEvent_t> ev(L + 1, nullptr);
+  hipError_t e = hipSuccess;
+  for (size_t j = 0; j <= L && e == hipSuccess; ++j) e = hipEventCreate(&ev[j]);
+  for (size_t j = 0; j < L && e == hipSuccess; ++j) {
+    e = hipEventRecord(ev[j], c->stream);
+    if (e == hipSuccess) e = issue(*p, p->launches[j], c->tune, c->stream);
+  }
+  if (e == hipSuccess) e = hipEventRecord(ev[L], c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ev[L]);
+  for (size_t j = 0; j < L && j < cap && e == hipSuccess; ++j) {
+    float f = 0.f;
+    e = hipEventElapsedTime(&f, ev[j], ev[j + 1]);
+    ms[j] = f;
+  }
+  for (hipEvent_t x : ev)
+    if (x) hipEventDestroy(x);
+  return hip_err(c, e, "program kernel times");
 }
 
 int qsvh_program_order(const qsvh_program* p, uint64_t* order, size_t cap, size_t* n) {

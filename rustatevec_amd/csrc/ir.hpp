@@ -23,9 +23,20 @@ struct Op {
   bool is_diagonal() const;
 };
 
+// One pass of a planned circuit (qsvh_plan_passes): ops [op_begin, op_begin + n_ops),
+// applied to one LDS tile of the state bits in tile_mask per HBM round trip -- exactly
+// (form 0: the ops are the circuit's own gates) or as an MFMA tile pass (form k > 0: the
+// ops are groups fused up to k qubits, dense groups on split-BF16 matrix cores).
+struct PlanPass {
+  int form = 0;
+  uint64_t op_begin = 0, n_ops = 0;
+  uint64_t tile_mask = 0;
+};
+
 struct Circuit {
   uint32_t n = 0;
   std::vector<Op> ops;
+  std::vector<PlanPass> plan;  // empty unless the circuit came from the pass planner
 };
 
 // Status values shared with include/qsvh.h.

@@ -100,6 +100,65 @@ uint64_t qubit_mask(const Launch& l) {
 
 }  // namespace
 
+void lower_exact_pass(const std::vector<Launch>& in, const std::vector<uint64_t>& take,
+                      uint32_t n, uint32_t tb, uint64_t tmask, TileBuilder& bld,
+                      std::vector<TilePassArg>& passes, std::vector<Launch>& out,
+                      std::vector<uint64_t>& order) {
+  const uint64_t low = (1ull << std::min(kTileLowBits, tb)) - 1;
+  // Tile-local positions: the low bits first (in order: contiguous runs), then the other
+  // tile qubits by their activity in this pass, least active first.  Positions from
+  // kTileLaneBits up are register bits (a 1-qubit op there pairs amplitudes in a lane's
+  // registers) and positions 4 .. kTileLaneBits - 1 need an LDS round trip, so the
+  // qubits the pass's non-diagonal ops act on most take the register positions.
+  // Activity: 2 per non-diagonal op acting on the qubit, 1 per diagonal gate on it.
+  uint32_t busy[64] = {0};
+  for (uint64_t i : take) {
+    const uint64_t act = action_mask(in[i]);
+    for (uint32_t j = 0; j < in[i].m; ++j) {
+      const uint32_t q = in[i].q[j];
+      if (in[i].kind == Launch::kDiag)
+        busy[q] += (tmask >> q) & 1u;
+      else if ((act >> q) & 1u)
+        busy[q] += 2;
+    }
+  }
+  std::vector<uint32_t> others;
+  for (uint32_t b = 0; b < 64; ++b)
+    if (((tmask & ~low) >> b) & 1u) others.push_back(b);
+  std::stable_sort(others.begin(), others.end(),
+                   [&](uint32_t a, uint32_t c) { return busy[a] < busy[c]; });
+  TilePassArg pa{};
+  pa.tb = tb;
+  uint32_t pos[64] = {0};
+  uint32_t k = 0, ks = 0;
+  for (uint32_t b = 0; b < 64; ++b) {
+    if ((tmask >> b) & 1u) pa.tsort[ks++] = b;
+    if ((low >> b) & 1u) {
+      pa.ts[k] = b;
+      pos[b] = k++;
+    }
+  }
+  for (uint32_t b : others) {
+    pa.ts[k] = b;
+    pos[b] = k++;
+  }
+  (void)n;
+  pa.op_begin = uint32_t(bld.ops.size());
+  bld.pass_begin = bld.ops.size();
+  for (uint64_t i : take) {
+    bld.lower(in[i], tmask, pos);
+    order.push_back(i);
+  }
+  bld.close_block();
+  bld.close_run();
+  pa.n_ops = uint32_t(bld.ops.size()) - pa.op_begin;
+  Launch l;
+  l.kind = Launch::kTile;
+  l.arena_off = passes.size();
+  passes.push_back(pa);
+  out.push_back(std::move(l));
+}
+
 void schedule_tiles(const std::vector<Launch>& in, uint32_t n, uint32_t tb, TileBuilder& bld,
                     std::vector<TilePassArg>& passes, std::vector<Launch>& out,
                     std::vector<uint64_t>& order) {
@@ -130,52 +189,9 @@ void schedule_tiles(const std::vector<Launch>& in, uint32_t n, uint32_t tb, Tile
         if (popcount64(tmask) == tb) break;
       }
       for (uint32_t b = 0; popcount64(tmask) < tb && b < n; ++b) tmask |= 1ull << b;
-      // Tile-local positions: the low bits first (in order: contiguous runs), then the other
-      // tile qubits by their activity in this pass, least active first.  Positions from
-      // kTileLaneBits up are register bits (a 1-qubit op there pairs amplitudes in a lane's
-      // registers) and positions 4 .. kTileLaneBits - 1 need an LDS round trip, so the
-      // qubits the pass's non-diagonal ops act on most take the register positions.
-      // Activity: 2 per non-diagonal op acting on the qubit, 1 per diagonal gate on it.
-      uint32_t busy[64] = {0};
-      {
-        uint64_t blk = 0;
-        for (uint64_t i : rem) {
-          if ((qm[i] & blk) || (act[i] & ~tmask)) {
-            blk |= qm[i];
-            continue;
-          }
-          for (uint32_t j = 0; j < in[i].m; ++j) {
-            const uint32_t q = in[i].q[j];
-            if (in[i].kind == Launch::kDiag)
-              busy[q] += (tmask >> q) & 1u;
-            else if ((act[i] >> q) & 1u)
-              busy[q] += 2;
-          }
-        }
-      }
-      std::vector<uint32_t> others;
-      for (uint32_t b = 0; b < 64; ++b)
-        if (((tmask & ~low) >> b) & 1u) others.push_back(b);
-      std::stable_sort(others.begin(), others.end(),
-                       [&](uint32_t a, uint32_t c) { return busy[a] < busy[c]; });
-      TilePassArg pa{};
-      pa.tb = tb;
-      uint32_t pos[64] = {0};
-      uint32_t k = 0, ks = 0;
-      for (uint32_t b = 0; b < 64; ++b) {
-        if ((tmask >> b) & 1u) pa.tsort[ks++] = b;
-        if ((low >> b) & 1u) {
-          pa.ts[k] = b;
-          pos[b] = k++;
-        }
-      }
-      for (uint32_t b : others) {
-        pa.ts[k] = b;
-        pos[b] = k++;
-      }
-      pa.op_begin = uint32_t(bld.ops.size());
-      bld.pass_begin = bld.ops.size();
-      std::vector<uint64_t> rest;
+      // the pass takes every gate whose action qubits are in the tile and that is not
+      // behind a blocked qubit
+      std::vector<uint64_t> take, rest;
       blocked = 0;
       for (uint64_t i : rem) {
         if ((qm[i] & blocked) || (act[i] & ~tmask)) {
@@ -183,17 +199,9 @@ void schedule_tiles(const std::vector<Launch>& in, uint32_t n, uint32_t tb, Tile
           rest.push_back(i);
           continue;
         }
-        bld.lower(in[i], tmask, pos);
-        order.push_back(i);
+        take.push_back(i);
       }
-      bld.close_block();
-      bld.close_run();
-      pa.n_ops = uint32_t(bld.ops.size()) - pa.op_begin;
-      Launch l;
-      l.kind = Launch::kTile;
-      l.arena_off = passes.size();
-      passes.push_back(pa);
-      out.push_back(std::move(l));
+      lower_exact_pass(in, take, n, tb, tmask, bld, passes, out, order);
       rem.swap(rest);
     }
   };
@@ -744,13 +752,12 @@ Launch widen_to_4(const Launch& l, const uint32_t* extra) {
 // each pass applies its ops in circuit order.
 constexpr size_t kMtLookahead = 1024;
 
-void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPlan* out) {
+// Emit one MFMA tile pass holding the launches `cur` (in order): the tile is the bits of S
+// (the dense ops' qubits and the low bits), filled up to tb bits with the lowest others.
+void emit_mt_pass(const std::vector<Launch>& in, const std::vector<size_t>& cur, uint32_t n,
+                  uint32_t tb, uint64_t S, MtPlan* out) {
   std::vector<MtOp>& ops = out->ops;
   std::vector<uint8_t>& pool = out->pool;
-  std::vector<size_t> cur;
-  const uint64_t low = (1ull << kTileLowBits) - 1;
-  uint64_t S = low;
-  uint32_t cur_tab = 0;  // phase entries of the pass's diagonal ops so far
   auto put = [&](const void* src, size_t bytes) {
     pool.resize((pool.size() + 255) & ~size_t(255));
     const uint64_t at = pool.size();
@@ -758,136 +765,144 @@ void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPla
     pool.insert(pool.end(), b, b + bytes);
     return at;
   };
+  if (cur.empty()) return;
+  uint64_t tm = S;
+  for (uint32_t b = 0; b < n && popcount64(tm) < tb; ++b) tm |= 1ull << b;
+  MtPassArg pa{};
+  pa.tb = tb;
+  pa.kmax = 4;
+  uint32_t pos[64] = {0};
+  uint32_t nb = 0;
+  for (uint32_t b = 0; b < n; ++b)
+    if ((tm >> b) & 1) {
+      pa.ts[nb] = b;
+      pos[b] = nb++;
+    }
+  pa.op_begin = uint32_t(ops.size());
+  std::vector<float> tabs;  // the pass's phase table, staged in LDS once per workgroup
+  for (size_t li : cur) {
+    const Launch& l = in[li];
+    out->order.push_back(li);
+    MtOp op{};
+    if (l.kind == Launch::kDiag) {
+      op.kind = 1;
+      op.k = l.m;
+      for (uint32_t j = 0; j < l.m; ++j) {
+        const uint32_t q = l.q[j];
+        if ((tm >> q) & 1) {
+          op.in_mask |= 1u << j;
+          op.set_bit(j, pos[q]);
+        } else {
+          op.set_bit(j, q);
+        }
+      }
+      op.tab = uint32_t(tabs.size() / 2);
+      for (size_t e = 0; e < (size_t(2) << l.m); ++e) tabs.push_back(float(l.mat[e]));
+    } else {
+      Launch g = l;
+      if (l.m < 4) {
+        uint32_t extra[4], ne = 0;
+        for (uint32_t b = 0; b < n && ne < 4 - l.m; ++b)
+          if (((tm >> b) & 1) && std::find(l.q, l.q + l.m, b) == l.q + l.m) extra[ne++] = b;
+        g = widen_to_4(l, extra);
+      }
+      uint32_t sorted[kMaxGateQubits];
+      const std::vector<float> a = block_matrix(g, sorted);
+      op.kind = 0;
+      op.k = g.m;
+      pa.kmax = std::max(pa.kmax, g.m);
+      uint32_t gm = 0;
+      for (uint32_t j = 0; j < g.m; ++j) {
+        op.set_bit(j, pos[sorted[j]]);
+        gm |= 1u << pos[sorted[j]];
+      }
+      // the other bits, the first 5 (a wave's 32 columns) chosen with distinct residues
+      // mod 5 where the group leaves them: tile-local bit b lands in LDS bank bit b mod 5
+      // (the tile's XOR fold), so such columns hit 32 banks
+      uint32_t o = 0, res = 0, used = gm;
+      for (uint32_t b = 0; b < tb && o < 5; ++b)
+        if (!((used >> b) & 1) && !((res >> (b % 5)) & 1)) {
+          op.set_obit(o++, b);
+          res |= 1u << (b % 5);
+          used |= 1u << b;
+        }
+      for (uint32_t b = 0; b < tb; ++b)
+        if (!((used >> b) & 1)) op.set_obit(o++, b);
+      std::vector<uint16_t> hl(2 * a.size());
+      for (size_t e = 0; e < a.size(); ++e) {
+        hl[e] = bf16_bits(a[e]);
+        hl[a.size() + e] = bf16_bits(a[e] - bf16_value(hl[e]));
+      }
+      op.mat = put(hl.data(), hl.size() * sizeof(uint16_t));
+    }
+    ops.push_back(op);
+  }
+  // merge runs of diagonal ops: consecutive ones (they commute) whose in-tile qubits fit 8
+  // tile bits share one per-tile table (a head op, then its members)
+  {
+    std::vector<MtOp> merged(ops.begin(), ops.begin() + pa.op_begin);
+    size_t e = pa.op_begin;
+    while (e < ops.size()) {
+      if (ops[e].kind != 1) {
+        merged.push_back(ops[e++]);
+        continue;
+      }
+      uint32_t U = 0;  // tile-local bits of the group
+      size_t f = e;
+      for (; f < ops.size() && ops[f].kind == 1; ++f) {
+        uint32_t u = U;
+        for (uint32_t j = 0; j < ops[f].k; ++j)
+          if ((ops[f].in_mask >> j) & 1) u |= 1u << ((ops[f].bits[j >> 2] >> (8 * (j & 3))) & 0xff);
+        if (popcount64(u) > 8) break;
+        U = u;
+      }
+      if (f - e < 2) {
+        merged.push_back(ops[e++]);
+        continue;
+      }
+      MtOp head{};
+      head.kind = 2;
+      head.members = uint32_t(f - e);
+      uint32_t rank[32] = {0};
+      for (uint32_t b = 0; b < 32; ++b)
+        if ((U >> b) & 1) {
+          rank[b] = head.k;
+          head.set_bit(head.k++, b);
+        }
+      merged.push_back(head);
+      for (; e < f; ++e) {
+        MtOp m = ops[e];
+        m.kind = 3;
+        m.bits[0] = m.bits[1] = 0;
+        for (uint32_t j = 0; j < m.k; ++j) {
+          const uint32_t b = (ops[e].bits[j >> 2] >> (8 * (j & 3))) & 0xff;
+          m.set_bit(j, ((m.in_mask >> j) & 1) ? rank[b] : b);
+        }
+        merged.push_back(m);
+      }
+    }
+    ops.swap(merged);
+  }
+  pa.n_ops = uint32_t(ops.size()) - pa.op_begin;
+  pa.tab_entries = uint32_t(tabs.size() / 2);
+  pa.tab_off = tabs.empty() ? 0 : put(tabs.data(), tabs.size() * sizeof(float));
+  for (size_t e = ops.size(); e-- > pa.op_begin;)
+    if (ops[e].kind != 0)
+      ops[e].run = 1 + (e + 1 < ops.size() && ops[e + 1].kind != 0 ? ops[e + 1].run : 0);
+  Launch L;
+  L.kind = Launch::kMfmaTile;
+  L.arena_off = out->passes.size();
+  out->passes.push_back(pa);
+  out->launches.push_back(std::move(L));
+}
+
+void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPlan* out) {
+  std::vector<size_t> cur;
+  const uint64_t low = (1ull << kTileLowBits) - 1;
+  uint64_t S = low;
+  uint32_t cur_tab = 0;  // phase entries of the pass's diagonal ops so far
   auto close = [&]() {
-    if (cur.empty()) return;
-    uint64_t tm = S;
-    for (uint32_t b = 0; b < n && popcount64(tm) < tb; ++b) tm |= 1ull << b;
-    MtPassArg pa{};
-    pa.tb = tb;
-    pa.kmax = 4;
-    uint32_t pos[64] = {0};
-    uint32_t nb = 0;
-    for (uint32_t b = 0; b < n; ++b)
-      if ((tm >> b) & 1) {
-        pa.ts[nb] = b;
-        pos[b] = nb++;
-      }
-    pa.op_begin = uint32_t(ops.size());
-    std::vector<float> tabs;  // the pass's phase table, staged in LDS once per workgroup
-    for (size_t li : cur) {
-      const Launch& l = in[li];
-      out->order.push_back(li);
-      MtOp op{};
-      if (l.kind == Launch::kDiag) {
-        op.kind = 1;
-        op.k = l.m;
-        for (uint32_t j = 0; j < l.m; ++j) {
-          const uint32_t q = l.q[j];
-          if ((tm >> q) & 1) {
-            op.in_mask |= 1u << j;
-            op.set_bit(j, pos[q]);
-          } else {
-            op.set_bit(j, q);
-          }
-        }
-        op.tab = uint32_t(tabs.size() / 2);
-        for (size_t e = 0; e < (size_t(2) << l.m); ++e) tabs.push_back(float(l.mat[e]));
-      } else {
-        Launch g = l;
-        if (l.m < 4) {
-          uint32_t extra[4], ne = 0;
-          for (uint32_t b = 0; b < n && ne < 4 - l.m; ++b)
-            if (((tm >> b) & 1) && std::find(l.q, l.q + l.m, b) == l.q + l.m) extra[ne++] = b;
-          g = widen_to_4(l, extra);
-        }
-        uint32_t sorted[kMaxGateQubits];
-        const std::vector<float> a = block_matrix(g, sorted);
-        op.kind = 0;
-        op.k = g.m;
-        pa.kmax = std::max(pa.kmax, g.m);
-        uint32_t gm = 0;
-        for (uint32_t j = 0; j < g.m; ++j) {
-          op.set_bit(j, pos[sorted[j]]);
-          gm |= 1u << pos[sorted[j]];
-        }
-        // the other bits, the first 5 (a wave's 32 columns) chosen with distinct residues
-        // mod 5 where the group leaves them: tile-local bit b lands in LDS bank bit b mod 5
-        // (the tile's XOR fold), so such columns hit 32 banks
-        uint32_t o = 0, res = 0, used = gm;
-        for (uint32_t b = 0; b < tb && o < 5; ++b)
-          if (!((used >> b) & 1) && !((res >> (b % 5)) & 1)) {
-            op.set_obit(o++, b);
-            res |= 1u << (b % 5);
-            used |= 1u << b;
-          }
-        for (uint32_t b = 0; b < tb; ++b)
-          if (!((used >> b) & 1)) op.set_obit(o++, b);
-        std::vector<uint16_t> hl(2 * a.size());
-        for (size_t e = 0; e < a.size(); ++e) {
-          hl[e] = bf16_bits(a[e]);
-          hl[a.size() + e] = bf16_bits(a[e] - bf16_value(hl[e]));
-        }
-        op.mat = put(hl.data(), hl.size() * sizeof(uint16_t));
-      }
-      ops.push_back(op);
-    }
-    // merge runs of diagonal ops: consecutive ones (they commute) whose in-tile qubits fit 8
-    // tile bits share one per-tile table (a head op, then its members)
-    {
-      std::vector<MtOp> merged(ops.begin(), ops.begin() + pa.op_begin);
-      size_t e = pa.op_begin;
-      while (e < ops.size()) {
-        if (ops[e].kind != 1) {
-          merged.push_back(ops[e++]);
-          continue;
-        }
-        uint32_t U = 0;  // tile-local bits of the group
-        size_t f = e;
-        for (; f < ops.size() && ops[f].kind == 1; ++f) {
-          uint32_t u = U;
-          for (uint32_t j = 0; j < ops[f].k; ++j)
-            if ((ops[f].in_mask >> j) & 1) u |= 1u << ((ops[f].bits[j >> 2] >> (8 * (j & 3))) & 0xff);
-          if (popcount64(u) > 8) break;
-          U = u;
-        }
-        if (f - e < 2) {
-          merged.push_back(ops[e++]);
-          continue;
-        }
-        MtOp head{};
-        head.kind = 2;
-        head.members = uint32_t(f - e);
-        uint32_t rank[32] = {0};
-        for (uint32_t b = 0; b < 32; ++b)
-          if ((U >> b) & 1) {
-            rank[b] = head.k;
-            head.set_bit(head.k++, b);
-          }
-        merged.push_back(head);
-        for (; e < f; ++e) {
-          MtOp m = ops[e];
-          m.kind = 3;
-          m.bits[0] = m.bits[1] = 0;
-          for (uint32_t j = 0; j < m.k; ++j) {
-            const uint32_t b = (ops[e].bits[j >> 2] >> (8 * (j & 3))) & 0xff;
-            m.set_bit(j, ((m.in_mask >> j) & 1) ? rank[b] : b);
-          }
-          merged.push_back(m);
-        }
-      }
-      ops.swap(merged);
-    }
-    pa.n_ops = uint32_t(ops.size()) - pa.op_begin;
-    pa.tab_entries = uint32_t(tabs.size() / 2);
-    pa.tab_off = tabs.empty() ? 0 : put(tabs.data(), tabs.size() * sizeof(float));
-    for (size_t e = ops.size(); e-- > pa.op_begin;)
-      if (ops[e].kind != 0)
-        ops[e].run = 1 + (e + 1 < ops.size() && ops[e + 1].kind != 0 ? ops[e + 1].run : 0);
-    Launch L;
-    L.kind = Launch::kMfmaTile;
-    L.arena_off = out->passes.size();
-    out->passes.push_back(pa);
-    out->launches.push_back(std::move(L));
+    emit_mt_pass(in, cur, n, tb, S, out);
     cur.clear();
     S = low;
     cur_tab = 0;

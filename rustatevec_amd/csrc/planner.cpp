@@ -95,13 +95,35 @@ void tile_schedule_features(const Circuit& c, int dtype, uint32_t tb, std::vecto
     if (l.kind != Launch::kTile) other->push_back(std::move(l));
 }
 
-constexpr int kMtFeatures = 10;
+constexpr int kMtFeatures = 11;
 
 // Feature vector of one MFMA tile pass (complex64, 2^12-amplitude tiles): counts of its
 // dense groups by width (4, 5, 6; /16), single diagonal ops (/32), merged diagonal runs
 // (/16) and their members (/64), the entries of the per-tile tables the merged runs build
-// and of the pass's phase table (/2048), and flags for a 5- or 6-qubit widest group (the
-// kernel's occupancy steps).
+// and of the pass's phase table (/2048), flags for a 5- or 6-qubit widest group (the
+// kernel's occupancy steps) and the state size n/32.
+void mt_pass_features(const MtPlan& plan, size_t i, uint32_t n, float* f) {
+  const MtPassArg& pa = plan.passes[i];
+  std::fill(f, f + kMtFeatures, 0.f);
+  for (uint32_t j = pa.op_begin; j < pa.op_begin + pa.n_ops; ++j) {
+    const MtOp& o = plan.ops[j];
+    if (o.kind == 0) {
+      f[o.k - 4] += 1.f / 16;
+    } else if (o.kind == 1) {
+      f[3] += 1.f / 32;
+    } else if (o.kind == 2) {
+      f[4] += 1.f / 16;
+      f[6] += float(1u << o.k) / 2048;
+    } else {
+      f[5] += 1.f / 64;
+    }
+  }
+  f[7] = float(pa.tab_entries) / 2048;
+  f[8] = pa.kmax == 5 ? 1.f : 0.f;
+  f[9] = pa.kmax == 6 ? 1.f : 0.f;
+  f[10] = float(n) / 32.f;
+}
+
 void mt_schedule_features(const Circuit& c, std::vector<float>* rows, std::vector<Launch>* other) {
   std::vector<Launch> in;
   in.reserve(c.ops.size());
@@ -109,26 +131,8 @@ void mt_schedule_features(const Circuit& c, std::vector<float>* rows, std::vecto
   MtPlan plan;
   build_mt_plan(in, c.n, 12, &plan);
   rows->assign(plan.passes.size() * kMtFeatures, 0.f);
-  for (size_t i = 0; i < plan.passes.size(); ++i) {
-    const MtPassArg& pa = plan.passes[i];
-    float* f = rows->data() + i * kMtFeatures;
-    for (uint32_t j = pa.op_begin; j < pa.op_begin + pa.n_ops; ++j) {
-      const MtOp& o = plan.ops[j];
-      if (o.kind == 0) {
-        f[o.k - 4] += 1.f / 16;
-      } else if (o.kind == 1) {
-        f[3] += 1.f / 32;
-      } else if (o.kind == 2) {
-        f[4] += 1.f / 16;
-        f[6] += float(1u << o.k) / 2048;
-      } else {
-        f[5] += 1.f / 64;
-      }
-    }
-    f[7] = float(pa.tab_entries) / 2048;
-    f[8] = pa.kmax == 5 ? 1.f : 0.f;
-    f[9] = pa.kmax == 6 ? 1.f : 0.f;
-  }
+  for (size_t i = 0; i < plan.passes.size(); ++i)
+    mt_pass_features(plan, i, c.n, rows->data() + i * kMtFeatures);
   other->clear();
   for (Launch& l : plan.launches)
     if (l.kind != Launch::kMfmaTile) other->push_back(std::move(l));
@@ -569,7 +573,7 @@ int qsvh_plan_form(qsvh_ctx* h, qsvh_mlp* group_model, qsvh_mlp* tile_model,
   if (tile_model->m.F != kTileFeatures)
     return set_err(&h->c, kInvalidArg, "plan: tile model has 18 inputs");
   if (mt_model && mt_model->m.F != kMtFeatures)
-    return set_err(&h->c, kInvalidArg, "plan: MFMA-tile model has 10 inputs");
+    return set_err(&h->c, kInvalidArg, "plan: MFMA-tile model has 11 inputs");
   qsvh_circuit* res = new (std::nothrow) qsvh_circuit;
   if (!res) return kOom;
   const int rc = plan_form(&h->c, group_model->m, tile_model->m, mt_model ? &mt_model->m : nullptr,

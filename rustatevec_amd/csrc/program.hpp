@@ -88,6 +88,18 @@ struct MtPlan {
   std::vector<uint64_t> order;  // indices into `in` in the order the plan applies them
 };
 void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPlan* out);
+// One MFMA tile pass of the launches `cur` (in order) on the tile of the bits of S filled up
+// to tb bits with the lowest others; appended to `out`.
+void emit_mt_pass(const std::vector<Launch>& in, const std::vector<size_t>& cur, uint32_t n,
+                  uint32_t tb, uint64_t S, MtPlan* out);
+// One exact tile pass of the launches `take` (in order; their action qubits in tmask, a set
+// of tb bits holding bits 0..3), appended to bld / passes / out / order.
+void lower_exact_pass(const std::vector<Launch>& in, const std::vector<uint64_t>& take,
+                      uint32_t n, uint32_t tb, uint64_t tmask, TileBuilder& bld,
+                      std::vector<TilePassArg>& passes, std::vector<Launch>& out,
+                      std::vector<uint64_t>& order);
 void release_extra(Program* p);
+// The launches of a circuit's recorded pass plan (passes.cpp).
+int plan_planned_passes(Program* p, const Circuit& c);
 
 }  // namespace qsvh

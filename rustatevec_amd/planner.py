@@ -19,7 +19,7 @@ from . import (C64, PREC_MFMA_BF16X3, Circuit, HipBackend, Program, QsvhError, _
 
 N_FEATURES = 12
 N_TILE_FEATURES = 18
-N_MT_FEATURES = 10
+N_MT_FEATURES = 11
 HIDDEN = 64
 
 
@@ -49,6 +49,11 @@ def _bind():
         "qsvh_plan_form": (i32, [P, P, P, P, P, i32, u32, u32, PP, C.POINTER(C.c_int),
                                  C.POINTER(C.c_int), C.POINTER(C.c_double),
                                  C.POINTER(C.c_int)]),
+        "qsvh_plan_passes": (i32, [P, P, P, P, i32, PP, C.POINTER(C.c_double)]),
+        "qsvh_circuit_plan_len": (i32, [P, C.POINTER(C.c_size_t)]),
+        "qsvh_circuit_plan_pass": (i32, [P, C.c_size_t, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint64), P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -379,7 +384,79 @@ def plan_form(backend: HipBackend, group_model: Mlp, tile_model: Mlp, circuit: C
     return Circuit(_handle=h), tq.value, mt.value, ms.value, chosen.value
 
 
+FORM_LEARNED = -1
+FORM_EXACT = 0
+
+
+def plan_passes(backend: HipBackend, tile_model: Mlp | None, mt_model: Mlp | None,
+                circuit: Circuit, form=FORM_LEARNED):
+    """(planned circuit, predicted ms): the circuit cut into tile passes, each run exactly or
+    as an MFMA tile pass of its gates fused up to 3, 4 or 5 qubits -- chosen per pass by the
+    two pass cost models (form -1), or one form for every pass (0 exact, k fused at k).  Run
+    the result with ``Program(..., precision=PREC_MFMA_BF16X3, planned=True)``."""
+    h = C.c_void_p()
+    ms = C.c_double(0)
+    _check(_bind().qsvh_plan_passes(backend.ctx, tile_model.handle if tile_model else None,
+                                    mt_model.handle if mt_model else None, circuit.handle,
+                                    int(form), C.byref(h), C.byref(ms)), backend.ctx)
+    return Circuit(_handle=h), ms.value
+
+
+def plan_info(circuit: Circuit, features=False):
+    """Passes of a planned circuit: (form, op_begin, n_ops, tile_mask[, feature row])."""
+    L = _bind()
+    n = C.c_size_t(0)
+    _check(L.qsvh_circuit_plan_len(circuit.handle, C.byref(n)))
+    out = []
+    for i in range(n.value):
+        form = C.c_int(0)
+        b, k, t = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        f = np.zeros(N_TILE_FEATURES, dtype=np.float32)
+        _check(L.qsvh_circuit_plan_pass(circuit.handle, i, C.byref(form), C.byref(b),
+                                        C.byref(k), C.byref(t), _ptr(f) if features else None))
+        row = (form.value, b.value, k.value, t.value)
+        if features:
+            row += (f[: N_TILE_FEATURES if form.value == 0 else N_MT_FEATURES].copy(),)
+        out.append(row)
+    return out
+
+
+def collect_pass_timings(backend: HipBackend, sizes=(22, 24), seeds=range(1000, 1004),
+                         forms=(FORM_EXACT, 3, 4, 5), reps=3):
+    """Time every pass of planned random-mix circuits (the planner workload's gate mix, 40 n
+    gates) run in each fixed form, one kernel per pass; returns (X_tile, y_tile, X_mt, y_mt):
+    the exact passes' and the MFMA passes' feature rows and y = log2 of the device
+    milliseconds scaled to 2^30 amplitudes."""
+    from . import mix
+    Xt, yt, Xm, ym = [], [], [], []
+    for n in sizes:
+        st = backend.alloc(n, C64)
+        for seed in seeds:
+            circ = mix(n, 40 * n, seed)
+            for form in forms:
+                planned, _ = plan_passes(backend, None, None, circ, form)
+                info = plan_info(planned, features=True)
+                p = Program(st, planned, precision=PREC_MFMA_BF16X3, planned=True)
+                backend.init_basis(st, 0)
+                p.launch()
+                ts = np.min([p.time_kernels() for _ in range(reps)], axis=0)
+                del p
+                assert len(ts) == len(info)
+                for (f, _, _, _, feat), ms in zip(info, ts):
+                    y = np.log2(ms * 2.0 ** (30 - n))
+                    if f == 0:
+                        Xt.append(feat)
+                        yt.append(y)
+                    else:
+                        Xm.append(feat)
+                        ym.append(y)
+        st.free()
+    return (np.array(Xt, dtype=np.float32), np.array(yt, dtype=np.float32),
+            np.array(Xm, dtype=np.float32), np.array(ym, dtype=np.float32))
+
+
 __all__ = ["Mlp", "group_features", "collect_timings", "train_cost_model", "plan",
            "tile_pass_features", "collect_tile_timings", "train_tile_model", "plan_form",
            "N_MT_FEATURES", "mt_pass_features", "collect_mt_timings", "train_mt_model",
+           "plan_passes", "plan_info", "collect_pass_timings", "FORM_LEARNED", "FORM_EXACT",
            "QsvhError"]

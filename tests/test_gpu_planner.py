@@ -1,6 +1,8 @@
 """The learned planner: the cost model's device forward/backward against the float64
 numpy restatement (oracle/mlp.py), device training, and plans that are correct (judged
-against the complex128 plain run) and no slower than greedy fusion.
+against the complex128 plain run), faster than greedy fusion at the reference's default
+width of 4 qubits and within 5-8% of the best fixed width or form (the per-pass planner's
+gains over every fixed form are tested in test_gpu_passes.py).
 
 Tolerances: FP32 network vs float64 oracle -- forward 1e-4 relative, gradients 2e-3
 relative to the largest gradient component."""
@@ -119,7 +121,11 @@ def test_planner_gain_over_greedy(backend, cost_model):
     print(f"greedy ms {greedy}; planned {t_plan:.3f} ms (predicted {pred:.3f}, "
           f"chosen {chosen}); gain vs greedy k=4: {greedy[4] / t_plan:.2f}x, "
           f"vs best greedy k={best_k}: {greedy[best_k] / t_plan:.2f}x")
-    assert t_plan <= 1.15 * greedy[best_k]
+    # the gain over the reference's default width (fusion.rs:31-34); against the best fixed
+    # width the dynamic programme only matches it (the per-pass planner, test_gpu_passes.py,
+    # is where the learned models beat every fixed form)
+    assert t_plan <= 0.85 * greedy[4]
+    assert t_plan <= 1.05 * greedy[best_k]
     st.free()
 
 
@@ -160,7 +166,8 @@ def test_form_planner_picks_the_faster_form(backend, cost_model, tile_model):
     best = min(min(greedy.values()), min(tiled.values()))
     print(f"form plan: tile_qubits {tq}, {t_plan:.3f} ms (predicted {pred:.3f}); greedy "
           f"{greedy}; tiled {tiled}")
-    assert t_plan <= 1.15 * best
+    assert t_plan <= 0.5 * greedy[4]  # gain over the reference's default fusion width
+    assert t_plan <= 1.08 * best
     st.free()
     ref.free()
 
@@ -205,7 +212,8 @@ def test_form_planner_with_mfma_tiles(backend, cost_model, tile_model, mt_model)
         forms[f"fused_k{k}"] = run(rs.fuse(backend, circ, k))[0]
     print(f"form plan with MFMA tiles: chosen {chosen}, {t_plan:.3f} ms (predicted "
           f"{pred:.3f}); fixed forms {forms}")
-    assert t_plan <= 1.15 * min(forms.values())
+    assert t_plan <= 0.5 * forms["fused_k4"]  # gain over the reference's default width
+    assert t_plan <= 1.08 * min(forms.values())
     st.free()
     ref.free()
 

@@ -38,6 +38,7 @@ def test_one_pass_counts_dense_merged_and_table_entries():
     want[6] = 4 / 2048        # per-tile table of the head (2 tile bits)
     want[7] = 6 / 2048        # the pass's phase table
     want[8] = 1.0             # widest group: 5 qubits
+    want[10] = 14 / 32        # the state size
     np.testing.assert_allclose(rows[0], want, rtol=1e-6)
 
 
@@ -62,6 +63,7 @@ def test_passes_split_at_the_tile_and_small_gates_widen():
     want = np.zeros(planner.N_MT_FEATURES, dtype=np.float32)
     want[2] = 1 / 16
     want[9] = 1.0
+    want[10] = 20 / 32
     np.testing.assert_allclose(second, want, rtol=1e-6)
 
 
@@ -88,12 +90,14 @@ def test_ops_that_commute_with_the_ops_left_behind_move_ahead():
     first[3] = 1 / 32      # the diagonal on {6, 18, 19}
     first[7] = 8 / 2048
     first[9] = 1.0
+    first[10] = 20 / 32
     second = np.zeros(planner.N_MT_FEATURES, dtype=np.float32)
     second[0] = 2 / 16     # {0..3} and {18, 19} widened
     second[2] = 1 / 16     # {12..17}
     second[3] = 1 / 32     # the diagonal on {17, 0}
     second[7] = 4 / 2048
     second[9] = 1.0
+    second[10] = 20 / 32
     np.testing.assert_allclose(rows[0], first, rtol=1e-6)
     np.testing.assert_allclose(rows[1], second, rtol=1e-6)
 
