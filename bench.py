@@ -114,6 +114,12 @@ def main():
     ap.add_argument("--qft-tile-only", action="store_true",
                     help="only 2 launches of QFT(30) complex128 as exact 2^12-amplitude tile "
                          "passes from the random-normal state, for profiler runs")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only 3 runs of the batched sweep (512 HEA circuits, 10..22 qubits, "
+                         "complex64) at the autotuned shape, for profiler runs")
+    ap.add_argument("--passes-only", action="store_true",
+                    help="only 3 launches of the random mix circuit (28 qubits, 1120 gates) as "
+                         "pass-planned MFMA tile passes fused at k = 4, c64, for profiler runs")
     ap.add_argument("--layered-mt-only", action="store_true",
                     help="only 3 launches of layered(28) depth 20 fused at k = 4 as split-BF16 "
                          "MFMA tile passes (2^12-amplitude tiles), c64, from |0>, for profiler runs")
@@ -144,6 +150,30 @@ def main():
         ms = [round(p.time(), 3) for _ in range(3)]
         print(json.dumps({"qft30_fused_k4_mfma_tiles12_ms": ms, "passes": p.num_kernels()}),
               flush=True)
+        return
+
+    if args.batch_only:
+        b = rs.HipBackend(0)
+        batch = rs.Batch(b, rs.hea_batch(512, 8, 0xC0FFEE), rs.C64)
+        tb_best, bl_best, _ = batch.autotune()
+        batch.run()
+        ms = [round(batch.time(), 3) for _ in range(3)]
+        print(json.dumps({"batch_ms": ms, "tile_qubits": tb_best, "blocks": bl_best,
+                          "launches": batch.info()["launches"],
+                          "hbm_bytes_per_run": batch.hbm_bytes()}), flush=True)
+        return
+
+    if args.passes_only:
+        from rustatevec_amd import planner
+        b = rs.HipBackend(0)
+        st = b.alloc(28, rs.C64)
+        pc, _ = planner.plan_passes(b, None, None, rs.mix(28, 40 * 28, 0xC0FFEE), 4)
+        p = rs.Program(st, pc, precision=rs.PREC_MFMA_BF16X3, planned=True)
+        ms = []
+        for _ in range(3):
+            b.init_basis(st, 0)
+            ms.append(round(p.time(), 3))
+        print(json.dumps({"mix28_passes_k4_ms": ms, "passes": p.num_kernels()}), flush=True)
         return
 
     if args.layered_mt_only:
@@ -366,6 +396,15 @@ def main():
                  "blocks": bl_best, "launches": batch.info()["launches"],
                  "batch_ms": round(b_ms, 3),
                  "plain_pass_equiv_gbs": round(equiv / (b_ms * 1e-3) / 1e9, 1),
+                 # HBM: each pass writes its circuit's state, each but the first reads it
+                 # (circuits that fit one tile are built and run in LDS, written once)
+                 "roofline": {"bound": "hbm", "achieved": round(batch.hbm_bytes()
+                                                                / (b_ms * 1e-3) / 1e9, 1),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(batch.hbm_bytes() / (b_ms * 1e-3) / 1e9
+                                            / HBM_PEAK_GBS, 4),
+                              "traffic": batch.hbm_bytes(),
+                              "algorithmic_bytes_per_run": batch.hbm_bytes()},
                  "z0_mean": float(z0.mean()), "z0_first": [round(float(x), 6) for x in z0[:4]],
                  "sampled_prob_sums": [round(x, 6) for x in psums]}
     log(f"batched HEA x{count}: {batch_rec}")
@@ -391,6 +430,76 @@ def main():
                                         per_size=300, seed=seed)
     mt_model, mt_loss = planner.train_mt_model(b, XM, yM, epochs=2000)
     t_mt_model = time.perf_counter() - t0
+
+    # the pass planner's models: exact / MFMA pass timings of mix circuits (training seeds
+    # 1000..1003, none of the measured circuits), HBM-resident sizes
+    t0 = time.perf_counter()
+    Xpt, ypt, Xpm, ypm = planner.collect_pass_timings(
+        b, sizes=(22, 24) if args.quick else (24, 26, 28), seeds=range(1000, 1004))
+    pt_model, pt_loss = planner.train_tile_model(b, Xpt, ypt, epochs=3000)
+    pm_model, pm_loss = planner.train_mt_model(b, Xpm, ypm, epochs=3000)
+    t_pass_models = time.perf_counter() - t0
+
+    def pass_legs(nq_, circ, ref_, st_, plain_s_, reset_, fixed_forms):
+        """The learned pass plan against fixed forms (each timed and judged against the
+        plain complex128 result ref_) and the uniform forms over the same passes."""
+        t0_ = time.perf_counter()
+        pc_, pred_ = planner.plan_passes(b, pt_model, pm_model, circ, planner.FORM_LEARNED)
+        plan_s_ = time.perf_counter() - t0_
+        pforms = [f for f, _, _, _ in planner.plan_info(pc_)]
+        runs = [("learned_passes", pc_, {"precision": rs.PREC_MFMA_BF16X3, "planned": True})]
+        for f in (0, 3, 4, 5):
+            pcf, _ = planner.plan_passes(b, None, None, circ, f)
+            runs.append((f"passes_uniform_{f}", pcf,
+                         {"precision": rs.PREC_MFMA_BF16X3, "planned": True}))
+        runs += fixed_forms
+        legs_ = {}
+        for name, c_, kw in runs:
+            p_ = rs.Program(st_, c_, **kw)
+            s_ = timed_steps(b, st_, p_, 2 if nq_ <= 28 else 1, 1, reset_)
+            reset_()
+            p_.launch()
+            err_, fid_ = b.compare(ref_, st_)
+            legs_[name] = {"passes": p_.num_kernels(), "circuit_s": round(s_, 5),
+                           "speedup_vs_plain_c128": round(plain_s_ / s_, 2),
+                           "max_abs_err": err_, "one_minus_fidelity": 1 - fid_}
+            log(f"  mix({nq_}) {name}: {s_:.4f} s, err {err_:.1e}")
+            del p_
+        lp = legs_["learned_passes"]
+        fixed = {k: v["circuit_s"] for k, v in legs_.items()
+                 if not k.startswith("passes_") and k != "learned_passes"}
+        unif = {k: v["circuit_s"] for k, v in legs_.items() if k.startswith("passes_uniform")}
+        best_f = min(fixed, key=fixed.get)
+        best_u = min(unif, key=unif.get)
+        # every pass reads and writes the state once (rocprof: 1.00x the algorithmic bytes)
+        hbm = lp["passes"] * 2 * (1 << nq_) * 8
+        return {"qubits": nq_, "gates": len(circ), "plan_s": round(plan_s_, 3),
+                "predicted_ms": round(pred_, 3),
+                "pass_forms": {str(k): pforms.count(k) for k in sorted(set(pforms))},
+                "legs": legs_,
+                "gain_vs_greedy_k4": round(fixed["greedy_k4"] / lp["circuit_s"], 3),
+                "best_fixed_form": best_f,
+                "gain_vs_best_fixed_form": round(fixed[best_f] / lp["circuit_s"], 3),
+                "best_uniform_pass_form": best_u,
+                "gain_vs_best_uniform_pass_form": round(unif[best_u] / lp["circuit_s"], 3),
+                "roofline": {"bound": "lds/valu (tile passes)",
+                             "achieved": round(hbm / lp["circuit_s"] / 1e9, 1),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(hbm / lp["circuit_s"] / 1e9 / HBM_PEAK_GBS, 4),
+                             "traffic": hbm}}
+
+    def fixed_forms_of(circ, full):
+        """The fixed forms the pass planner is judged against: greedy fusion at 4/5/6
+        (per-group matrix-core launches), exact tile passes at 11..13 bits, greedy fusion at
+        4/5 in MFMA tile passes (full=False: greedy 4, tiles 13 and MFMA tiles 4 only)."""
+        mf = rs.PREC_MFMA_BF16X3
+        out_ = [("greedy_k%d" % k, rs.fuse(b, circ, k), {"precision": mf})
+                for k in ((4, 5, 6) if full else (4,))]
+        out_ += [("tiles_t%d" % t, circ, {"tile_qubits": t})
+                 for t in ((11, 12, 13) if full else (13,))]
+        out_ += [("mfma_tiles_k%d" % k, rs.fuse(b, circ, k), {"precision": mf, "mfma_tiles": 12})
+                 for k in ((4, 5) if full else (4,))]
+        return out_
 
     def form_name(tq, chosen):
         if chosen >= 100:
@@ -453,6 +562,21 @@ def main():
                                     "gain_vs_greedy_k6": round(
                                         legs["greedy_k6"]["circuit_s"]
                                         / legs["learned_form"]["circuit_s"], 3)}}
+    # the learned pass planner on the same circuit
+    pass_recs = [pass_legs(nq, mixc, ref, st64, mix_plain_s, reset64,
+                           fixed_forms_of(mixc, True))]
+    learned_rec["pass_planner"] = {"exact_timings": int(len(Xpt)), "mfma_timings": int(len(Xpm)),
+                                   "tile_train_mse_std": round(pt_loss, 4),
+                                   "mt_train_mse_std": round(pm_loss, 4),
+                                   "model_s": round(t_pass_models, 2), "sizes": pass_recs}
+    # the learned form's HBM rate (each pass reads and writes the state once)
+    lf = legs["learned_form"]
+    lf_bytes = lf["passes"] * 2 * (1 << nq) * 8
+    lf["roofline"] = {"bound": "lds/valu (tile passes)",
+                      "achieved": round(lf_bytes / lf["circuit_s"] / 1e9, 1),
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(lf_bytes / lf["circuit_s"] / 1e9 / HBM_PEAK_GBS, 4),
+                      "traffic": lf_bytes}
     log(f"learned planner: {learned_rec}")
     st64.free()
     ref.free()
@@ -466,6 +590,7 @@ def main():
         resetr = lambda: b.init_basis(ref, 0)  # noqa: E731
         p = rs.Program(ref, mc)
         plain_s = timed_steps(b, ref, p, 1, 0, resetr)
+        log(f"  mix({nq2}) plain complex128: {plain_s:.3f} s")
         resetr()
         p.launch()
         del p
@@ -486,9 +611,13 @@ def main():
             reset64()
             p.launch()
             err, fid = b.compare(ref, st64)
+            log(f"  mix({nq2}) {name}: {s_:.4f} s, err {err:.1e}")
             rec[name] = {"circuit_s": round(s_, 5), "speedup_vs_plain_c128": round(plain_s / s_, 2),
                          "max_abs_err": err, "one_minus_fidelity": 1 - fid}
             del p
+        pr = pass_legs(nq2, mc, ref, st64, plain_s, reset64, fixed_forms_of(mc, nq2 <= 28))
+        log(f"learned pass planner at {nq2} qubits: {pr}")
+        learned_rec["pass_planner"]["sizes"].append(pr)
         log(f"learned planner at {nq2} qubits: {rec}")
         sweep.append(rec)
         st64.free()

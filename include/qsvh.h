@@ -312,6 +312,9 @@ int qsvh_batch_run(qsvh_batch* b);
 int qsvh_batch_time(qsvh_batch* b, double* ms);
 int qsvh_batch_info(const qsvh_batch* b, size_t* count, int* tile_qubits, int* blocks,
                     uint64_t* n_launches, uint64_t* total_amplitudes);
+/* Passes of circuit i (the launches that run it; the first builds |0...0> in LDS instead
+ * of reading the state, so a run moves (2 passes - 1) x 2^n_i amplitudes through HBM). */
+int qsvh_batch_passes(const qsvh_batch* b, size_t i, uint32_t* passes);
 /* <Z_0> of every circuit after the last run (out[count]). */
 int qsvh_batch_expectation_z0(qsvh_batch* b, double* out);
 /* Final state / probabilities of circuit i (host buffers of 2^n_i elements). */

@@ -124,6 +124,7 @@ def lib() -> C.CDLL:
         "qsvh_batch_autotune": (i32, [P, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_double)]),
         "qsvh_batch_run": (i32, [P]),
+        "qsvh_batch_passes": (i32, [P, C.c_size_t, C.POINTER(C.c_uint32)]),
         "qsvh_batch_time": (i32, [P, C.POINTER(C.c_double)]),
         "qsvh_batch_info": (i32, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
                                   C.POINTER(C.c_int), C.POINTER(C.c_uint64),
@@ -625,6 +626,19 @@ class Batch:
                                      C.byref(nl), C.byref(amps)))
         return {"count": n.value, "tile_qubits": t.value, "blocks": bl.value,
                 "launches": nl.value, "amplitudes": amps.value}
+
+    def passes(self, i: int) -> int:
+        """Passes of circuit i (its first builds |0...0> in LDS without reading HBM)."""
+        n = C.c_uint32(0)
+        _check(lib().qsvh_batch_passes(self._h, i, C.byref(n)))
+        return n.value
+
+    def hbm_bytes(self) -> int:
+        """Algorithmic HBM bytes of one run: every pass writes its circuit's state, every
+        pass but the first reads it ((2 passes - 1) x 2^n x bytes per amplitude)."""
+        amp = 16 if self.dtype == C128 else 8
+        return sum((2 * self.passes(i) - 1) * (1 << c.n_qubits) * amp
+                   for i, c in enumerate(self.circuits))
 
     def expectation_z0(self) -> np.ndarray:
         out = np.zeros(len(self.circuits))

@@ -356,6 +356,21 @@ int qsvh_batch_time(qsvh_batch* bh, double* ms) {
   return timed(&b, ms);
 }
 
+int qsvh_batch_passes(const qsvh_batch* bh, size_t i, uint32_t* passes) {
+  if (!bh || !passes) return kInvalidArg;
+  const Batch& b = bh->b;
+  if (i >= b.circuits.size()) return kInvalidArg;
+  uint32_t n = 0;
+  for (const auto& L : b.launches)
+    for (const BatchPass& p : L)
+      if (p.state_off == b.off[i]) {
+        ++n;
+        break;
+      }
+  *passes = n;
+  return kOk;
+}
+
 int qsvh_batch_info(const qsvh_batch* bh, size_t* count, int* tile_qubits, int* blocks,
                     uint64_t* n_launches, uint64_t* total_amplitudes) {
   if (!bh) return kInvalidArg;
