@@ -417,7 +417,7 @@ class Program:
 
 
 TUNE_KEYS = ("max_blocks", "unroll", "nontemporal", "mfma_blocks", "mfma_layout",
-             "tile_blocks", "contig", "xcd")
+             "tile_blocks", "contig", "xcd", "mq_lane", "diag_blocks")
 
 
 class HipBackend:

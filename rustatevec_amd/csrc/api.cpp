@@ -100,6 +100,8 @@ hipError_t issue(const Program& p, const Launch& l, const Tuning& t, hipStream_t
       return launch_ctrl1q(st, l.m, l.q, l.fix, l.mat.data(), t, s);
     case Launch::kDiagSub:
       return launch_diag_sub(st, l.m, l.q, l.fix, l.mat.data(), t, s);
+    case Launch::kSwapSub:
+      return launch_swap_sub(st, l.q, t, s);
     default:
       return hipErrorInvalidValue;
   }
@@ -337,6 +339,10 @@ int qsvh_tune_set(qsvh_ctx* h, const char* key, int64_t value) {
     t.contig = int(value);
   } else if (k == "xcd" && (value == 0 || value == 1)) {
     t.xcd = int(value);
+  } else if (k == "mq_lane" && (value == 0 || value == 1)) {
+    t.mq_lane = int(value);
+  } else if (k == "diag_blocks" && value >= 0 && value <= (1ll << 20)) {
+    t.diag_blocks = int(value);
   } else {
     return set_err(&h->c, kInvalidArg, "unknown tuning key or value: " + k);
   }
@@ -355,6 +361,8 @@ int qsvh_tune_get(qsvh_ctx* h, const char* key, int64_t* value) {
   else if (k == "tile_blocks") *value = t.tile_blocks;
   else if (k == "contig") *value = t.contig;
   else if (k == "xcd") *value = t.xcd;
+  else if (k == "mq_lane") *value = t.mq_lane;
+  else if (k == "diag_blocks") *value = t.diag_blocks;
   else return set_err(&h->c, kInvalidArg, "unknown tuning key: " + k);
   return kOk;
 }

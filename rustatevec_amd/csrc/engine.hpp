@@ -29,6 +29,12 @@ struct Tuning {
   // 1: blocks b and b + 1 of a streaming launch take neighbouring address ranges of one
   // XCD (the hardware deals blocks to the 8 XCDs round robin, b -> XCD b % 8).
   int xcd = 1;
+  // 1: dense 2- and 3-qubit gates with one 16-byte chunk per lane (k_mq_lane); 0: one
+  // block of 2^m amplitudes per thread (k_mq_reg)
+  int mq_lane = 1;
+  // workgroups of the LDS-table diagonal kernel (k_diag, m >= 3): 0 one thread per work
+  // item (the phase table staged by every workgroup); n > 0 a grid-stride loop over n
+  int diag_blocks = 0;
 };
 
 struct Ctx {
@@ -85,7 +91,7 @@ struct MqIndex {
 // subspace); kDiagSub: the phase mat[0..1] on the amplitudes whose bits q[0..m) equal
 // `fix`'s (the one non-unit entry of a diagonal gate).
 struct Launch {
-  enum Kind { k1q, kDiag, kMq, kFusedMfma, kTile, kCtrl1q, kDiagSub, kMfmaTile } kind;
+  enum Kind { k1q, kDiag, kMq, kFusedMfma, kTile, kCtrl1q, kDiagSub, kMfmaTile, kSwapSub } kind;
   uint32_t m = 0;
   uint32_t q[kMaxGateQubits] = {0, 0, 0, 0, 0, 0};
   // matrix in f64 (interleaved re, im), row-major; for diagonal launches the 2^m phases.
@@ -111,6 +117,8 @@ hipError_t launch_ctrl1q(const State& st, uint32_t m, const uint32_t* q, uint64_
                          const double* g, const Tuning& t, hipStream_t s);
 hipError_t launch_diag_sub(const State& st, uint32_t m, const uint32_t* q, uint64_t fix,
                            const double* phase, const Tuning& t, hipStream_t s);
+// kSwapSub: SWAP(q[0], q[1]) on its |01>, |10> amplitudes only.
+hipError_t launch_swap_sub(const State& st, const uint32_t* q, const Tuning& t, hipStream_t s);
 hipError_t launch_probs(const State& st, void* out, hipStream_t s);
 // f64 sum of |psi_i|^2 * sign(i & mask) (mask 0: the norm) into *d_result.
 hipError_t launch_reduce(const State& st, uint64_t mask, double* d_partials, int max_partials,

@@ -276,7 +276,7 @@ struct MatArg {
 // mq_block (cpu.rs:221-247) for m = 2, 3 with the matrix in kernel arguments (scalar
 // registers).  A work item is VW consecutive blocks o..o+VW-1; when the lowest gate qubit
 // is >= log2(VW) their amplitudes are consecutive, so every gather is one 16-byte chunk.
-template <typename T, int M, int VW>
+template <typename T, int M, int VW, bool NT>
 __global__ __launch_bounds__(kBlock) void k_mq_reg(T* __restrict__ amp, uint64_t n_items,
                                                    MqIdx<M> idx, MatArg<T, M> g, int xcd) {
   constexpr int D = 1 << M;
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_mq_reg(T* __restrict__ amp, uint64_t
     const uint64_t base = insert_zero_bits<M>(v * VW, idx.sorted);
     T ar[D][VW], ai[D][VW];
 #pragma unroll
-    for (int s = 0; s < D; ++s) cload<T, VW>(amp, base + idx.off[s], ar[s], ai[s]);
+    for (int s = 0; s < D; ++s) cload<T, VW, NT>(amp, base + idx.off[s], ar[s], ai[s]);
 #pragma unroll
     for (int r = 0; r < D; ++r) {
       T yr[VW], yi[VW];
@@ -303,8 +303,113 @@ __global__ __launch_bounds__(kBlock) void k_mq_reg(T* __restrict__ amp, uint64_t
         yr[w] = accr;
         yi[w] = acci;
       }
-      cstore<T, VW>(amp, base + idx.off[r], yr, yi);
+      cstore<T, VW, NT>(amp, base + idx.off[r], yr, yi);
     }
+  }
+}
+
+// mq_block for m = 2, 3 with one 16-byte chunk per lane (the access pattern of the
+// diagonal and k_1q_lane kernels: one load and one store per lane).  A wave covers 64 >> M
+// consecutive work items (A consecutive blocks each, as in k_mq_reg: the lowest gate qubit
+// is >= log2(A)); its lanes are 2^M groups of 64 >> M, group s loading member s (internal
+// sub-index s, offset scatter_bits(s, q)) of every item.  A lane gathers the other members
+// of its item from the lanes 8, 16 and 32 apart (DPP row rotate by 8, v_permlane16_swap,
+// v_permlane32_swap) and computes its own output row s with k_mq_reg's operations in the
+// same order, so results are bitwise those of k_mq_reg.
+template <typename T>
+__device__ __forceinline__ void swap_rows(uint32_t v, uint32_t& lo, uint32_t& hi, int bit) {
+  // the value of lane (lane & ~2^bit) into lo and of lane (lane | 2^bit) into hi
+  if (bit == 5) {
+    const auto sw = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    lo = sw[0];
+    hi = sw[1];
+  } else {
+    const auto sw = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    lo = sw[0];
+    hi = sw[1];
+  }
+}
+
+template <typename T, int M, int A, bool NT>
+__global__ __launch_bounds__(kBlock) void k_mq_lane(T* __restrict__ amp, uint64_t n_waves,
+                                                    MqIdx<M> idx, MatArg<T, M> g, int xcd) {
+  constexpr int D = 1 << M;
+  constexpr int GL = 64 >> M;          // lanes per member group = items per wave
+  constexpr int W = 4;                 // 32-bit words per 16-byte chunk
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t s = lane / GL;        // this lane's member (and output row)
+  // this lane's row of the gate (kernel arguments selected once per lane)
+  T er[D], ei[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    T r = g.r[c], i = g.i[c];
+#pragma unroll
+    for (int rr = 1; rr < D; ++rr)
+      if (s == uint32_t(rr)) {
+        r = g.r[rr * D + c];
+        i = g.i[rr * D + c];
+      }
+    er[c] = r;
+    ei[c] = i;
+  }
+  uint64_t my_off = idx.off[0];
+#pragma unroll
+  for (int rr = 1; rr < D; ++rr)
+    if (s == uint32_t(rr)) my_off = idx.off[rr];
+  const uint64_t nw = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+  for (uint64_t w = (xcd_block(xcd) * blockDim.x + threadIdx.x) >> 6; w < n_waves; w += nw) {
+    const uint64_t item = w * GL + (lane % GL);
+    const uint64_t a = insert_zero_bits<M>(item * A, idx.sorted) + my_off;
+    T xr[A], xi[A];
+    cload<T, A, NT>(amp, a, xr, xi);
+    uint32_t v[W];
+    __builtin_memcpy(v, xr, 8);
+    __builtin_memcpy(v + 2, xi, 8);
+    // x[c][k]: word k of member c's chunk
+    uint32_t x[D][W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      uint32_t y[D];
+      if constexpr (M == 3) {
+        // lanes 8 apart (member bit 0): the row-rotated copy, ordered by the lane's bit 3
+        const uint32_t o = uint32_t(__builtin_amdgcn_update_dpp(int(v[k]), int(v[k]), 0x128,
+                                                                0xF, 0xF, true));
+        const bool b3 = (lane >> 3) & 1u;
+        const uint32_t e0 = b3 ? o : v[k], e1 = b3 ? v[k] : o;
+        uint32_t a0, a1, b0, b1;
+        swap_rows<T>(e0, a0, a1, 4);  // member bit 1
+        swap_rows<T>(e1, b0, b1, 4);
+        swap_rows<T>(a0, y[0], y[4], 5);  // member bit 2
+        swap_rows<T>(b0, y[1], y[5], 5);
+        swap_rows<T>(a1, y[2], y[6], 5);
+        swap_rows<T>(b1, y[3], y[7], 5);
+      } else {
+        uint32_t e0, e1;
+        swap_rows<T>(v[k], e0, e1, 4);  // member bit 0
+        swap_rows<T>(e0, y[0], y[2], 5);  // member bit 1
+        swap_rows<T>(e1, y[1], y[3], 5);
+      }
+#pragma unroll
+      for (int c = 0; c < D; ++c) x[c][k] = y[c];
+    }
+    T yr[A], yi[A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+      T accr = T(0), acci = T(0);
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        T cr[A], ci[A];
+        __builtin_memcpy(cr, x[c], 8);
+        __builtin_memcpy(ci, x[c] + 2, 8);
+        const T pr = er[c] * cr[j] - ei[c] * ci[j];
+        const T pi = er[c] * ci[j] + ei[c] * cr[j];
+        accr = accr + pr;
+        acci = acci + pi;
+      }
+      yr[j] = accr;
+      yi[j] = acci;
+    }
+    cstore<T, A, NT>(amp, a, yr, yi);
   }
 }
 
@@ -495,6 +600,25 @@ __global__ __launch_bounds__(kBlock) void k_ctrl1q(T* __restrict__ amp, uint64_t
   }
 }
 
+// SWAP on its |01>, |10> subspace: amplitudes i + 2^a and i + 2^b (i with both bits clear)
+// trade places; the |00> and |11> amplitudes, which the dense product (mq_block,
+// cpu.rs:221-247) returns unchanged, are not touched -- half the bytes of the full pass,
+// the same values (each output is one input times exactly 1 plus exact zeros).
+template <typename T, int A, bool NT>
+__global__ __launch_bounds__(kBlock) void k_swap_sub(T* __restrict__ amp, uint64_t n_items,
+                                                     SubIdx a, uint64_t da, uint64_t db,
+                                                     int xcd) {
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v = xcd_block(xcd) * blockDim.x + threadIdx.x; v < n_items; v += nthr) {
+    const uint64_t i = sub_index(v * A, a);
+    T ar[A], ai[A], br[A], bi[A];
+    cload<T, A, NT>(amp, i + da, ar, ai);
+    cload<T, A, NT>(amp, i + db, br, bi);
+    cstore<T, A, NT>(amp, i + da, br, bi);
+    cstore<T, A, NT>(amp, i + db, ar, ai);
+  }
+}
+
 // The one non-unit phase of a diagonal gate on its amplitudes (diag_chunk's multiply,
 // cpu.rs:99-114; the others would be multiplied by exactly 1 + 0i).
 template <typename T, int A>
@@ -634,8 +758,9 @@ hipError_t launch_diag_any(const State& st, uint32_t m, const uint32_t* q, const
     constexpr bool NT = decltype(ntc)::value;
     if (st.dim() >= uint64_t(A)) {
       const uint64_t items = st.dim() / A;
-      k_diag<T, A, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(amp, items, a, t.contig,
-                                                                  t.xcd);
+      unsigned grid = grid_for(items, U, t);
+      if (t.diag_blocks > 0 && grid > unsigned(t.diag_blocks)) grid = unsigned(t.diag_blocks);
+      k_diag<T, A, U, NT><<<grid, kBlock, 0, s>>>(amp, items, a, t.contig, t.xcd);
     } else {
       const uint64_t items = st.dim();
       k_diag<T, 1, U, NT><<<grid_for(items, U, t), kBlock, 0, s>>>(amp, items, a, t.contig,
@@ -677,11 +802,33 @@ hipError_t launch_mq_reg(const State& st, const uint32_t* q, const double* gd,
   T* amp = static_cast<T*>(st.amp);
   constexpr int A = amps_per_chunk<T>();
   const uint64_t blocks = st.dim() >> M;
-  if (int(idx.sorted[0]) >= ilog2(A) && blocks >= uint64_t(A)) {
+  constexpr uint64_t GL = 64 >> M;
+  // non-temporal only when every load of a wave covers whole 128-byte lines (the lowest gate
+  // qubit at or above 128 B / amplitude size): on partial lines it cost 2-6x (CX(0, 1) at
+  // 30 qubits: 4.7 -> 1.5 TB/s), on whole ones it gains 4-7%
+  const bool nt = nontemporal_for(st, t, true) &&
+                  (uint64_t(1) << idx.sorted[0]) * 2 * sizeof(T) >= 128;
+  // the lane form measured faster only for complex64 2-qubit gates (5.8-6.0 against
+  // 5.4-5.7 TB/s at 30 qubits); for complex128 its exchanges cost more than they save
+  if (t.mq_lane && sizeof(T) == 4 && M == 2 && int(idx.sorted[0]) >= ilog2(A) &&
+      blocks % (A * GL) == 0 && blocks > 0) {
+    const uint64_t waves = blocks / A / GL;
+    const unsigned grid = grid_for(waves * 64, 1, t);
+    if (nt)
+      k_mq_lane<T, M, A, true><<<grid, kBlock, 0, s>>>(amp, waves, idx, g, t.xcd);
+    else
+      k_mq_lane<T, M, A, false><<<grid, kBlock, 0, s>>>(amp, waves, idx, g, t.xcd);
+  } else if (int(idx.sorted[0]) >= ilog2(A) && blocks >= uint64_t(A)) {
     const uint64_t items = blocks / A;
-    k_mq_reg<T, M, A><<<grid_for(items, 1, t), kBlock, 0, s>>>(amp, items, idx, g, t.xcd);
+    if (nt)
+      k_mq_reg<T, M, A, true><<<grid_for(items, 1, t), kBlock, 0, s>>>(amp, items, idx, g, t.xcd);
+    else
+      k_mq_reg<T, M, A, false><<<grid_for(items, 1, t), kBlock, 0, s>>>(amp, items, idx, g, t.xcd);
   } else {
-    k_mq_reg<T, M, 1><<<grid_for(blocks, 1, t), kBlock, 0, s>>>(amp, blocks, idx, g, t.xcd);
+    if (nt)
+      k_mq_reg<T, M, 1, true><<<grid_for(blocks, 1, t), kBlock, 0, s>>>(amp, blocks, idx, g, t.xcd);
+    else
+      k_mq_reg<T, M, 1, false><<<grid_for(blocks, 1, t), kBlock, 0, s>>>(amp, blocks, idx, g, t.xcd);
   }
   return hipGetLastError();
 }
@@ -833,7 +980,37 @@ hipError_t launch_diag_sub_t(const State& st, uint32_t m, const uint32_t* q, uin
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_swap_sub_t(const State& st, const uint32_t* q, const Tuning& t,
+                             hipStream_t s) {
+  uint32_t lowest;
+  const SubIdx a = make_sub(2, q, 0, &lowest);
+  constexpr int A = amps_per_chunk<T>();
+  const uint64_t items = st.dim() >> 2;
+  const uint64_t da = uint64_t(1) << q[0], db = uint64_t(1) << q[1];
+  T* amp = static_cast<T*>(st.amp);
+  const bool nt = nontemporal_for(st, t, true) && (uint64_t(1) << lowest) * 2 * sizeof(T) >= 128;
+  if (int(lowest) >= ilog2(A) && items >= uint64_t(A)) {
+    if (nt)
+      k_swap_sub<T, A, true><<<grid_for(items / A, 1, t), kBlock, 0, s>>>(amp, items / A, a, da,
+                                                                          db, t.xcd);
+    else
+      k_swap_sub<T, A, false><<<grid_for(items / A, 1, t), kBlock, 0, s>>>(amp, items / A, a, da,
+                                                                           db, t.xcd);
+  } else {
+    k_swap_sub<T, 1, false><<<grid_for(items, 1, t), kBlock, 0, s>>>(amp, items, a, da, db,
+                                                                     t.xcd);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_swap_sub(const State& st, const uint32_t* q, const Tuning& t, hipStream_t s) {
+  if (st.n < 2) return hipErrorInvalidValue;
+  return st.dtype == 0 ? launch_swap_sub_t<double>(st, q, t, s)
+                       : launch_swap_sub_t<float>(st, q, t, s);
+}
 
 hipError_t launch_ctrl1q(const State& st, uint32_t m, const uint32_t* q, uint64_t fix,
                          const double* g, const Tuning& t, hipStream_t s) {

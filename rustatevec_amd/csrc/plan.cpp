@@ -593,6 +593,19 @@ int plan_tiles(Program* p, int tile_qubits) {
 // one entry other than exactly 1 + 0i touches only that entry's amplitudes; an m = 2, 3
 // gate that is a 1-qubit gate on one qubit under controls on the others (one control value
 // with a non-identity block, identity blocks elsewhere) touches only that control subspace.
+// Exactly the SWAP matrix (gate.rs:160-168): ones at (0,0), (1,2), (2,1), (3,3), zeros
+// elsewhere (its qubit order does not matter).
+static bool is_swap(const Launch& l) {
+  for (uint32_t r = 0; r < 4; ++r)
+    for (uint32_t c = 0; c < 4; ++c) {
+      const bool one = (r == 0 && c == 0) || (r == 3 && c == 3) || (r == 1 && c == 2) ||
+                       (r == 2 && c == 1);
+      const size_t e = 2 * (size_t(r) * 4 + c);
+      if (l.mat[e] != (one ? 1.0 : 0.0) || l.mat[e + 1] != 0.0) return false;
+    }
+  return true;
+}
+
 void plan_subspace(Program* p) {
   for (Launch& l : p->launches) {
     if (l.kind == Launch::kDiag) {
@@ -610,6 +623,9 @@ void plan_subspace(Program* p) {
       l.kind = Launch::kDiagSub;
       l.fix = fix;
       l.mat = {pr, pi};
+    } else if (l.kind == Launch::kMq && l.m == 2 && is_swap(l)) {
+      l.kind = Launch::kSwapSub;
+      l.mat.clear();
     } else if (l.kind == Launch::kMq && l.m <= 3) {
       const uint32_t D = 1u << l.m;
       auto nz = [&](uint32_t r, uint32_t c) {

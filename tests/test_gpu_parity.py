@@ -158,6 +158,48 @@ def test_1q_lane_and_pair_kernels_agree_on_every_target(backend, dtype):
         backend.tune(unroll=old)
 
 
+def random_unitary(d, rng):
+    q, r = np.linalg.qr(rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+@pytest.mark.parametrize("dtype", [rs.C128, rs.C64])
+def test_mq_lane_and_block_kernels_agree_on_every_placement(backend, dtype):
+    """Dense 2- and 3-qubit gates run with one chunk per lane (members gathered across
+    lanes 8, 16, 32 apart) by default and one block per thread with "mq_lane" 0; both must
+    give the oracle's result bitwise for every ordered placement of the gate's qubits
+    (low, high, scattered, reversed), on states from below one wave of blocks upward."""
+    npt = np.float64 if dtype == rs.C128 else np.float32
+    rng = np.random.default_rng(12)
+    old = backend.tune()["mq_lane"]
+    try:
+        for n in (3, 5, 8, 9, 14):
+            psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)) * 0.01
+            ops = []
+            for a in range(n):
+                b = (a * 5 + 3) % n
+                if b != a:
+                    ops.append((random_unitary(4, rng), [a, b]))
+                    ops.append((oracle.gate_matrix("swap", []), [b, a]))
+                    ops.append((oracle.gate_matrix("cx", []), [a, b]))
+                c = (a * 3 + 1) % n
+                if len({a, b, c}) == 3:
+                    ops.append((random_unitary(8, rng), [c, a, b]))
+                    ops.append((oracle.gate_matrix("ccx", []), [a, b, c]))
+            want = oracle.execute(oracle.CPU_SERIAL, n, ops, dtype=npt, start=psi)
+            for lane in (1, 0):
+                backend.tune(mq_lane=lane)
+                st = backend.alloc(n, dtype)
+                backend.upload(st, psi)
+                for g, qs in ops:
+                    backend.apply(st, g, qs)
+                got = backend.download(st)
+                st.free()
+                assert np.array_equal(got, want), (n, lane)
+    finally:
+        backend.tune(mq_lane=old)
+
+
 def test_upload_download_roundtrip_and_qft_from_random_start(backend, golden):
     z, _ = golden
     v = z["start_random7"]
@@ -385,8 +427,9 @@ def _run_program(backend, circ, dtype, **opts):
 
 @pytest.mark.parametrize("dtype", [rs.C128, rs.C64])
 def test_subspace_launches_match_the_full_passes(backend, dtype):
-    """Program option `subspace`: controlled 1-qubit gates on their control subspace and
-    diagonal gates with one non-unit phase on that phase's amplitudes give the same values
+    """Program option `subspace`: controlled 1-qubit gates on their control subspace,
+    diagonal gates with one non-unit phase on that phase's amplitudes and SWAP on its
+    |01>, |10> amplitudes give the same values
     as the full passes (bit for bit against the oracle's CpuBackend), with the same number
     of launches."""
     npdt = np.float64 if dtype == rs.C128 else np.float32
@@ -403,6 +446,7 @@ def test_subspace_launches_match_the_full_passes(backend, dtype):
     c.push(cu0, [0, 5, 2])
     c.push(np.diag([1, 1j, 1, 1]), [3, 7])  # one non-unit entry at sub-index 1
     c.ccx(8, 1, 0).ccz(2, 6, 7).cx(4, 0).cz(1, 3).cphase(0, 8, 0.4)
+    c.swap(0, 8).swap(4, 2).swap(7, 6).swap(1, 0)  # SWAP on its |01>, |10> amplitudes
     circs.append(c)
     for circ in circs:
         n = circ.n_qubits
