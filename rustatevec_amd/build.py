@@ -68,9 +68,15 @@ def _compile(name: str, flags: list[str], verbose: bool) -> str:
     return obj
 
 
+# Kernels allowed a small spill, measured faster with it: the MFMA tile pass for groups of
+# up to 4 qubits at four waves per SIMD spills a few registers around the merged diagonal
+# tables and still runs 7-10% faster than at three waves without spilling.
+SCRATCH_ALLOWED = {"k_mfma_tilesILi12ELi4E": 32}
+
+
 def _check_no_scratch(name: str, remarks: str) -> None:
     """Fail the build when a kernel of `name` uses private memory (spills or arrays
-    the compiler could not keep in registers)."""
+    the compiler could not keep in registers), beyond the allowances above."""
     func = None
     bad = []
     for line in remarks.splitlines():
@@ -78,7 +84,9 @@ def _check_no_scratch(name: str, remarks: str) -> None:
             func = line.split("Function Name:")[1].split()[0]
         elif "ScratchSize [bytes/lane]:" in line:
             size = int(line.split("ScratchSize [bytes/lane]:")[1].split()[0])
-            if size > 0:
+            allowed = max([v for k, v in SCRATCH_ALLOWED.items() if func and k in func],
+                          default=0)
+            if size > allowed:
                 bad.append(f"{func}: {size} bytes/lane")
     if bad:
         raise RuntimeError(f"{name}: kernels using scratch memory:\n  " + "\n  ".join(bad))

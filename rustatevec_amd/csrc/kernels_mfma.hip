@@ -26,6 +26,7 @@
 // outside the tile fixed per tile) are applied to it in order.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <type_traits>
 
 #include "amp.hpp"
@@ -541,12 +542,11 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
 
   floatx16 acc[BPW];
   uint32_t wb[BPW];  // B rows 16 ks + 8 h + j of block bi
-#pragma unroll
-  for (int bi = 0; bi < BPW; ++bi) {
+  auto init = [&](int bi) {
     acc[bi] = floatx16{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
                        0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     wb[bi] = wl ^ colw(nt0 + bi) ^ (h ? U[3] : 0u);
-  }
+  };
   // one K step of block bi: its B fragment read from LDS, split, three products
   auto step = [&](int bi, int ks) {
     // opaque per step: the step's read addresses are formed here, not all held at once
@@ -572,24 +572,41 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
     // block's reads would be live at once
     asm volatile("" ::: "memory");
   };
-  // block by block (alternating the blocks' steps, to overlap their chains of dependent
-  // products, measured no faster)
-#pragma unroll
-  for (int bi = 0; bi < BPW; ++bi)
-#pragma unroll
-    for (int ks = 0; ks < KSTEPS; ++ks) step(bi, ks);
-  __syncthreads();
-#pragma unroll
-  for (int bi = 0; bi < BPW; ++bi) {
-    uint32_t wb = wl ^ colw(nt0 + bi) ^ (h ? U[2] : 0u);  // output rows ... + 4 h + ...
+  auto write = [&](int bi) {
+    uint32_t wo = wl ^ colw(nt0 + bi) ^ (h ? U[2] : 0u);  // output rows ... + 4 h + ...
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      if ((r & 3) == 0) asm volatile("" : "+v"(wb));
+      if ((r & 3) == 0) asm volatile("" : "+v"(wo));
       const uint32_t rowu = uint32_t(32 * mt + 8 * (r >> 2) + (r & 3));
       float* dst = (rowu >> K) ? xi : xr;
-      *reinterpret_cast<float*>(reinterpret_cast<char*>(dst) + (wb ^ uxor<K>(U, rowu & (D - 1)))) =
+      *reinterpret_cast<float*>(reinterpret_cast<char*>(dst) + (wo ^ uxor<K>(U, rowu & (D - 1)))) =
           acc[bi][r];
     }
+  };
+  if constexpr (R == 32) {
+    // one row band (k = 4): a wave's output blocks are its own columns, which no other wave
+    // reads, so each block is written as soon as it is formed -- no barrier between the
+    // reads and the writes, one block's accumulators live at a time
+#pragma unroll
+    for (int bi = 0; bi < BPW; ++bi) {
+      init(bi);
+#pragma unroll
+      for (int ks = 0; ks < KSTEPS; ++ks) step(bi, ks);
+      write(bi);
+    }
+  } else {
+    // several row bands read every column: all waves read before any writes.  Block by
+    // block (alternating the blocks' steps, to overlap their chains of dependent products,
+    // measured no faster)
+#pragma unroll
+    for (int bi = 0; bi < BPW; ++bi) init(bi);
+#pragma unroll
+    for (int bi = 0; bi < BPW; ++bi)
+#pragma unroll
+      for (int ks = 0; ks < KSTEPS; ++ks) step(bi, ks);
+    __syncthreads();
+#pragma unroll
+    for (int bi = 0; bi < BPW; ++bi) write(bi);
   }
   // no barrier here: whatever reads the tile next starts with one (the next dense group
   // after its A-fragment loads, a diagonal run, the tile's store)
@@ -690,11 +707,13 @@ __device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const float2* 
   }
 }
 
-// Three workgroups per CU for passes of groups up to 5 qubits (their LDS allows it; the
-// register budget of 3 waves per SIMD is asked of the compiler), two with 6-qubit groups
-// (asked for 3, that kernel spills).
+// Waves per SIMD asked of the compiler: four for passes of groups up to 4 qubits (128
+// registers: a few spill around the merged diagonal tables, and the fourth workgroup per CU
+// still gains 7-10% where the pass's phase table leaves it room in LDS), three up to 5
+// qubits, two with 6-qubit groups (asked for 3, that kernel spills).
 template <int TB, int KMAX, bool NTA>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KMAX == 6 ? 2 : 3)))
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(KMAX == 4 ? 4 : (KMAX == 6 ? 2 : 3))))
 void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
                                                        MtPassArg pa,
                                                        const MtOp* __restrict__ ops,
@@ -703,8 +722,10 @@ void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
   constexpr int N = 1 << TB;
   constexpr int CH = N / 2 / kBlock;  // 16-byte chunks (2 amplitudes) per lane
   __shared__ float xr[N], xi[N];
-  __shared__ float2 tabs[kMtTabEntries];
   __shared__ float2 tgroup[2 * 256];  // merged diagonal runs' per-tile tables
+  // the pass's phase table: dynamic LDS of its own size (pa.tab_entries), so passes with
+  // few diagonal phases leave room for another workgroup per CU
+  extern __shared__ float2 tabs[];
   {  // the pass's phase table, once per workgroup
     const float2* src = reinterpret_cast<const float2*>(pool + pa.tab_off);
     for (uint32_t e = threadIdx.x; e < pa.tab_entries; e += kBlock) tabs[e] = src[e];
@@ -774,17 +795,22 @@ hipError_t launch_mt(const State& st, const MtPassArg& pa, const MtOp* ops, cons
   const int cus = st.ctx && st.ctx->cus > 0 ? st.ctx->cus : 256;
   // default grid: as many workgroups as fit on the chip at once (measured best: fewer
   // leave LDS latency unhidden, more queue behind the resident ones)
-  static const int per_cu = [] {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_mfma_tiles<TB, KMAX, NT>, kBlock,
-                                                     0) != hipSuccess || n < 1)
-      n = 1;
-    return n;
-  }();
+  const size_t dyn = size_t(pa.tab_entries > 0 ? pa.tab_entries : 1) * sizeof(float2);
+  // workgroups per CU for this phase-table size (in 512-entry steps), looked up once
+  static std::atomic<int> per_cu_by_size[kMtTabEntries / 512 + 1];
+  const size_t slot = (dyn + 512 * sizeof(float2) - 1) / (512 * sizeof(float2));
+  int per_cu = per_cu_by_size[slot].load(std::memory_order_relaxed);
+  if (per_cu < 1) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mfma_tiles<TB, KMAX, NT>, kBlock,
+                                                     slot * 512 * sizeof(float2)) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    per_cu_by_size[slot].store(per_cu, std::memory_order_relaxed);
+  }
   const uint64_t cap =
       t.mfma_blocks > 0 ? uint64_t(t.mfma_blocks) : uint64_t(per_cu) * uint64_t(cus);
   const uint64_t g = n_tiles < cap ? n_tiles : cap;
-  k_mfma_tiles<TB, KMAX, NT><<<unsigned(g), kBlock, 0, s>>>(
+  k_mfma_tiles<TB, KMAX, NT><<<unsigned(g), kBlock, dyn, s>>>(
       static_cast<float*>(st.amp), n_tiles, pa, ops,
       static_cast<const unsigned char*>(pool), t.xcd);
   return hipGetLastError();
