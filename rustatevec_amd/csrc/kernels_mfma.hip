@@ -713,7 +713,7 @@ __device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const float2* 
 // qubits, two with 6-qubit groups (asked for 3, that kernel spills).
 template <int TB, int KMAX, bool NTA>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(KMAX == 4 ? 4 : (KMAX == 6 ? 2 : 3))))
+__attribute__((amdgpu_waves_per_eu(KMAX == 6 ? 2 : 4)))
 void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
                                                        MtPassArg pa,
                                                        const MtOp* __restrict__ ops,
