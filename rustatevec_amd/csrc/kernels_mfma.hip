@@ -452,10 +452,11 @@ hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, con
 // ======================================================================= MFMA tile passes
 namespace {
 
-// LDS word of tile-local index e: the 5-bit groups of e above bit 4 are XOR-folded into its
-// low 5 bits.  The map is linear over GF(2), so the word of a | b (disjoint bits) is the
-// XOR of the words of a and b, and 32 lanes stepping through 5 tile bits whose positions
-// differ mod 5 hit 32 distinct banks, wherever in the tile those bits are.
+// LDS slot (8 bytes: re, im) of tile-local amplitude e: the 5-bit groups of e above bit 4
+// are XOR-folded into its low 5 bits.  The map is linear over GF(2), so the slot of a | b
+// (disjoint bits) is the XOR of the slots of a and b, and 32 lanes stepping through 5 tile
+// bits whose positions differ mod 5 hit 32 distinct slots of a 256-byte row of banks,
+// wherever in the tile those bits are.
 __device__ __forceinline__ uint32_t swz(uint32_t e) {
   return e ^ (((e >> 5) ^ (e >> 10)) & 31u);
 }
@@ -495,15 +496,20 @@ __device__ __forceinline__ uint32_t uxor(const uint32_t (&U)[K], uint32_t v) {
 }
 
 // One dense k-qubit group on the tile in LDS: Y = U X as the real block product, split-BF16
-// x3 on v_mfma_f32_32x32x16_bf16 (as k_fused_bf16x3).  Row s, column n of X is tile-local
-// index dep(s over the group bits) | dep(n over the other bits); the 2^(TB-9) output blocks
-// of 32 x 32 are dealt 2^(TB-11) per wave, all of one row band, so the band's A fragments
-// serve them all.  Every wave reads before any writes: the product is in place.  The caller
-// puts a barrier between the last group's writes and any later read of the tile.
+// x3 on v_mfma_f32_32x32x16_bf16 (as k_fused_bf16x3).  Row s, column n of X is the real
+// (s < D) or imaginary (s >= D) part of tile-local amplitude dep(s mod D over the group
+// bits) | dep(n over the other bits).  The tile is interleaved complex in LDS, so one
+// 8-byte read gives a column's entry for a real K step and for the matching imaginary one.
+// The block matrix comes from the host with its rows permuted: row band mt (32 rows) holds
+// the real rows 16 mt .. 16 mt + 15, then the imaginary rows D + 16 mt .. D + 16 mt + 15, so
+// a lane's accumulators pair up into the re, im of 8 output amplitudes per block.  The
+// 2^(TB-9) output blocks of 32 x 32 are dealt 2^(TB-11) per wave, all of one row band, so
+// the band's A fragments serve them all.  The caller puts a barrier between the last
+// group's writes and any later read of the tile.
 template <int K, int TB>
-__device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
+__device__ __forceinline__ void mt_dense(float2* buf, const MtOp& op,
                                          const unsigned char* pool) {
-  constexpr int D = 1 << K, R = 2 * D, KSTEPS = R / 16;
+  constexpr int D = 1 << K, R = 2 * D, KSTEPS = R / 16, KH = KSTEPS / 2;
   constexpr int NBC = 1 << (TB - K - 5);  // 32-column blocks per row band
   constexpr int BPW = 1 << (TB - 11);     // output blocks per wave
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -522,40 +528,24 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
   // the tile's previous op done by every wave (the A loads above, issued first, are in
   // flight meanwhile)
   __syncthreads();
-  // LDS byte offsets (4 x the words: the XOR structure carries over)
+  // LDS byte offsets (8 x the slots: the XOR structure carries over)
   uint32_t U[K];
 #pragma unroll
-  for (int i = 0; i < K; ++i) U[i] = 4u * swz(1u << op_byte(op.bits, i));
+  for (int i = 0; i < K; ++i) U[i] = 8u * swz(1u << op_byte(op.bits, i));
   uint32_t wl = 0;  // the lane's column l over the other bits
 #pragma unroll
   for (int i = 0; i < 5; ++i) wl |= uint32_t((l >> i) & 1) << op_byte(op.obits, i);
-  wl = 4u * swz(wl);
+  wl = 8u * swz(wl);
   auto colw = [&](int nt) {  // block nt's first column over the other bits (uniform)
     uint32_t u = 0;
 #pragma unroll
     for (int i = 5; i < TB - K; ++i) u |= uint32_t((nt >> (i - 5)) & 1) << op_byte(op.obits, i);
-    return 4u * swz(u);
+    return 8u * swz(u);
   };
-  auto at = [](const float* base, uint32_t byte) {
-    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte);
+  auto slot = [&](uint32_t byte) -> float2& {
+    return *reinterpret_cast<float2*>(reinterpret_cast<char*>(buf) + byte);
   };
-
-  floatx16 acc[BPW];
-  uint32_t wb[BPW];  // B rows 16 ks + 8 h + j of block bi
-  auto init = [&](int bi) {
-    acc[bi] = floatx16{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
-                       0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    wb[bi] = wl ^ colw(nt0 + bi) ^ (h ? U[3] : 0u);
-  };
-  // one K step of block bi: its B fragment read from LDS, split, three products
-  auto step = [&](int bi, int ks) {
-    // opaque per step: the step's read addresses are formed here, not all held at once
-    asm volatile("" : "+v"(wb[bi]));
-    const float* src = ((16 * ks) >> K) ? xi : xr;  // block rows >= D: the imaginary part
-    float x[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      x[j] = at(src, wb[bi] ^ uxor<K>(U, uint32_t((16 * ks + j) & (D - 1))));
+  auto mfma3 = [](const bf16x8& a_hi, const bf16x8& a_lo, const float* x, floatx16 c) {
     bf16x2 hh[4], lo[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) split_bf16x2(x[2 * j], x[2 * j + 1], hh[j], lo[j]);
@@ -565,22 +555,44 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
     const bf16x8 bl = __builtin_shufflevector(
         __builtin_shufflevector(lo[0], lo[1], 0, 1, 2, 3),
         __builtin_shufflevector(lo[2], lo[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
-    acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bh, acc[bi], 0, 0, 0);
-    acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks], bl, acc[bi], 0, 0, 0);
-    acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[ks], bh, acc[bi], 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_hi, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_hi, bl, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_lo, bh, c, 0, 0, 0);
+  };
+
+  floatx16 acc[BPW];
+  uint32_t wb[BPW];  // B rows 16 ks + 8 h + j of block bi
+  auto init = [&](int bi) {
+    acc[bi] = floatx16{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                       0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    wb[bi] = wl ^ colw(nt0 + bi) ^ (h ? U[3] : 0u);
+  };
+  // K steps kr (real rows 16 kr + 8 h + j) and kr + KH (the imaginary rows of the same
+  // amplitudes) of block bi: one read of 8 amplitudes, two splits, six products
+  auto steps = [&](int bi, int kr) {
+    // opaque per step: the step's read addresses are formed here, not all held at once
+    asm volatile("" : "+v"(wb[bi]));
+    float xr[8], xi[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float2 v = slot(wb[bi] ^ uxor<K>(U, uint32_t(16 * kr + j)));
+      xr[j] = v.x;
+      xi[j] = v.y;
+    }
+    acc[bi] = mfma3(ah[kr], al[kr], xr, acc[bi]);
+    acc[bi] = mfma3(ah[kr + KH], al[kr + KH], xi, acc[bi]);
     // the next step's LDS reads are issued behind these products: hoisted, all of a
     // block's reads would be live at once
     asm volatile("" ::: "memory");
   };
+  // output amplitude 16 mt + 8 (r >> 2) + 4 h + (r & 3): re in acc[r], im in acc[r + 8]
   auto write = [&](int bi) {
-    uint32_t wo = wl ^ colw(nt0 + bi) ^ (h ? U[2] : 0u);  // output rows ... + 4 h + ...
+    uint32_t wo = wl ^ colw(nt0 + bi) ^ (h ? U[2] : 0u);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
+    for (int r = 0; r < 8; ++r) {
       if ((r & 3) == 0) asm volatile("" : "+v"(wo));
-      const uint32_t rowu = uint32_t(32 * mt + 8 * (r >> 2) + (r & 3));
-      float* dst = (rowu >> K) ? xi : xr;
-      *reinterpret_cast<float*>(reinterpret_cast<char*>(dst) + (wo ^ uxor<K>(U, rowu & (D - 1)))) =
-          acc[bi][r];
+      const uint32_t a = uint32_t(16 * mt + 8 * (r >> 2) + (r & 3));
+      slot(wo ^ uxor<K>(U, a)) = float2{acc[bi][r], acc[bi][r + 8]};
     }
   };
   if constexpr (R == 32) {
@@ -591,7 +603,7 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
     for (int bi = 0; bi < BPW; ++bi) {
       init(bi);
 #pragma unroll
-      for (int ks = 0; ks < KSTEPS; ++ks) step(bi, ks);
+      for (int kr = 0; kr < KH; ++kr) steps(bi, kr);
       write(bi);
     }
   } else {
@@ -603,7 +615,7 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
 #pragma unroll
     for (int bi = 0; bi < BPW; ++bi)
 #pragma unroll
-      for (int ks = 0; ks < KSTEPS; ++ks) step(bi, ks);
+      for (int kr = 0; kr < KH; ++kr) steps(bi, kr);
     __syncthreads();
 #pragma unroll
     for (int bi = 0; bi < BPW; ++bi) write(bi);
@@ -620,20 +632,16 @@ __device__ __forceinline__ void mt_dense(float* xr, float* xi, const MtOp& op,
 // tile, the product of its members' phases over its <= 8 tile bits (one entry per thread,
 // in one of two LDS tables used in turn), then applies that table once.
 template <int TB>
-__device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const float2* tabs,
-                                            float2* tgroup, const MtOp* ops, uint32_t i0,
-                                            uint32_t i1, uint64_t base) {
+__device__ __forceinline__ void mt_diag_run(float2* buf, const float2* tabs, float2* tgroup,
+                                            const MtOp* ops, uint32_t i0, uint32_t i1,
+                                            uint64_t base) {
   constexpr int E = (1 << TB) / kBlock;
-  // opaque, so that the amplitudes' LDS words are formed here, not held across the tiles
+  // opaque, so that the amplitudes' LDS slots are formed here, not held across the tiles
   uint32_t tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
-  float vr[E], vi[E];
+  float2 v[E];
 #pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const uint32_t w = swz(tid + kBlock * j);
-    vr[j] = xr[w];
-    vi[j] = xi[w];
-  }
+  for (int j = 0; j < E; ++j) v[j] = buf[swz(tid + kBlock * j)];
   // multiply every amplitude by tab[its bits at the k positions `bw` (tile-local bits where
   // in_mask is set, else state bits)]
   auto apply = [&](const float2* tab, uint32_t k, uint32_t in_mask, uint64_t bw) {
@@ -662,9 +670,8 @@ __device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const float2* 
       for (int c = 0; c < TB - 8; ++c)
         if ((j >> c) & 1) idx ^= g8[c];
       const float2 ph = tab[idx];
-      const float a = vr[j], b = vi[j];
-      vr[j] = a * ph.x - b * ph.y;
-      vi[j] = a * ph.y + b * ph.x;
+      const float a = v[j].x, b = v[j].y;
+      v[j] = float2{a * ph.x - b * ph.y, a * ph.y + b * ph.x};
     }
   };
   auto words = [](const MtOp& o) { return uint64_t(o.bits[0]) | (uint64_t(o.bits[1]) << 32); };
@@ -700,28 +707,23 @@ __device__ __forceinline__ void mt_diag_run(float* xr, float* xi, const float2* 
     i += 1 + op.members;
   }
 #pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const uint32_t w = swz(tid + kBlock * j);
-    xr[w] = vr[j];
-    xi[w] = vi[j];
-  }
+  for (int j = 0; j < E; ++j) buf[swz(tid + kBlock * j)] = v[j];
 }
 
-// Waves per SIMD asked of the compiler: four for passes of groups up to 4 qubits (128
+// Waves per SIMD asked of the compiler: four for passes of groups up to 5 qubits (128
 // registers: a few spill around the merged diagonal tables, and the fourth workgroup per CU
-// still gains 7-10% where the pass's phase table leaves it room in LDS), three up to 5
-// qubits, two with 6-qubit groups (asked for 3, that kernel spills).
+// still gains 7-10% where the pass's phase table leaves it room in LDS), two with 6-qubit
+// groups (asked for 3, that kernel spills).
 template <int TB, int KMAX, bool NTA>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(KMAX == 6 ? 2 : 4)))
-void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
-                                                       MtPassArg pa,
-                                                       const MtOp* __restrict__ ops,
-                                                       const unsigned char* __restrict__ pool,
-                                                       int xcd) {
+void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles, MtPassArg pa,
+                  const MtOp* __restrict__ ops, const unsigned char* __restrict__ pool,
+                  int xcd) {
   constexpr int N = 1 << TB;
   constexpr int CH = N / 2 / kBlock;  // 16-byte chunks (2 amplitudes) per lane
-  __shared__ float xr[N], xi[N];
+  // the tile, interleaved complex: slot swz(e) holds amplitude e
+  __shared__ float2 buf[N];
   __shared__ float2 tgroup[2 * 256];  // merged diagonal runs' per-tile tables
   // the pass's phase table: dynamic LDS of its own size (pa.tab_entries), so passes with
   // few diagonal phases leave room for another workgroup per CU
@@ -747,11 +749,9 @@ void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
 #pragma unroll
       for (int c = 0; c < CH; ++c)
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const uint32_t w = swz(2u * threadIdx.x + uint32_t(2 * kBlock * c + e));
-          xr[w] = v[c][2 * e];
-          xi[w] = v[c][2 * e + 1];
-        }
+        for (int e = 0; e < 2; ++e)
+          buf[swz(2u * threadIdx.x + uint32_t(2 * kBlock * c + e))] =
+              float2{v[c][2 * e], v[c][2 * e + 1]};
     }
     for (uint32_t i = 0; i < pa.n_ops;) {
       const MtOp& op = pops[i];
@@ -759,14 +759,14 @@ void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
         // (mt_dense starts with the barrier after its A-fragment loads)
         // instantiated up to the pass's widest group only: a kernel holding the 6-qubit
         // product's registers runs at lower occupancy
-        if (KMAX == 4 || op.k == 4) mt_dense<4, TB>(xr, xi, op, pool);
-        else if (KMAX == 5 || op.k == 5) mt_dense<KMAX == 4 ? 4 : 5, TB>(xr, xi, op, pool);
-        else mt_dense<KMAX, TB>(xr, xi, op, pool);
+        if (KMAX == 4 || op.k == 4) mt_dense<4, TB>(buf, op, pool);
+        else if (KMAX == 5 || op.k == 5) mt_dense<KMAX == 4 ? 4 : 5, TB>(buf, op, pool);
+        else mt_dense<KMAX, TB>(buf, op, pool);
         ++i;
       } else {
         const uint32_t j = i + op.run;
         __syncthreads();  // the diagonal run's amplitudes were staged by other lanes
-        mt_diag_run<TB>(xr, xi, tabs, tgroup, pops, i, j, base);
+        mt_diag_run<TB>(buf, tabs, tgroup, pops, i, j, base);
         i = j;
       }
     }
@@ -777,9 +777,9 @@ void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles,
       floatx4 y;
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
-        const uint32_t w = swz(2u * threadIdx.x + uint32_t(2 * kBlock * c + e));
-        y[2 * e] = xr[w];
-        y[2 * e + 1] = xi[w];
+        const float2 a = buf[swz(2u * threadIdx.x + uint32_t(2 * kBlock * c + e))];
+        y[2 * e] = a.x;
+        y[2 * e + 1] = a.y;
       }
       vstore<NTA>(reinterpret_cast<floatx4*>(
                       amp + 2 * (base + (off0 | mt_offset(uint32_t(2 * kBlock * c), pa)))),

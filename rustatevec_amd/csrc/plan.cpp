@@ -844,10 +844,19 @@ void emit_mt_pass(const std::vector<Launch>& in, const std::vector<size_t>& cur,
         }
       for (uint32_t b = 0; b < tb; ++b)
         if (!((used >> b) & 1)) op.set_obit(o++, b);
+      // rows in the kernel's band order: band b (32 rows) holds the real rows 16 b ..
+      // 16 b + 15, then the imaginary rows D + 16 b .. D + 16 b + 15 (the identity at k = 4)
+      const uint32_t D = 1u << g.m, R = 2 * D;
       std::vector<uint16_t> hl(2 * a.size());
-      for (size_t e = 0; e < a.size(); ++e) {
-        hl[e] = bf16_bits(a[e]);
-        hl[a.size() + e] = bf16_bits(a[e] - bf16_value(hl[e]));
+      for (uint32_t row = 0; row < R; ++row) {
+        const uint32_t band = row / 32, i = row % 32;
+        const uint32_t src = i < 16 ? 16 * band + i : D + 16 * band + (i - 16);
+        for (uint32_t col = 0; col < R; ++col) {
+          const float x = a[size_t(src) * R + col];
+          const size_t e = size_t(row) * R + col;
+          hl[e] = bf16_bits(x);
+          hl[a.size() + e] = bf16_bits(x - bf16_value(hl[e]));
+        }
       }
       op.mat = put(hl.data(), hl.size() * sizeof(uint16_t));
     }
