@@ -150,8 +150,9 @@ hipError_t launch_fused_mfma(const State& st, uint32_t k, const uint32_t* q, con
 // so that a kernel reads every field with scalar (uniform) loads.
 // Kinds: 0 a dense group; 1 a diagonal group applied on its own; 2 the head of a merged run
 // of diagonal groups -- `members` kind-3 ops follow it, whose in-tile qubits span at most 8
-// tile bits (the head's `bits`, k of them): per tile, the product of their phases over those
-// bits is built as one table, then applied once.
+// tile bits (the head's `bits`, k of them): per tile, the product of the head's base table
+// (the run's ops with every qubit in the tile, composed on the host) and the members'
+// phases over those bits is built as one table, then applied once.
 struct MtOp {
   uint32_t kind;
   uint32_t k;         // dense: group qubits (4..6); diagonal / member: m (1..6); head: its
@@ -159,7 +160,9 @@ struct MtOp {
   uint32_t in_mask;   // diagonal / member: bit j set when qubit j is in the tile
   uint32_t run;       // kinds 1..3: the consecutive non-dense ops from this one to the
                       // pass's next dense op or end (this one included)
-  uint32_t tab;       // diagonal / member: first entry of its phases in the pass's table
+  uint32_t tab;       // diagonal / member: first entry of its phases in the pass's table;
+                      // head: its base table (the product of the run's ops with every
+                      // qubit in the tile) or kMtNoTable
   uint32_t members;   // head: the member ops that follow it
   uint32_t bits[2];   // byte j: dense: tile-local bit j of the group, ascending; diagonal:
                       // tile-local bit of qubit j (in the tile) or its state bit (outside);
@@ -167,19 +170,27 @@ struct MtOp {
                       // head: its tile-local bits, ascending
   uint32_t obits[3];  // byte i: dense: the tb - k other tile-local bits, ascending
   uint64_t mat;       // dense: byte offset in the pool of the bf16 block matrix hi, then lo
-                      // (2D x 2D each)
+                      // (2D x 2D each); head: of its members' records (8 words each: the
+                      // table offset, per qubit the head's rank of its tile bit or 15, per
+                      // qubit its state bit or 63; padded to a multiple of 4 records)
   void set_bit(uint32_t j, uint32_t b) { bits[j >> 2] |= (b & 0xffu) << (8 * (j & 3)); }
   void set_obit(uint32_t i, uint32_t b) { obits[i >> 2] |= (b & 0xffu) << (8 * (i & 3)); }
 };
 constexpr uint32_t kMtTabEntries = 2048;  // phase entries of a pass's diagonal ops (LDS)
+constexpr uint32_t kMtNoTable = 0xffffffffu;  // a head op without a base table
 struct MtPassArg {
   uint32_t tb;
   uint32_t kmax;     // widest dense group of the pass (4..6; 4 when it has none)
   uint32_t ts[13];   // state bits of the tile, ascending (tile-local bit j = state bit ts[j])
   uint32_t op_begin, n_ops;
   uint32_t tab_entries;  // the pass's phase table: complex f32 entries at pool + tab_off
+  uint32_t grp_entries;  // entries of a merged run's per-tile table (2^ its widest head's
+                         // bits; 0 without heads), two of them after the phase table in LDS
   uint64_t tab_off;
 };
+// Phase-table entries (pass table plus the two per-tile tables) up to which an MFMA tile
+// pass keeps four workgroups per CU: 32 KiB of tile plus 8 KiB of tables each.
+constexpr uint32_t kMtDynEntriesFour = 1024;
 hipError_t launch_mfma_tiles(const State& st, const MtPassArg& pa, const MtOp* ops,
                              const void* pool, const Tuning& t, hipStream_t s);
 

@@ -629,12 +629,14 @@ __device__ __forceinline__ void mt_dense(float2* buf, const MtOp& op,
 // outside the tile come from the tile's base (the same for the whole tile), those in it
 // from e, split into the lane's part and a per-j part.  A single diagonal op reads its
 // phases from the pass's table; a merged run (head op, then members) first builds, per
-// tile, the product of its members' phases over its <= 8 tile bits (one entry per thread,
-// in one of two LDS tables used in turn), then applies that table once.
+// tile, the product of its base table and its members' phases over its <= 8 tile bits (one
+// entry per thread, in one of two LDS tables of `grp` entries used in turn), then applies
+// that table once.
 template <int TB>
 __device__ __forceinline__ void mt_diag_run(float2* buf, const float2* tabs, float2* tgroup,
-                                            const MtOp* ops, uint32_t i0, uint32_t i1,
-                                            uint64_t base) {
+                                            uint32_t grp, const MtOp* ops,
+                                            const unsigned char* pool, uint32_t i0,
+                                            uint32_t i1, uint64_t base) {
   constexpr int E = (1 << TB) / kBlock;
   // opaque, so that the amplitudes' LDS slots are formed here, not held across the tiles
   uint32_t tid = threadIdx.x;
@@ -683,24 +685,38 @@ __device__ __forceinline__ void mt_diag_run(float2* buf, const float2* tabs, flo
       ++i;
       continue;
     }
-    float2* T = tgroup + 256 * (g++ & 1u);
+    float2* T = tgroup + grp * (g++ & 1u);
     if (tid < (1u << op.k)) {  // entry tid: the head's bits r set where bit r of tid is
-      float pr = 1.f, pi = 0.f;
-      for (uint32_t m = i + 1; m <= i + op.members; ++m) {
-        const MtOp& mo = ops[m];
-        const uint64_t bw = words(mo);
-        uint32_t idx = 0;
-        for (uint32_t q = 0; q < mo.k; ++q) {
-          const uint32_t b = uint32_t(bw >> (8 * q)) & 0xffu;
-          const uint32_t v = ((mo.in_mask >> q) & 1u) ? (tid >> b) : uint32_t(base >> b);
-          idx |= (v & 1u) << q;
+      float2 p = op.tab == kMtNoTable ? float2{1.f, 0.f} : tabs[op.tab + tid];
+      // members four at a time from their records: the records, then the phases, fetched
+      // together, then multiplied in order.  Index bit q of a member is bit rank[q] of tid
+      // (in the tile) or bit sbit[q] of the tile's base (outside); an unused rank 15 or
+      // state bit 63 reads 0.
+      const uint32_t* rec = reinterpret_cast<const uint32_t*>(pool + op.mat);
+      const uint32_t nm = op.members;
+      for (uint32_t m0 = 0; m0 < nm; m0 += 4) {
+        uint32_t at[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t* r = rec + 8 * (m0 + u);
+          uint32_t idx = 0, fix = 0;
+#pragma unroll
+          for (int q = 0; q < int(kMaxGateQubits); ++q) {
+            const uint32_t rk = (r[1 + q / 4] >> (8 * (q % 4))) & 0xffu;
+            const uint32_t sb = (r[3 + q / 4] >> (8 * (q % 4))) & 0xffu;
+            idx |= ((tid >> rk) & 1u) << q;
+            fix |= uint32_t((base >> sb) & 1u) << q;
+          }
+          at[u] = r[0] + (idx | fix);
         }
-        const float2 ph = tabs[mo.tab + idx];
-        const float a = pr;
-        pr = a * ph.x - pi * ph.y;
-        pi = a * ph.y + pi * ph.x;
+        float2 ph[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ph[u] = tabs[at[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (m0 + u < nm) p = float2{p.x * ph[u].x - p.y * ph[u].y, p.x * ph[u].y + p.y * ph[u].x};
       }
-      T[tid] = float2{pr, pi};
+      T[tid] = p;
     }
     __syncthreads();  // the table is complete; the other one's last readers are done
     apply(T, op.k, (1u << op.k) - 1u, words(op));
@@ -724,10 +740,11 @@ void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles, MtPassArg pa,
   constexpr int CH = N / 2 / kBlock;  // 16-byte chunks (2 amplitudes) per lane
   // the tile, interleaved complex: slot swz(e) holds amplitude e
   __shared__ float2 buf[N];
-  __shared__ float2 tgroup[2 * 256];  // merged diagonal runs' per-tile tables
-  // the pass's phase table: dynamic LDS of its own size (pa.tab_entries), so passes with
-  // few diagonal phases leave room for another workgroup per CU
+  // the pass's phase table, then the merged runs' two per-tile tables: dynamic LDS of their
+  // own size (pa.tab_entries + 2 pa.grp_entries), so passes with few diagonal phases leave
+  // room for another workgroup per CU
   extern __shared__ float2 tabs[];
+  float2* const tgroup = tabs + pa.tab_entries;
   {  // the pass's phase table, once per workgroup
     const float2* src = reinterpret_cast<const float2*>(pool + pa.tab_off);
     for (uint32_t e = threadIdx.x; e < pa.tab_entries; e += kBlock) tabs[e] = src[e];
@@ -766,7 +783,7 @@ void k_mfma_tiles(float* __restrict__ amp, uint64_t n_tiles, MtPassArg pa,
       } else {
         const uint32_t j = i + op.run;
         __syncthreads();  // the diagonal run's amplitudes were staged by other lanes
-        mt_diag_run<TB>(buf, tabs, tgroup, pops, i, j, base);
+        mt_diag_run<TB>(buf, tabs, tgroup, pa.grp_entries, pops, pool, i, j, base);
         i = j;
       }
     }
@@ -795,9 +812,10 @@ hipError_t launch_mt(const State& st, const MtPassArg& pa, const MtOp* ops, cons
   const int cus = st.ctx && st.ctx->cus > 0 ? st.ctx->cus : 256;
   // default grid: as many workgroups as fit on the chip at once (measured best: fewer
   // leave LDS latency unhidden, more queue behind the resident ones)
-  const size_t dyn = size_t(pa.tab_entries > 0 ? pa.tab_entries : 1) * sizeof(float2);
-  // workgroups per CU for this phase-table size (in 512-entry steps), looked up once
-  static std::atomic<int> per_cu_by_size[kMtTabEntries / 512 + 1];
+  const size_t entries = size_t(pa.tab_entries) + 2 * size_t(pa.grp_entries);
+  const size_t dyn = (entries > 0 ? entries : 1) * sizeof(float2);
+  // workgroups per CU for this table size (in 512-entry steps), looked up once
+  static std::atomic<int> per_cu_by_size[(kMtTabEntries + 2 * 256) / 512 + 1];
   const size_t slot = (dyn + 512 * sizeof(float2) - 1) / (512 * sizeof(float2));
   int per_cu = per_cu_by_size[slot].load(std::memory_order_relaxed);
   if (per_cu < 1) {
@@ -821,7 +839,7 @@ hipError_t launch_mt(const State& st, const MtPassArg& pa, const MtOp* ops, cons
 hipError_t launch_mfma_tiles(const State& st, const MtPassArg& pa, const MtOp* ops,
                              const void* pool, const Tuning& t, hipStream_t s) {
   if (st.dtype != 1 || pa.tb != 12 || st.n < pa.tb || pa.ts[0] != 0 || pa.kmax < 4 ||
-      pa.kmax > 6 || pa.tab_entries > kMtTabEntries)
+      pa.kmax > 6 || pa.tab_entries > kMtTabEntries || pa.grp_entries > 256)
     return hipErrorInvalidValue;
   const bool nt = nontemporal_for(st, t, true);
   auto by_k = [&](auto tb) {

@@ -770,6 +770,10 @@ constexpr size_t kMtLookahead = 1024;
 
 // Emit one MFMA tile pass holding the launches `cur` (in order): the tile is the bits of S
 // (the dense ops' qubits and the low bits), filled up to tb bits with the lowest others.
+static uint32_t op_bit(const MtOp& o, uint32_t j) {
+  return (o.bits[j >> 2] >> (8 * (j & 3))) & 0xffu;
+}
+
 void emit_mt_pass(const std::vector<Launch>& in, const std::vector<size_t>& cur, uint32_t n,
                   uint32_t tb, uint64_t S, MtPlan* out) {
   std::vector<MtOp>& ops = out->ops;
@@ -795,10 +799,11 @@ void emit_mt_pass(const std::vector<Launch>& in, const std::vector<size_t>& cur,
       pos[b] = nb++;
     }
   pa.op_begin = uint32_t(ops.size());
-  std::vector<float> tabs;  // the pass's phase table, staged in LDS once per workgroup
+  std::vector<const Launch*> src;  // the launch of each of this pass's ops
   for (size_t li : cur) {
     const Launch& l = in[li];
     out->order.push_back(li);
+    src.push_back(&l);
     MtOp op{};
     if (l.kind == Launch::kDiag) {
       op.kind = 1;
@@ -812,8 +817,6 @@ void emit_mt_pass(const std::vector<Launch>& in, const std::vector<size_t>& cur,
           op.set_bit(j, q);
         }
       }
-      op.tab = uint32_t(tabs.size() / 2);
-      for (size_t e = 0; e < (size_t(2) << l.m); ++e) tabs.push_back(float(l.mat[e]));
     } else {
       Launch g = l;
       if (l.m < 4) {
@@ -833,8 +836,8 @@ void emit_mt_pass(const std::vector<Launch>& in, const std::vector<size_t>& cur,
         gm |= 1u << pos[sorted[j]];
       }
       // the other bits, the first 5 (a wave's 32 columns) chosen with distinct residues
-      // mod 5 where the group leaves them: tile-local bit b lands in LDS bank bit b mod 5
-      // (the tile's XOR fold), so such columns hit 32 banks
+      // mod 5 where the group leaves them: tile-local bit b lands in LDS slot bit b mod 5
+      // (the tile's XOR fold), so such columns hit 32 slots of a row of banks
       uint32_t o = 0, res = 0, used = gm;
       for (uint32_t b = 0; b < tb && o < 5; ++b)
         if (!((used >> b) & 1) && !((res >> (b % 5)) & 1)) {
@@ -850,9 +853,9 @@ void emit_mt_pass(const std::vector<Launch>& in, const std::vector<size_t>& cur,
       std::vector<uint16_t> hl(2 * a.size());
       for (uint32_t row = 0; row < R; ++row) {
         const uint32_t band = row / 32, i = row % 32;
-        const uint32_t src = i < 16 ? 16 * band + i : D + 16 * band + (i - 16);
+        const uint32_t sr = i < 16 ? 16 * band + i : D + 16 * band + (i - 16);
         for (uint32_t col = 0; col < R; ++col) {
-          const float x = a[size_t(src) * R + col];
+          const float x = a[size_t(sr) * R + col];
           const size_t e = size_t(row) * R + col;
           hl[e] = bf16_bits(x);
           hl[a.size() + e] = bf16_bits(x - bf16_value(hl[e]));
@@ -862,51 +865,162 @@ void emit_mt_pass(const std::vector<Launch>& in, const std::vector<size_t>& cur,
     }
     ops.push_back(op);
   }
-  // merge runs of diagonal ops: consecutive ones (they commute) whose in-tile qubits fit 8
-  // tile bits share one per-tile table (a head op, then its members)
+  // The pass's phase table (staged in LDS once per workgroup): every diagonal op's phases,
+  // rounded to f32 once.
+  std::vector<float> tabs;
+  auto add_table = [&](const double* ph, uint32_t entries) {
+    const uint32_t at = uint32_t(tabs.size() / 2);
+    for (uint32_t e = 0; e < 2 * entries; ++e) tabs.push_back(float(ph[e]));
+    return at;
+  };
+  // Runs of diagonal ops: consecutive ones (they commute) whose in-tile qubits fit 8 tile
+  // bits share one table over those bits (the head op's).  The members with every qubit in
+  // the tile compose into the head's base table here, once, in f64; the others (a qubit
+  // outside the tile, fixed per tile) stay member ops, multiplied into a per-tile copy of
+  // the base table by the kernel.  A run without such members becomes one single diagonal
+  // op over its tile bits.  A run keeps all its members per tile where its base table
+  // would take the pass's tables past kMtTabEntries, or past kMtDynEntriesFour when they
+  // fit it without base tables (the fourth workgroup per CU is worth more).
   {
-    std::vector<MtOp> merged(ops.begin(), ops.begin() + pa.op_begin);
-    size_t e = pa.op_begin;
-    while (e < ops.size()) {
+    struct Run {
+      size_t e, f;       // ops [e, f)
+      uint32_t U, k;     // tile-local bits, their count
+      uint32_t own_static, n_dyn;  // entries of the in-tile members' own tables; the others
+      bool fold;
+    };
+    std::vector<Run> runs;
+    uint32_t tab = 0;  // entries of the ops' own tables (the callers keep this in bounds)
+    for (size_t e = pa.op_begin; e < ops.size();) {
       if (ops[e].kind != 1) {
-        merged.push_back(ops[e++]);
+        ++e;
         continue;
       }
-      uint32_t U = 0;  // tile-local bits of the group
-      size_t f = e;
-      for (; f < ops.size() && ops[f].kind == 1; ++f) {
-        uint32_t u = U;
-        for (uint32_t j = 0; j < ops[f].k; ++j)
-          if ((ops[f].in_mask >> j) & 1) u |= 1u << ((ops[f].bits[j >> 2] >> (8 * (j & 3))) & 0xff);
+      Run r{e, e, 0, 0, 0, 0, false};
+      for (; r.f < ops.size() && ops[r.f].kind == 1; ++r.f) {
+        uint32_t u = r.U;
+        for (uint32_t j = 0; j < ops[r.f].k; ++j)
+          if ((ops[r.f].in_mask >> j) & 1) u |= 1u << op_bit(ops[r.f], j);
         if (popcount64(u) > 8) break;
-        U = u;
+        r.U = u;
       }
-      if (f - e < 2) {
-        merged.push_back(ops[e++]);
+      for (size_t i = r.e; i < r.f; ++i) tab += 1u << ops[i].k;
+      if (r.f - r.e >= 2) {
+        r.k = uint32_t(popcount64(r.U));
+        for (size_t i = r.e; i < r.f; ++i) {
+          if (ops[i].in_mask == (1u << ops[i].k) - 1) r.own_static += 1u << ops[i].k;
+          else ++r.n_dyn;
+        }
+        runs.push_back(r);
+      }
+      e = r.f;
+    }
+    uint32_t grp_dyn = 0;  // per-tile table entries the runs with outside qubits need anyway
+    for (const Run& r : runs)
+      if (r.n_dyn) grp_dyn = std::max(grp_dyn, 1u << r.k);
+    uint32_t grp_static = 0;  // the same for the all-inside runs kept as heads
+    for (const Run& r : runs) grp_static = std::max(grp_static, r.n_dyn ? 0u : 1u << r.k);
+    const uint32_t cap = tab + 2 * std::max(grp_dyn, grp_static) <= kMtDynEntriesFour
+                             ? kMtDynEntriesFour
+                             : kMtTabEntries + 2 * 256;
+    grp_static = 0;
+    for (Run& r : runs) {
+      if (!r.own_static) continue;
+      const uint32_t t2 = tab - r.own_static + (1u << r.k);
+      // (the later all-inside runs are counted as folded)
+      const uint32_t g2 = std::max(grp_dyn, grp_static);
+      r.fold = t2 <= kMtTabEntries && t2 + 2 * g2 <= cap;
+      if (r.fold) tab = t2;
+      else if (!r.n_dyn) grp_static = std::max(grp_static, 1u << r.k);
+    }
+
+    std::vector<MtOp> merged(ops.begin(), ops.begin() + pa.op_begin);
+    auto own_table = [&](size_t i) {
+      const Launch& l = *src[i - pa.op_begin];
+      return add_table(l.mat.data(), 1u << l.m);
+    };
+    size_t ri = 0;
+    uint32_t grp = 0;
+    for (size_t e = pa.op_begin; e < ops.size();) {
+      if (ri < runs.size() && runs[ri].e == e) {
+        const Run& r = runs[ri++];
+        MtOp head{};
+        head.kind = 2;
+        uint32_t rank[32] = {0};
+        for (uint32_t b = 0; b < 32; ++b)
+          if ((r.U >> b) & 1) {
+            rank[b] = head.k;
+            head.set_bit(head.k++, b);
+          }
+        const uint32_t entries = 1u << head.k;
+        head.tab = kMtNoTable;
+        if (r.fold) {
+          // base table: entry x (the head's bits j set where bit j of x is) = product of
+          // the in-tile members' phases
+          std::vector<double> base(2 * entries);
+          for (uint32_t x = 0; x < entries; ++x) {
+            double pr = 1.0, pi = 0.0;
+            for (size_t i = r.e; i < r.f; ++i) {
+              if (ops[i].in_mask != (1u << ops[i].k) - 1) continue;
+              const Launch& l = *src[i - pa.op_begin];
+              uint32_t idx = 0;
+              for (uint32_t j = 0; j < ops[i].k; ++j)
+                idx |= ((x >> rank[op_bit(ops[i], j)]) & 1u) << j;
+              const double a = l.mat[2 * idx], b = l.mat[2 * idx + 1];
+              const double re = pr * a - pi * b;
+              pi = pr * b + pi * a;
+              pr = re;
+            }
+            base[2 * x] = pr;
+            base[2 * x + 1] = pi;
+          }
+          head.tab = add_table(base.data(), entries);
+          if (!r.n_dyn) {  // one diagonal op over the run's tile bits
+            head.kind = 1;
+            head.in_mask = entries - 1;
+            merged.push_back(head);
+            e = r.f;
+            continue;
+          }
+        }
+        grp = std::max(grp, entries);
+        // members: byte j of the bits is the head's rank of qubit j's tile bit (in_mask
+        // bit j set) or its state bit.  The kernel reads them as records in the pool, 8
+        // words each, padded to a multiple of 4 (with copies of the last): the table
+        // offset; per qubit j (bytes) the head's rank of its tile bit or 15; per qubit j
+        // its state bit or 63 (a rank 15 or state bit 63 reads 0).
+        const size_t hi = merged.size();
+        merged.push_back(head);
+        std::vector<uint32_t> rec;
+        for (size_t i = r.e; i < r.f; ++i) {
+          if (r.fold && ops[i].in_mask == (1u << ops[i].k) - 1) continue;
+          MtOp m = ops[i];
+          m.kind = 3;
+          m.bits[0] = m.bits[1] = 0;
+          uint32_t w[8] = {0, 0x0f0f0f0fu, 0x0f0f0f0fu, 0x3f3f3f3fu, 0x3f3f3f3fu, 0, 0, 0};
+          for (uint32_t j = 0; j < m.k; ++j) {
+            const uint32_t b = op_bit(ops[i], j);
+            const bool in_tile = (m.in_mask >> j) & 1;
+            m.set_bit(j, in_tile ? rank[b] : b);
+            uint32_t& word = w[(in_tile ? 1 : 3) + j / 4];
+            const uint32_t sh = 8 * (j % 4);
+            word = (word & ~(0xffu << sh)) | ((in_tile ? rank[b] : b) << sh);
+          }
+          m.tab = w[0] = own_table(i);
+          merged.push_back(m);
+          rec.insert(rec.end(), w, w + 8);
+        }
+        while ((rec.size() / 8) % 4) rec.insert(rec.end(), rec.end() - 8, rec.end());
+        merged[hi].mat = put(rec.data(), rec.size() * sizeof(uint32_t));
+        merged[hi].members = uint32_t(merged.size() - hi - 1);
+        e = r.f;
         continue;
       }
-      MtOp head{};
-      head.kind = 2;
-      head.members = uint32_t(f - e);
-      uint32_t rank[32] = {0};
-      for (uint32_t b = 0; b < 32; ++b)
-        if ((U >> b) & 1) {
-          rank[b] = head.k;
-          head.set_bit(head.k++, b);
-        }
-      merged.push_back(head);
-      for (; e < f; ++e) {
-        MtOp m = ops[e];
-        m.kind = 3;
-        m.bits[0] = m.bits[1] = 0;
-        for (uint32_t j = 0; j < m.k; ++j) {
-          const uint32_t b = (ops[e].bits[j >> 2] >> (8 * (j & 3))) & 0xff;
-          m.set_bit(j, ((m.in_mask >> j) & 1) ? rank[b] : b);
-        }
-        merged.push_back(m);
-      }
+      merged.push_back(ops[e]);
+      if (ops[e].kind == 1) merged.back().tab = own_table(e);
+      ++e;
     }
     ops.swap(merged);
+    pa.grp_entries = grp;
   }
   pa.n_ops = uint32_t(ops.size()) - pa.op_begin;
   pa.tab_entries = uint32_t(tabs.size() / 2);

@@ -65,6 +65,8 @@ def exact(n, ops, psi):
 
 
 def errors(got, want):
+    # (inner products in f64: complex64 sums carry ~1e-7 of rounding, the size of the bar)
+    got = got.astype(np.complex128)
     err = max_abs_diff(got, want) / np.abs(want).max()
     fid = abs(np.vdot(want, got)) ** 2 / (np.vdot(want, want).real * np.vdot(got, got).real)
     return err, 1 - fid

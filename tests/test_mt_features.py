@@ -20,8 +20,10 @@ def diag(rng, m):
 def test_one_pass_counts_dense_merged_and_table_entries():
     # n = 14: the dense groups' qubits {0..3} and {4..8} with qubits 0..3 fill 9 of the 12
     # tile bits; the tile is qubits 0..11.  The two diagonal groups between them are
-    # consecutive with in-tile qubits {5, 4} (qubit 13 is outside): one merged run (a head
-    # over 2 tile bits, 2 members).  Phase table: 4 + 2 entries.
+    # consecutive with in-tile qubits {5, 4} (qubit 13 is outside): one merged run, a head
+    # over 2 tile bits whose base table holds the group on {4} (every qubit in the tile),
+    # and one member (the group on {5, 13}).  Phase table: the base table's 4 entries and
+    # the member's 4.
     rng = np.random.default_rng(1)
     c = rs.Circuit(14)
     c.push(dense(rng, 4), [0, 1, 2, 3])
@@ -34,11 +36,32 @@ def test_one_pass_counts_dense_merged_and_table_entries():
     want[0] = 1 / 16          # dense 4-qubit
     want[1] = 1 / 16          # dense 5-qubit
     want[4] = 1 / 16          # merged run heads
-    want[5] = 2 / 64          # their members
+    want[5] = 1 / 64          # their members
     want[6] = 4 / 2048        # per-tile table of the head (2 tile bits)
-    want[7] = 6 / 2048        # the pass's phase table
+    want[7] = 8 / 2048        # the pass's phase table
     want[8] = 1.0             # widest group: 5 qubits
     want[10] = 14 / 32        # the state size
+    np.testing.assert_allclose(rows[0], want, rtol=1e-6)
+
+
+def test_a_run_inside_the_tile_is_one_diagonal_op():
+    # the same pass with the diagonal groups on {4, 5} and {6}: every qubit of the run is in
+    # the tile, so its product over tile bits {4, 5, 6} is composed once, on the host, into
+    # one diagonal op of 8 phases -- no head, no members
+    rng = np.random.default_rng(1)
+    c = rs.Circuit(14)
+    c.push(dense(rng, 4), [0, 1, 2, 3])
+    c.push(diag(rng, 2), [4, 5])
+    c.push(diag(rng, 1), [6])
+    c.push(dense(rng, 5), [4, 5, 6, 7, 8])
+    rows, other = planner.mt_pass_features(c)
+    assert other == 0 and rows.shape == (1, planner.N_MT_FEATURES)
+    want = np.zeros(planner.N_MT_FEATURES, dtype=np.float32)
+    want[0] = want[1] = 1 / 16
+    want[3] = 1 / 32          # one single diagonal op
+    want[7] = 8 / 2048
+    want[8] = 1.0
+    want[10] = 14 / 32
     np.testing.assert_allclose(rows[0], want, rtol=1e-6)
 
 
