@@ -10,7 +10,8 @@
 // or 5 qubits (the fuse rule restricted to the pass: a group stays diagonal or inside the
 // tile; dense groups are split-BF16 matrix-core products).  The exact-pass and MFMA-pass
 // cost-model networks price every form of every pass from its lowered ops, and the plan
-// keeps the cheapest form of each pass.
+// keeps the cheapest form of each pass -- or, when that mixed plan is not predicted clearly
+// faster than the best plan of one form for every pass, that one-form plan.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,6 +41,7 @@ constexpr uint64_t kLowMask = 15;  // qubits 0..3: 16-amplitude runs in every ti
 constexpr size_t kPassLookahead = 1024;
 constexpr int kTileF = 18, kMtF = 11;
 constexpr uint32_t kFuseMin = 3, kFuseMax = 5;
+constexpr double kMixMargin = 0.03;
 
 uint32_t popc(uint64_t x) { return uint32_t(__builtin_popcountll(x)); }
 
@@ -274,6 +276,37 @@ int plan_passes(Ctx* ctx, Mlp* tile_mlp, Mlp* mt_mlp, const Circuit& c, int form
     }
     for (size_t r = 0; r < rt.size(); ++r) t_exact[rt[r]] = std::exp2(double(yt[r])) * scale;
     for (size_t r = 0; r < rm.size(); ++r) t_mt[rm[r]] = std::exp2(double(ym[r])) * scale;
+  }
+  // A learned plan that mixes forms must be predicted faster than the best one-form plan by
+  // kMixMargin: consecutive passes of different kernels ran 1-4% slower back to back than
+  // the sum of their times measured one by one (random mix at 24 and 28 qubits), which the
+  // per-pass models do not see.  Otherwise every pass takes that one form (passes it cannot
+  // run keep their cheapest form).
+  if (form < 0 && tile_mlp && mt_mlp) {
+    auto pass_t = [&](size_t p, int f) {
+      return f == 0 ? t_exact[p] : t_mt[size_t(f) * P + p];
+    };
+    auto cheapest = [&](size_t p) {
+      double b = t_exact[p];
+      for (uint32_t k = kFuseMin; k <= kFuseMax; ++k) b = std::min(b, t_mt[k * P + p]);
+      return b;
+    };
+    double mixed = 0, best_uniform = inf;
+    int best_f = -1;
+    for (size_t p = 0; p < P; ++p) mixed += cheapest(p);
+    for (int f = 0; f <= int(kFuseMax); ++f) {
+      if (f > 0 && f < int(kFuseMin)) continue;
+      double tot = 0;
+      for (size_t p = 0; p < P; ++p) {
+        const bool ok = f == 0 ? forms[p].exact_ok : forms[p].mt_ok[f];
+        tot += ok ? pass_t(p, f) : cheapest(p);
+      }
+      if (tot < best_uniform) {
+        best_uniform = tot;
+        best_f = f;
+      }
+    }
+    if (best_f >= 0 && !(mixed < (1.0 - kMixMargin) * best_uniform)) form = best_f;
   }
   // the form of every pass
   out->n = n;

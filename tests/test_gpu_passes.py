@@ -169,31 +169,38 @@ def test_learned_pass_plan_gain(backend, pass_models, n, seed):
     ref = backend.execute(circ, rs.C128)
     st = backend.alloc(n, rs.C64)
 
-    def run(c, **kw):
-        p = rs.Program(st, c, **kw)
-        backend.init_basis(st, 0)
-        p.launch()
-        err, fid = backend.compare(ref, st)
-        t = min(p.time() for _ in range(5))
-        del p
-        return t, err, fid
+    progs = {}
+
+    def add(name, c, **kw):
+        progs[name] = rs.Program(st, c, **kw)
 
     planned, pred = planner.plan_passes(backend, tm, mm, circ, planner.FORM_LEARNED)
     forms = [f for f, _, _, _ in planner.plan_info(planned)]
-    t_plan, err, fid = run(planned, precision=rs.PREC_MFMA_BF16X3, planned=True)
+    add("learned", planned, precision=rs.PREC_MFMA_BF16X3, planned=True)
+    backend.init_basis(st, 0)
+    progs["learned"].launch()
+    err, fid = backend.compare(ref, st)
     assert err < 1e-6 and 1 - fid < 1e-8, (err, 1 - fid)
-    fixed = {}
     for k in (4, 5, 6):
-        fixed[f"greedy_k{k}"] = run(rs.fuse(backend, circ, k), precision=rs.PREC_MFMA_BF16X3)[0]
+        add(f"greedy_k{k}", rs.fuse(backend, circ, k), precision=rs.PREC_MFMA_BF16X3)
     for tb in (11, 12, 13):
-        fixed[f"tiles_t{tb}"] = run(circ, tile_qubits=tb)[0]
+        add(f"tiles_t{tb}", circ, tile_qubits=tb)
     for k in (4, 5):
-        fixed[f"mfma_tiles_k{k}"] = run(rs.fuse(backend, circ, k),
-                                        precision=rs.PREC_MFMA_BF16X3, mfma_tiles=12)[0]
-    uniform = {}
+        add(f"mfma_tiles_k{k}", rs.fuse(backend, circ, k), precision=rs.PREC_MFMA_BF16X3,
+            mfma_tiles=12)
     for f in (0, 3, 4, 5):
         pc, _ = planner.plan_passes(backend, None, None, circ, f)
-        uniform[f"passes_{f}"] = run(pc, precision=rs.PREC_MFMA_BF16X3, planned=True)[0]
+        add(f"passes_{f}", pc, precision=rs.PREC_MFMA_BF16X3, planned=True)
+    # rounds over all programs, each program's best round: clock and cache drift over the
+    # run reach every program alike
+    times = {name: float("inf") for name in progs}
+    for _ in range(5):
+        for name, p in progs.items():
+            times[name] = min(times[name], p.time())
+    progs.clear()
+    t_plan = times["learned"]
+    fixed = {k: v for k, v in times.items() if not k.startswith(("learned", "passes_"))}
+    uniform = {k: v for k, v in times.items() if k.startswith("passes_")}
     best = min(fixed, key=fixed.get)
     best_u = min(uniform, key=uniform.get)
     print(f"mix({n}) seed {seed:#x}: learned passes {t_plan:.3f} ms (predicted {pred:.3f}; "
