@@ -208,6 +208,14 @@ hipError_t launch_mfma_tiles(const State& st, const MtPassArg& pa, const MtOp* o
 //  kind 3: a register block on bits lq[0..m); its n_diag members (1-qubit dense ops whose
 //  lq[0] is an index into the block's bits and insm[0] / lcval a condition on the block's
 //  sub-index) follow it in the op list.
+//  kind 4: a lane run; its n_diag members follow it (kind 1 diagonal members, kind 0
+//  1-qubit members with a condition insm[0] / lcval on the run index u = tid + k NT) in the
+//  run's coordinates: u's bits below kTileLaneBits are the lane's, the ones above the
+//  register index k's.  m = 0: u is the tile-local index; m = 1: register bit j < lq[0] of
+//  k is tile-local bit perm[j] (ascending) and the lane's bits fill the others in order.
+//  kind 5 (lane-run member): n_diag phases controlled by one outside bit on the run index
+//  bit lq[0]: gate d (diags[off + d], outside bit oq) multiplies the amplitudes whose bit
+//  lq[0] is 1 by pool[ins[0] + d] in tiles whose base has bit oq set.
 struct TileOp {
   uint32_t kind;
   uint32_t m;
@@ -227,22 +235,23 @@ struct TileOp {
 // so tile-local bits below kTileLaneBits are lane bits and the others register bits.
 constexpr uint32_t kTileLaneBits = 9;
 constexpr uint32_t kOutside = 0xffffffffu;
-// A diagonal gate inside a diag run, lowered for one pass.
-//  m <= 2: the gate's bits are sorted into outside bits (state bits oq0, oq1 of the tile
-//  base: so = bit oq0 | bit oq1 << 1, nout of them) and up to two "index bits" forming
-//  c (0..3), each absent, a lane bit (tid bit lp0 / lp1) or a register bit (k bit p);
-//  variant = kind0 | kind1 << 4 with kind 0 absent, 1 lane, 2 + p register bit p.  The
-//  phase of an amplitude is pool[tab + 4 so + c]; bit 4 so + c of idw: it is exactly 1 + 0i.
+// A diagonal gate inside a diag run, lowered for one pass (its first 8 words are what a
+// kernel reads per gate: one scalar load).
+//  m <= 2: the gate's bits are sorted into outside bits (state bits oq0 = oq & 63, oq1 =
+//  oq >> 8 of the tile base: so = bit oq0 | bit oq1 << 1, nout of them) and up to two
+//  "index bits" forming c (0..3), each absent, a lane bit (tid bit lp0 / lp1) or a
+//  register bit (k bit p); code = variant | lp0 << 8 | lp1 << 12 | nout << 16, variant =
+//  kind0 | kind1 << 4 with kind 0 absent, 1 lane, 2 + p register bit p.  The phase of an
+//  amplitude is pool[tab + 4 so + c]; bit 4 so + c of idw: it is exactly 1 + 0i.
 //  m > 2: phases pool[tab + s], s = gather of bits q[j] (tile-local bit loc[j] or, for
 //  kOutside, the base bit).
 struct TileDiag {
   uint32_t m;
-  uint32_t variant;
-  uint32_t lp0, lp1;
-  uint32_t oq0, oq1;
-  uint32_t nout;
+  uint32_t code;
+  uint32_t oq;
   uint32_t tab;
   uint32_t idw;
+  uint32_t pad[3];
   uint32_t q[kMaxGateQubits];
   uint32_t loc[kMaxGateQubits];
 };

@@ -65,6 +65,10 @@ std::vector<float> block_matrix(const Launch& l, uint32_t* sorted) {
 namespace {
 
 constexpr uint32_t kTileLowBits = 4;  // bits 0..3 always in a tile: 16-amplitude runs
+// A 1-qubit op on a bit that the diagonal gates after it use this many times as an index
+// bit starts a lane run with that bit among the register bits (a lane-bit index costs the
+// multiply on every amplitude of the lane's registers, a register-bit one on those it picks).
+constexpr uint32_t kHeavyDiag = 4;
 
 uint32_t popcount64(uint64_t x) { return uint32_t(__builtin_popcountll(x)); }
 
@@ -145,7 +149,24 @@ void lower_exact_pass(const std::vector<Launch>& in, const std::vector<uint64_t>
   (void)n;
   pa.op_begin = uint32_t(bld.ops.size());
   bld.pass_begin = bld.ops.size();
-  for (uint64_t i : take) {
+  // Lane runs choose their register bits when the pass fills the kernel's tile (a smaller
+  // tile, a batch circuit below the batch's tile size, keeps the fixed layout).
+  bld.tile_bits = tb;
+  bld.run_lk = (bld.kernel_tb == 0 || bld.kernel_tb == tb) && tb > kTileLaneBits
+                   ? std::min<uint32_t>(tb - kTileLaneBits, 4)
+                   : 0;
+  for (size_t t = 0; t < take.size(); ++t) {
+    const uint64_t i = take[t];
+    // the tile bits that the diagonal gates up to the next other op use kHeavyDiag times
+    bld.heavy = 0;
+    if (in[i].kind != Launch::kDiag && bld.run_lk > 0) {
+      uint32_t use[64] = {0};
+      for (size_t u = t + 1; u < take.size() && in[take[u]].kind == Launch::kDiag; ++u)
+        for (uint32_t j = 0; j < in[take[u]].m; ++j) {
+          const uint32_t q = in[take[u]].q[j];
+          if ((tmask >> q) & 1u && ++use[pos[q]] == kHeavyDiag) bld.heavy |= 1ull << pos[q];
+        }
+    }
     bld.lower(in[i], tmask, pos);
     order.push_back(i);
   }
@@ -237,41 +258,9 @@ void TileBuilder::lower(const Launch& l, uint64_t tmask, const uint32_t* pos) {
       }
       if (l.m > 2) {
         d.tab = uint32_t(add_matrix(l.mat.data(), size_t(1) << l.m));
-      } else {
-        // index bits: lane bits first, then register bits by ascending k bit
-        struct Bit { uint32_t j, kind, lp; };
-        Bit ib[2];
-        uint32_t ni = 0, oj[2], no = 0;
-        for (uint32_t j = 0; j < l.m; ++j) {
-          if (d.loc[j] == kOutside) {
-            oj[no++] = j;
-          } else if (d.loc[j] < kTileLaneBits) {
-            ib[ni++] = Bit{j, 1, d.loc[j]};
-          } else {
-            ib[ni++] = Bit{j, 2 + d.loc[j] - kTileLaneBits, 0};
-          }
-        }
-        if (ni == 2 && (ib[0].kind > ib[1].kind)) std::swap(ib[0], ib[1]);
-        d.variant = (ni > 0 ? ib[0].kind : 0) | ((ni > 1 ? ib[1].kind : 0) << 4);
-        d.lp0 = ni > 0 && ib[0].kind == 1 ? ib[0].lp : 0;
-        d.lp1 = ni > 1 && ib[1].kind == 1 ? ib[1].lp : 0;
-        d.nout = no;
-        d.oq0 = no > 0 ? l.q[oj[0]] : 0;
-        d.oq1 = no > 1 ? l.q[oj[1]] : 0;
-        // 4 phases per outside value so, indexed by c; unreachable c hold zeros
-        std::vector<double> tab(2 * 4 * (size_t(1) << no), 0.0);
-        for (uint32_t so = 0; so < (1u << no); ++so)
-          for (uint32_t c = 0; c < 4; ++c) {
-            if ((c >> ni) != 0) continue;
-            uint32_t s = 0;
-            for (uint32_t i = 0; i < ni; ++i) s |= ((c >> i) & 1u) << ib[i].j;
-            for (uint32_t o = 0; o < no; ++o) s |= ((so >> o) & 1u) << oj[o];
-            const size_t at = 2 * (4 * size_t(so) + c);
-            tab[at] = l.mat[2 * s];
-            tab[at + 1] = l.mat[2 * s + 1];
-            if (tab[at] == 1.0 && tab[at + 1] == 0.0) d.idw |= 1u << (4 * so + c);
-          }
-        d.tab = uint32_t(add_matrix(tab.data(), tab.size() / 2));
+        diag_phases.emplace_back();
+      } else {  // its table waits for the run's coordinates
+        diag_phases.emplace_back(l.mat.begin(), l.mat.begin() + 2 * (size_t(1) << l.m));
       }
       diags.push_back(d);
       if (open_run >= 0 && ops.back().kind == 1 && ops.size() > size_t(open_run) + 1 &&
@@ -318,6 +307,45 @@ void TileBuilder::lower(const Launch& l, uint64_t tmask, const uint32_t* pos) {
       lower_block(sub.data(), nin, in_j, l.q, pos, cmask, cval);
     }
   }
+
+// Phase table and lowering variant of a diagonal gate of m <= 2 qubits whose loc[] are in
+// its run's coordinates (lane bits below kTileLaneBits, register bits from there up).
+void TileBuilder::finish_small_diag(TileDiag& d, const std::vector<double>& mat) {
+  // index bits: lane bits first, then register bits by ascending k bit
+  struct Bit { uint32_t j, kind, lp; };
+  Bit ib[2];
+  uint32_t ni = 0, oj[2], no = 0;
+  for (uint32_t j = 0; j < d.m; ++j) {
+    if (d.loc[j] == kOutside) {
+      oj[no++] = j;
+    } else if (d.loc[j] < kTileLaneBits) {
+      ib[ni++] = Bit{j, 1, d.loc[j]};
+    } else {
+      ib[ni++] = Bit{j, 2 + d.loc[j] - kTileLaneBits, 0};
+    }
+  }
+  if (ni == 2 && (ib[0].kind > ib[1].kind)) std::swap(ib[0], ib[1]);
+  const uint32_t variant = (ni > 0 ? ib[0].kind : 0) | ((ni > 1 ? ib[1].kind : 0) << 4);
+  const uint32_t lp0 = ni > 0 && ib[0].kind == 1 ? ib[0].lp : 0;
+  const uint32_t lp1 = ni > 1 && ib[1].kind == 1 ? ib[1].lp : 0;
+  d.code = variant | lp0 << 8 | lp1 << 12 | no << 16;
+  d.oq = (no > 0 ? d.q[oj[0]] : 0) | (no > 1 ? d.q[oj[1]] : 0) << 8;
+  d.idw = 0;
+  // 4 phases per outside value so, indexed by c; unreachable c hold zeros
+  std::vector<double> tab(2 * 4 * (size_t(1) << no), 0.0);
+  for (uint32_t so = 0; so < (1u << no); ++so)
+    for (uint32_t c = 0; c < 4; ++c) {
+      if ((c >> ni) != 0) continue;
+      uint32_t s = 0;
+      for (uint32_t i = 0; i < ni; ++i) s |= ((c >> i) & 1u) << ib[i].j;
+      for (uint32_t o = 0; o < no; ++o) s |= ((so >> o) & 1u) << oj[o];
+      const size_t at = 2 * (4 * size_t(so) + c);
+      tab[at] = mat[2 * s];
+      tab[at + 1] = mat[2 * s + 1];
+      if (tab[at] == 1.0 && tab[at + 1] == 0.0) d.idw |= 1u << (4 * so + c);
+    }
+  d.tab = uint32_t(add_matrix(tab.data(), tab.size() / 2));
+}
 
 // Emit the ops of one sub-block (gate order over the inside bits in_j, nin of them, with
 // the outside condition cmask/cval).  Inside bits the block is block-diagonal in (no
@@ -462,7 +490,42 @@ void TileBuilder::run_member(const TileOp& m) {
   ops.push_back(m);
 }
 
+// Position of tile-local bit b among the bits that are not the open run's register bits
+// (its lane position once those are its register bits).
+static uint32_t lane_rank(uint32_t b, const uint32_t* reg, uint32_t nreg) {
+  uint32_t r = b;
+  for (uint32_t j = 0; j < nreg; ++j) r -= reg[j] < b ? 1u : 0u;
+  return r;
+}
+
 void TileBuilder::emit(const TileOp& op) {
+  if (op.kind == 0 && op.m == 1 && run_lk > 0) {
+    // a run of free register bits: the op joins the open run or starts one
+    close_block();
+    const uint32_t b = op.lq[0];
+    bool in = false;
+    if (open_run >= 0) {
+      for (uint32_t j = 0; j < run_nreg; ++j) in = in || run_reg[j] == b;
+      if (!in && run_nreg < run_lk) {
+        run_reg[run_nreg++] = b;
+        in = true;
+      }
+      if (!in && lane_rank(b, run_reg, run_nreg) < kRunLaneBits && !((heavy >> b) & 1u))
+        in = true;
+    }
+    if (!in) {
+      close_run();
+      run_reg[0] = b;
+      run_nreg = 1;
+    }
+    TileOp mem = op;
+    uint32_t cm = 0;  // condition bits: the op's inserted bits other than its target
+    for (uint32_t i = 0; i < op.nins; ++i)
+      if (op.ins[i] != op.lq[0]) cm |= 1u << op.ins[i];
+    mem.insm[0] = cm;
+    run_member(mem);
+    return;
+  }
   if (op.kind == 0 && op.m == 1 && (op.lq[0] < kRunLaneBits || op.lq[0] >= kTileLaneBits)) {
     close_block();
     TileOp mem = op;
@@ -498,6 +561,131 @@ void TileBuilder::emit(const TileOp& op) {
   }
   close_block();
   ops.push_back(op);
+}
+
+// Close the open lane run: choose its register bits (those its 1-qubit members claimed,
+// then the in-tile bits its diagonal members use most, then the highest tile bits), record
+// them in the head op (m = 1, lq[0] of them, perm[j] = tile-local position of register bit
+// j, ascending)
+// and rewrite the members in the run's coordinates: lane position (rank among the other
+// tile bits) for the rest, kTileLaneBits + j for register bit j.  With fixed register bits
+// the coordinates are the tile-local bits themselves.
+void TileBuilder::close_run() {
+  if (open_run < 0) return;
+  const size_t h = size_t(open_run);
+  open_run = -1;
+  const uint32_t nmem = ops[h].n_diag;
+  uint32_t map[32];
+  for (uint32_t b = 0; b < 32; ++b) map[b] = b;
+  if (run_lk > 0) {
+    uint32_t use[32] = {0};
+    for (size_t k = h + 1; k <= h + nmem; ++k)
+      if (ops[k].kind == 1)
+        for (uint32_t d = 0; d < ops[k].n_diag; ++d) {
+          const TileDiag& g = diags[ops[k].off + d];
+          for (uint32_t j = 0; j < g.m; ++j)
+            if (g.loc[j] != kOutside) use[g.loc[j]]++;
+        }
+    auto taken = [&](uint32_t b) {
+      for (uint32_t j = 0; j < run_nreg; ++j)
+        if (run_reg[j] == b) return true;
+      return false;
+    };
+    while (run_nreg < run_lk) {
+      uint32_t best = 32;
+      for (uint32_t b = 0; b < tile_bits; ++b)
+        if (!taken(b) && use[b] > 0 && (best == 32 || use[b] > use[best])) best = b;
+      if (best == 32)  // the highest free tile bit
+        for (uint32_t b = tile_bits; b-- > 0;)
+          if (!taken(b)) {
+            best = b;
+            break;
+          }
+      run_reg[run_nreg++] = best;
+    }
+    std::sort(run_reg, run_reg + run_nreg);
+    for (uint32_t b = 0; b < tile_bits; ++b) map[b] = lane_rank(b, run_reg, run_nreg);
+    for (uint32_t j = 0; j < run_nreg; ++j) map[run_reg[j]] = kTileLaneBits + j;
+    ops[h].m = 1;
+    ops[h].lq[0] = run_nreg;
+    for (uint32_t j = 0; j < run_nreg; ++j) ops[h].perm[j] = uint8_t(run_reg[j]);
+  }
+  auto remap = [&](uint32_t mask) {
+    uint32_t r = 0;
+    for (uint32_t b = 0; b < 32; ++b)
+      if ((mask >> b) & 1u) r |= 1u << map[b];
+    return r;
+  };
+  for (size_t k = h + 1; k <= h + nmem; ++k) {
+    TileOp& g = ops[k];
+    if (g.kind == 1) {
+      for (uint32_t d = 0; d < g.n_diag; ++d) {
+        TileDiag& dg = diags[g.off + d];
+        for (uint32_t j = 0; j < dg.m; ++j)
+          if (dg.loc[j] != kOutside) dg.loc[j] = map[dg.loc[j]];
+        if (dg.m <= 2) finish_small_diag(dg, diag_phases[g.off + d]);
+      }
+    } else {
+      g.lq[0] = map[g.lq[0]];
+      g.insm[0] = remap(g.insm[0]);
+      g.lcval = remap(g.lcval);
+    }
+  }
+  run_nreg = 0;
+  // Stretches of diagonal gates that are one phase controlled by one outside bit, on the
+  // same index bit (a controlled phase whose control is outside the tile: QFT's ladders)
+  // become kind 5 members: no per-gate lowering variant to decode, the phases side by side.
+  const std::vector<TileOp> mem(ops.begin() + std::ptrdiff_t(h) + 1, ops.end());
+  ops.resize(h + 1);
+  for (const TileOp& g : mem) {
+    if (g.kind != 1) {
+      ops.push_back(g);
+      continue;
+    }
+    uint32_t d = 0;
+    while (d < g.n_diag) {
+      uint32_t e = d;
+      while (e < g.n_diag && ctrl_phase_bit(diags[g.off + e]) < 0) ++e;
+      if (e > d) {
+        TileOp t = g;
+        t.off = g.off + d;
+        t.n_diag = e - d;
+        ops.push_back(t);
+      }
+      if (e == g.n_diag) break;
+      const int bit = ctrl_phase_bit(diags[g.off + e]);
+      std::vector<double> ph;
+      uint32_t f = e;
+      for (; f < g.n_diag && ctrl_phase_bit(diags[g.off + f]) == bit; ++f) {
+        const size_t at = 2 * (size_t(diags[g.off + f].tab) + 5);  // so = 1, c = 1
+        ph.push_back(pool[at]);
+        ph.push_back(pool[at + 1]);
+      }
+      TileOp t{};
+      t.kind = 5;
+      t.off = g.off + e;
+      t.n_diag = f - e;
+      t.lq[0] = uint32_t(bit);
+      t.ins[0] = uint32_t(add_matrix(ph.data(), f - e));
+      t.cmask = g.cmask;
+      t.cval = g.cval;
+      ops.push_back(t);
+      d = f;
+    }
+  }
+  ops[h].n_diag = uint32_t(ops.size() - h - 1);
+}
+
+// The index bit (run coordinates) of a lowered diagonal gate that multiplies by one phase
+// exactly the amplitudes whose one outside bit and one in-tile bit are both 1 (every other
+// phase of its table exactly 1 + 0i), else -1.
+int TileBuilder::ctrl_phase_bit(const TileDiag& d) const {
+  if (d.m > 2 || ((d.code >> 16) & 3u) != 1u) return -1;
+  const uint32_t kind0 = d.code & 15u, kind1 = (d.code >> 4) & 15u;
+  if (kind0 == 0 || kind1 != 0) return -1;
+  // identities: so = 0 at c = 0, 1; so = 1 at c = 0, not at c = 1
+  if ((d.idw & 0x33u) != 0x13u) return -1;
+  return kind0 == 1 ? int((d.code >> 8) & 15u) : int(kTileLaneBits + kind0 - 2);
 }
 
 // Finalise the open register block: its bits (ascending) become lq / ins of the block

@@ -41,7 +41,7 @@ void tile_pass_features(const TileBuilder& b, const TilePassArg& pa, uint32_t n,
                         float* f) {
   float cnt[kTileFeatures] = {0};
   auto diag = [&](const TileDiag& d) {
-    const uint32_t k0 = d.variant & 15u;
+    const uint32_t k0 = d.code & 15u;  // the lowering variant's first index bit
     cnt[d.m > 2 ? 6 : k0 == 0 ? 3 : k0 == 1 ? 4 : 5] += 1.f;
   };
   const uint32_t end = pa.op_begin + pa.n_ops;
@@ -53,6 +53,10 @@ void tile_pass_features(const TileBuilder& b, const TilePassArg& pa, uint32_t n,
         const TileOp& g = b.ops[o + u];
         if (g.kind == 1) {
           for (uint32_t d = 0; d < g.n_diag; ++d) diag(b.diags[g.off + d]);
+          continue;
+        }
+        if (g.kind == 5) {  // controlled phases: lane- or register-bit index variants
+          cnt[g.lq[0] >= kTileLaneBits ? 5 : 4] += float(g.n_diag);
           continue;
         }
         const bool cheap = g.real != 0;

@@ -56,11 +56,28 @@ struct TileBuilder {
   uint32_t block_bits = 0;  // tile-local bits of the open block
   void emit(const TileOp& op);
   void close_block();
-  // Diagonal gates and 1-qubit ops on lane bits (< 6) or register bits (>= kTileLaneBits)
-  // are gathered, in order, under one lane-run op (kind 4) whose members follow it.
+  // Diagonal gates and 1-qubit ops on lane bits (< 4) or register bits are gathered, in
+  // order, under one lane-run op (kind 4) whose members follow it.  A lane run may choose
+  // its own register bits (run_lk of them; 0: the tile-local bits from kTileLaneBits up):
+  // a 1-qubit op on a bit outside them joins while there is room, or as a lane member when
+  // its lane position is below 4 and the coming diagonal gates do not lean on that bit
+  // (`heavy`); otherwise it starts a new run.  Members are lowered in tile-local bits and
+  // rewritten in the run's coordinates (lane bits 0..8, register bits from kTileLaneBits
+  // up) when the run closes.
   int64_t open_run = -1;  // index of the open lane-run op, or -1
+  uint32_t kernel_tb = 0;  // tile bits of the kernel running the passes (0: each pass's own)
+  uint32_t run_lk = 0;     // register bits the current pass's runs choose (0: fixed)
+  uint32_t tile_bits = 0;  // tile bits of the current pass
+  uint32_t run_reg[4] = {0, 0, 0, 0};  // the open run's register bits (tile-local)
+  uint32_t run_nreg = 0;
+  uint64_t heavy = 0;  // tile-local bits the diagonal gates after the current gate lean on
+  // the phases of each diagonal gate of <= 2 qubits (parallel to diags), lowered to its
+  // phase table when its run closes
+  std::vector<std::vector<double>> diag_phases;
   void run_member(const TileOp& m);
-  void close_run() { open_run = -1; }
+  void close_run();
+  void finish_small_diag(TileDiag& d, const std::vector<double>& mat);
+  int ctrl_phase_bit(const TileDiag& d) const;
 };
 
 // Tile schedule of one circuit's classified launches on n qubits with tb tile bits

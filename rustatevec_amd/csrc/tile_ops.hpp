@@ -57,6 +57,60 @@ __device__ __forceinline__ void cmul(T& xr, T& xi, T dr, T di) {
   xi = b;
 }
 
+// cmul in place, for multiplies under a branch (a phase skipped when it is exactly 1): the
+// result lands in the registers of x, so the join after the branch needs no copies (written
+// in C, the product goes to new registers and both arms copy into them, two moves per
+// amplitude whether or not it is multiplied).  The operations and their order are cmul's:
+// dr xr - di xi, dr xi + di xr.  S: the phase is uniform (scalar registers).
+template <bool S>
+__device__ __forceinline__ void cmul_ip(double& xr, double& xi, double dr, double di) {
+  double t0, t1;
+  if constexpr (S)
+    asm volatile(
+        "v_mul_f64 %2, %4, %0\n\t"
+        "v_mul_f64 %3, %5, %0\n\t"
+        "v_mul_f64 %0, %5, %1\n\t"
+        "v_mul_f64 %1, %4, %1\n\t"
+        "v_add_f64 %0, %2, -%0\n\t"
+        "v_add_f64 %1, %1, %3"
+        : "+v"(xr), "+v"(xi), "=&v"(t0), "=&v"(t1)
+        : "s"(dr), "s"(di));
+  else
+    asm volatile(
+        "v_mul_f64 %2, %4, %0\n\t"
+        "v_mul_f64 %3, %5, %0\n\t"
+        "v_mul_f64 %0, %5, %1\n\t"
+        "v_mul_f64 %1, %4, %1\n\t"
+        "v_add_f64 %0, %2, -%0\n\t"
+        "v_add_f64 %1, %1, %3"
+        : "+v"(xr), "+v"(xi), "=&v"(t0), "=&v"(t1)
+        : "v"(dr), "v"(di));
+}
+template <bool S>
+__device__ __forceinline__ void cmul_ip(float& xr, float& xi, float dr, float di) {
+  float t0, t1;
+  if constexpr (S)
+    asm volatile(
+        "v_mul_f32 %2, %4, %0\n\t"
+        "v_mul_f32 %3, %5, %0\n\t"
+        "v_mul_f32 %0, %5, %1\n\t"
+        "v_mul_f32 %1, %4, %1\n\t"
+        "v_sub_f32 %0, %2, %0\n\t"
+        "v_add_f32 %1, %1, %3"
+        : "+v"(xr), "+v"(xi), "=&v"(t0), "=&v"(t1)
+        : "s"(dr), "s"(di));
+  else
+    asm volatile(
+        "v_mul_f32 %2, %4, %0\n\t"
+        "v_mul_f32 %3, %5, %0\n\t"
+        "v_mul_f32 %0, %5, %1\n\t"
+        "v_mul_f32 %1, %4, %1\n\t"
+        "v_sub_f32 %0, %2, %0\n\t"
+        "v_add_f32 %1, %1, %3"
+        : "+v"(xr), "+v"(xi), "=&v"(t0), "=&v"(t1)
+        : "v"(dr), "v"(di));
+}
+
 // First tile-local index of block `blk` of a dense or permutation op: zeros inserted at
 // the op's action and condition bits (ascending), condition bits set to their values.
 __device__ __forceinline__ uint32_t op_base(uint32_t blk, const TileOp& op) {
@@ -235,10 +289,10 @@ __device__ __forceinline__ void diag_uniform(T (&xr)[KMAX], T (&xi)[KMAX], uint3
   static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
     constexpr int k = kc.value;
     const int c = cidx(kc);
-    if (!((id >> c) & 1u) && threadIdx.x + uint32_t(k) * NT < N) {
-      asm volatile("");  // keeps the uniform skip a branch (no speculated multiply + select)
-      cmul(xr[k], xi[k], pr[c], pi[c]);
-    }
+    // (the in-place asm multiply also keeps the uniform skip a branch: no speculated
+    // multiply + select)
+    if (!((id >> c) & 1u) && threadIdx.x + uint32_t(k) * NT < N)
+      cmul_ip<true>(xr[k], xi[k], pr[c], pi[c]);
   });
 }
 
@@ -257,31 +311,56 @@ __device__ __forceinline__ void diag_lane(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t
   static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
     constexpr int k = kc.value;
     constexpr bool hi = KA >= 0 && ((k >> (KA < 0 ? 0 : KA)) & 1);
-    if (!(hi ? skip1 : skip0) && threadIdx.x + uint32_t(k) * NT < N) {
-      asm volatile("");
-      cmul(xr[k], xi[k], hi ? p1r : p0r, hi ? p1i : p0i);
-    }
+    if (!(hi ? skip1 : skip0) && threadIdx.x + uint32_t(k) * NT < N)
+      cmul_ip<false>(xr[k], xi[k], hi ? p1r : p0r, hi ? p1i : p0i);
   });
+}
+
+// The words of a TileDiag a gate's multiply needs, and its 4 phases for the tile (the
+// table of its outside bits' values): one scalar load each.
+struct DiagHdr {
+  uint32_t m, code, oq, tab, idw, pad[3];
+};
+template <typename T>
+struct DiagPh {
+  T r[4], i[4];
+};
+
+__device__ __forceinline__ DiagHdr load_hdr(const TileDiag* __restrict__ dg) {
+  return *reinterpret_cast<const DiagHdr*>(dg);
+}
+
+template <typename T>
+__device__ __forceinline__ DiagPh<T> load_ph(const DiagHdr& h, uint64_t base,
+                                             const T* __restrict__ pool) {
+  const uint32_t nout = (h.code >> 16) & 3u;
+  uint32_t so = 0;
+  if (nout > 0) so = uint32_t((base >> (h.oq & 63u)) & 1u);
+  if (nout > 1) so |= uint32_t((base >> ((h.oq >> 8) & 63u)) & 1u) << 1;
+  const T* __restrict__ tab = pool + 2 * (h.tab + 4 * so);
+  DiagPh<T> p;
+  static_for<0, 4>([&](auto c) __attribute__((always_inline)) {
+    p.r[c.value] = tab[2 * c.value];
+    p.i[c.value] = tab[2 * c.value + 1];
+  });
+  return p;
 }
 
 template <typename T, int NT, int KMAX>
 __device__ __forceinline__ void diag_gate_small(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N,
-                                                uint64_t base, const TileDiag* __restrict__ dg,
-                                                const T* __restrict__ pool) {
+                                                uint64_t base, const DiagHdr& h,
+                                                const DiagPh<T>& ph) {
   constexpr int LK = int(ilog2c(KMAX));
-  const uint32_t nout = dg->nout;
+  const uint32_t nout = (h.code >> 16) & 3u;
   uint32_t so = 0;
-  if (nout > 0) so = uint32_t((base >> dg->oq0) & 1u);
-  if (nout > 1) so |= uint32_t((base >> dg->oq1) & 1u) << 1;
-  const T* __restrict__ tab = pool + 2 * (dg->tab + 4 * so);
-  T er[4], ei[4];
-  static_for<0, 4>([&](auto c) __attribute__((always_inline)) {
-    er[c.value] = tab[2 * c.value];
-    ei[c.value] = tab[2 * c.value + 1];
-  });
-  const uint32_t id = (dg->idw >> (4 * so)) & 15u;
-  const uint32_t v = dg->variant;
+  if (nout > 0) so = uint32_t((base >> (h.oq & 63u)) & 1u);
+  if (nout > 1) so |= uint32_t((base >> ((h.oq >> 8) & 63u)) & 1u) << 1;
+  const T(&er)[4] = ph.r;
+  const T(&ei)[4] = ph.i;
+  const uint32_t id = (h.idw >> (4 * so)) & 15u;
+  const uint32_t v = h.code & 255u;
   const uint32_t kind0 = v & 15u, kind1 = v >> 4;
+  const uint32_t lp0 = (h.code >> 8) & 15u, lp1 = (h.code >> 12) & 15u;
   if (kind0 == 0) {  // one phase for the whole tile
     diag_uniform<T, NT, KMAX, -1, -1>(xr, xi, N, er, ei, id);
   } else if (kind0 >= 2) {  // register bits only: phases uniform for each k
@@ -298,7 +377,7 @@ __device__ __forceinline__ void diag_gate_small(T (&xr)[KMAX], T (&xi)[KMAX], ui
       });
     }
   } else {  // index bit 0 is a lane bit
-    const bool l0 = (threadIdx.x >> dg->lp0) & 1u;
+    const bool l0 = (threadIdx.x >> lp0) & 1u;
     if (kind1 == 0) {
       if ((id & 3u) != 3u) {
         const T pr = l0 ? er[1] : er[0], pi = l0 ? ei[1] : ei[0];
@@ -306,7 +385,7 @@ __device__ __forceinline__ void diag_gate_small(T (&xr)[KMAX], T (&xi)[KMAX], ui
       }
     } else if (kind1 == 1) {
       if (id != 15u) {
-        const uint32_t c = uint32_t(l0) | (((threadIdx.x >> dg->lp1) & 1u) << 1);
+        const uint32_t c = uint32_t(l0) | (((threadIdx.x >> lp1) & 1u) << 1);
         const T pr = sel4(er, c), pi = sel4(ei, c);
         diag_lane<T, NT, KMAX, -1>(xr, xi, N, pr, pi, pr, pi, false, false);
       }
@@ -343,6 +422,80 @@ __device__ __forceinline__ void diag_gate_wide(T (&xr)[KMAX], T (&xi)[KMAX], uin
   });
 }
 
+// Diagonal gates dgs[0 .. n) in order on the lane's registers: per gate one scalar load of
+// its words, then one of its phases.  (Issuing a gate's loads one gate ahead was measured
+// slower: QFT(30) complex128 tile passes 232 -> 283 ms.)
+template <typename T, int NT, int KMAX>
+__device__ __forceinline__ void diag_gates(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N,
+                                           uint64_t base, const TileDiag* __restrict__ dgs,
+                                           uint32_t n, const T* __restrict__ pool) {
+  for (uint32_t d = 0; d < n; ++d) {
+    const DiagHdr h = load_hdr(dgs + d);
+    if (h.m <= 2)
+      diag_gate_small<T, NT, KMAX>(xr, xi, N, base, h, load_ph<T>(h, base, pool));
+    else
+      diag_gate_wide<T, NT, KMAX>(xr, xi, N, base, dgs + d, pool);
+  }
+}
+
+// Kind 5 lane-run member: n phases, each controlled by one outside bit (diags[off + d].oq)
+// on the run index bit b: where the tile's base has bit oq set, the amplitudes whose bit b is
+// 1 are multiplied by pool[ins[0] + d] -- the one phase of the gate's table that is not
+// exactly 1 (the identities' multiplies are left out, as in diag_gate_small).  Each gate
+// costs a bit test and the multiplies; its two loads do not depend on each other, and four
+// gates' loads are issued together.
+template <typename T, int NT, int KMAX>
+__device__ __forceinline__ void ctrl_phases(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N,
+                                            uint64_t base, const TileOp& g,
+                                            const TileDiag* __restrict__ diags,
+                                            const T* __restrict__ pool) {
+  constexpr int LK = int(ilog2c(KMAX));
+  const TileDiag* __restrict__ dg = diags + g.off;
+  const T* __restrict__ ph = pool + 2 * size_t(g.ins[0]);
+  const uint32_t n = g.n_diag, b = g.lq[0];
+  auto each = [&](auto&& mul) __attribute__((always_inline)) {
+    uint32_t d = 0;
+    for (; d + 4 <= n; d += 4) {
+      uint32_t oq[4];
+      T pr[4], pi[4];
+      static_for<0, 4>([&](auto u) __attribute__((always_inline)) {
+        oq[u.value] = dg[d + u.value].oq;
+        pr[u.value] = ph[2 * (d + u.value)];
+        pi[u.value] = ph[2 * (d + u.value) + 1];
+      });
+      static_for<0, 4>([&](auto u) __attribute__((always_inline)) {
+        if ((base >> (oq[u.value] & 63u)) & 1u) mul(pr[u.value], pi[u.value]);
+      });
+    }
+    for (; d < n; ++d) {
+      const uint32_t oq = dg[d].oq;
+      const T pr = ph[2 * d], pi = ph[2 * d + 1];
+      if ((base >> (oq & 63u)) & 1u) mul(pr, pi);
+    }
+  };
+  if (b >= kTileLaneBits) {
+    if constexpr (LK > 0)
+      with_kpos<LK>(b - kTileLaneBits, [&](auto P) __attribute__((always_inline)) {
+        each([&](T pr, T pi) __attribute__((always_inline)) {
+          static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = kc.value;
+            if constexpr ((k >> P.value) & 1)
+              if (threadIdx.x + uint32_t(k) * NT < N) cmul_ip<true>(xr[k], xi[k], pr, pi);
+          });
+        });
+      });
+  } else {
+    const bool on = (threadIdx.x >> b) & 1u;
+    each([&](T pr, T pi) __attribute__((always_inline)) {
+      if (on)
+        static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
+          constexpr int k = kc.value;
+          if (threadIdx.x + uint32_t(k) * NT < N) cmul_ip<true>(xr[k], xi[k], pr, pi);
+        });
+    });
+  }
+}
+
 // A run of diagonal gates.  Each lane keeps its KMAX amplitudes (e = tid + k NT) in
 // registers for the whole run and multiplies them by every gate's phase in order -- the
 // multiply of diag_chunk, in the same order.
@@ -359,13 +512,7 @@ __device__ __forceinline__ void apply_diag_run(T* sr, T* si, uint32_t N, uint64_
       xi[kc.value] = si[e];
     }
   });
-  for (uint32_t d = 0; d < op.n_diag; ++d) {
-    const TileDiag* dg = diags + op.off + d;
-    if (dg->m <= 2)
-      diag_gate_small<T, NT, KMAX>(xr, xi, N, base, dg, pool);
-    else
-      diag_gate_wide<T, NT, KMAX>(xr, xi, N, base, dg, pool);
-  }
+  diag_gates<T, NT, KMAX>(xr, xi, N, base, diags + op.off, op.n_diag, pool);
   static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
     const uint32_t e = threadIdx.x + uint32_t(kc.value) * NT;
     if (e < N) {
@@ -715,12 +862,35 @@ __device__ __forceinline__ void apply_lane_run(T* sr, T* si, uint32_t N, uint64_
                                                const T* __restrict__ pool) {
   constexpr int LB = int(ilog2c(NT));    // lane + wave bits
   constexpr int LK = int(ilog2c(KMAX));  // register bits
+  // Run coordinates u = tid + k NT: with the run's own register bits (op.m = 1; tile-local
+  // positions perm[0..lq[0]), ascending, lq[0] <= LK register bits, N = NT 2^lq[0]) register
+  // bit j of k is tile-local bit perm[j] and the lane's bits fill the other positions in
+  // order; registers k >= 2^lq[0] hold nothing (u >= N).
+  const bool own = op.m != 0;
+  const uint32_t nreg = op.lq[0];
+  uint32_t lane_e = threadIdx.x;
+  if (own) {
+    static_for<0, LK>([&](auto jc) __attribute__((always_inline)) {
+      const uint32_t p = op.perm[jc.value];
+      if (uint32_t(jc.value) < nreg) lane_e += lane_e & ~((1u << p) - 1u);
+    });
+  }
+  auto tile_index = [&](auto kc) __attribute__((always_inline)) {
+    constexpr int k = decltype(kc)::value;
+    if (!own) return threadIdx.x + uint32_t(k) * NT;
+    uint32_t r = 0;
+    static_for<0, LK>([&](auto jc) __attribute__((always_inline)) {
+      if constexpr ((k >> jc.value) & 1)
+        if (uint32_t(jc.value) < nreg) r |= 1u << op.perm[jc.value];
+    });
+    return lane_e | r;
+  };
   T xr[KMAX], xi[KMAX];
   static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
-    const uint32_t e = threadIdx.x + uint32_t(kc.value) * NT;
+    const uint32_t e = tile_index(kc);
     xr[kc.value] = T(0);
     xi[kc.value] = T(0);
-    if (e < N) {
+    if (threadIdx.x + uint32_t(kc.value) * NT < N) {
       xr[kc.value] = sr[e];
       xi[kc.value] = si[e];
     }
@@ -730,13 +900,11 @@ __device__ __forceinline__ void apply_lane_run(T* sr, T* si, uint32_t N, uint64_
     const TileOp& g = mem[u];
     if ((base & g.cmask) != g.cval) continue;
     if (g.kind == 1) {  // consecutive diagonal gates diags[off .. off + n_diag)
-      for (uint32_t d = 0; d < g.n_diag; ++d) {
-        const TileDiag* dg = diags + g.off + d;
-        if (dg->m <= 2)
-          diag_gate_small<T, NT, KMAX>(xr, xi, N, base, dg, pool);
-        else
-          diag_gate_wide<T, NT, KMAX>(xr, xi, N, base, dg, pool);
-      }
+      diag_gates<T, NT, KMAX>(xr, xi, N, base, diags + g.off, g.n_diag, pool);
+      continue;
+    }
+    if (g.kind == 5) {  // phases controlled by outside bits
+      ctrl_phases<T, NT, KMAX>(xr, xi, N, base, g, diags, pool);
       continue;
     }
     const T* __restrict__ mat = pool + 2 * g.off;
@@ -755,8 +923,8 @@ __device__ __forceinline__ void apply_lane_run(T* sr, T* si, uint32_t N, uint64_
     });
   }
   static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
-    const uint32_t e = threadIdx.x + uint32_t(kc.value) * NT;
-    if (e < N) {
+    const uint32_t e = tile_index(kc);
+    if (threadIdx.x + uint32_t(kc.value) * NT < N) {
       sr[e] = xr[kc.value];
       si[e] = xi[kc.value];
     }

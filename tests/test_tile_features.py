@@ -10,31 +10,33 @@ from rustatevec_amd import planner
 def test_features_count_every_lowered_op():
     # n = 16, 12 tile bits: the scheduler takes bits 0..3, the action qubits in order and
     # fills up with the lowest bits, so the tile is qubits 0..11.  Tile-local bits 0..3 are
-    # qubits 0..3 (lane bits); the other tile qubits take bits 4..11 least active first
-    # (activity: 2 per non-diagonal op acting on the qubit, 1 per diagonal gate on it):
-    # q8, q9 (0), q4, q5, q6, q7, q11 (2), q10 (3) -> bits 4..11, so q7, q11, q10 sit on
-    # the register bits 9, 10, 11 and q8, q9, q4, q5, q6 on the LDS bits 4..8.
+    # qubits 0..3; the other tile qubits take bits 4..11 least active first (activity: 2
+    # per non-diagonal op acting on the qubit, 1 per diagonal gate on it): q8, q9 (0), q4,
+    # q5, q6, q7, q11 (2), q10 (3) -> bits 4..11.  Lane runs choose 3 register bits each:
+    # a 1-qubit op's bit joins them while there is room, an op whose bit is not among them
+    # and whose lane position would be 4 or more starts a new run, and a run that closes
+    # with room left takes the bits its diagonal gates use most.
     c = rs.Circuit(16)
-    c.h(0)                      # lane-run member, lane bit, real (cheap)
-    c.u3(1, 0.1, 0.2, 0.3)      # lane-run member, lane bit, general
-    c.u3(10, 0.4, 0.5, 0.6)     # lane-run member, register bit, general
-    c.h(11)                     # lane-run member, register bit, cheap
-    c.h(7)                      # lane-run member, register bit, cheap
-    c.rz(3, 0.7)                # diagonal, lane bit
-    c.rz(10, 0.8)               # diagonal, register bit
-    c.rz(14, 0.9)               # diagonal, outside: phase uniform per tile
-    c.gate("ccz", [0, 1, 2])    # diagonal on 3 qubits
-    c.swap(4, 5)                # permutation op (closes the run)
-    c.cx(13, 2)                 # lane-run member, lane bit, swap variant, outside control
-    c.cx(2, 1)                  # lane-run member, lane bit, swap variant, conditioned on bit 2
-    c.h(6)                      # bit 4: a standalone LDS op (closes the run)
+    c.h(0)                      # run 1 (register bits 0, 1, 11): register, real (cheap)
+    c.u3(1, 0.1, 0.2, 0.3)      # run 1: register, general
+    c.u3(10, 0.4, 0.5, 0.6)     # run 1: register, general (bit 11)
+    c.h(11)                     # bit 10: lane position 8, so run 2: register, cheap
+    c.h(7)                      # run 2 (bit 9): register, cheap
+    c.rz(3, 0.7)                # run 2: diagonal on a lane bit
+    c.rz(10, 0.8)               # run 2: diagonal on a lane bit (bit 11, lane position 8)
+    c.rz(14, 0.9)               # run 2: diagonal outside: phase uniform per tile
+    c.gate("ccz", [0, 1, 2])    # run 2: diagonal on 3 qubits (run 2 closes with bits 0, 9, 10)
+    c.swap(4, 5)                # permutation op (closes run 2)
+    c.cx(13, 2)                 # run 3: register, swap variant, outside control
+    c.cx(2, 1)                  # run 3: register, swap variant, conditioned on bit 2
+    c.h(6)                      # run 3 (bits 1, 2, 8): register, cheap
     rows, other = planner.tile_pass_features(c, rs.C128, 12)
     assert rows.shape == (1, planner.N_TILE_FEATURES) and other == 0
     f = rows[0]
     assert f[0] == 0.0 and np.isclose(f[1], 12 / 13) and np.isclose(f[2], 16 / 32)
     counts = np.rint(f[3:] * 16).astype(int)
-    want = {3: 1, 4: 1, 5: 1, 6: 1, 7: 1, 8: 3, 9: 1, 10: 2, 11: 0, 12: 0, 13: 0, 14: 1,
-            15: 1, 16: 2, 17: 1}
+    want = {3: 1, 4: 2, 5: 0, 6: 1, 7: 0, 8: 0, 9: 2, 10: 6, 11: 0, 12: 0, 13: 0, 14: 0,
+            15: 1, 16: 3, 17: 1}
     assert {i + 3: int(v) for i, v in enumerate(counts)} == want
 
 
