@@ -216,6 +216,8 @@ hipError_t launch_mfma_tiles(const State& st, const MtPassArg& pa, const MtOp* o
 //  kind 5 (lane-run member): n_diag phases controlled by one outside bit on the run index
 //  bit lq[0]: gate d (diags[off + d], outside bit oq) multiplies the amplitudes whose bit
 //  lq[0] is 1 by pool[ins[0] + d] in tiles whose base has bit oq set.
+//  kind 6 (lane-run member): n_diag phases on two run index bits: gate d multiplies the
+//  amplitudes whose bits lq[0] and diags[off + d].oq are both 1 by pool[ins[0] + d].
 struct TileOp {
   uint32_t kind;
   uint32_t m;

@@ -643,9 +643,15 @@ void TileBuilder::close_run() {
       continue;
     }
     uint32_t d = 0;
+    auto special = [&](uint32_t i) {
+      uint32_t x, y;
+      return ctrl_phase_bit(diags[g.off + i]) >= 0 ||
+             (ctrl_pair_bits(diags[g.off + i], &x, &y) &&
+              (x >= kTileLaneBits || y >= kTileLaneBits));
+    };
     while (d < g.n_diag) {
       uint32_t e = d;
-      while (e < g.n_diag && ctrl_phase_bit(diags[g.off + e]) < 0) ++e;
+      while (e < g.n_diag && !special(e)) ++e;
       if (e > d) {
         TileOp t = g;
         t.off = g.off + d;
@@ -653,19 +659,42 @@ void TileBuilder::close_run() {
         ops.push_back(t);
       }
       if (e == g.n_diag) break;
-      const int bit = ctrl_phase_bit(diags[g.off + e]);
       std::vector<double> ph;
       uint32_t f = e;
-      for (; f < g.n_diag && ctrl_phase_bit(diags[g.off + f]) == bit; ++f) {
-        const size_t at = 2 * (size_t(diags[g.off + f].tab) + 5);  // so = 1, c = 1
-        ph.push_back(pool[at]);
-        ph.push_back(pool[at + 1]);
-      }
       TileOp t{};
-      t.kind = 5;
+      const int bit = ctrl_phase_bit(diags[g.off + e]);
+      if (bit >= 0) {
+        for (; f < g.n_diag && ctrl_phase_bit(diags[g.off + f]) == bit; ++f) {
+          const size_t at = 2 * (size_t(diags[g.off + f].tab) + 5);  // so = 1, c = 1
+          ph.push_back(pool[at]);
+          ph.push_back(pool[at + 1]);
+        }
+        t.kind = 5;
+        t.lq[0] = uint32_t(bit);
+      } else {
+        // both bits in the tile: the stretch keeps a register bit its gates share (QFT's
+        // ladder target); each gate's other bit goes into its oq
+        uint32_t x, y;
+        ctrl_pair_bits(diags[g.off + e], &x, &y);
+        if (x < kTileLaneBits) std::swap(x, y);  // x: a register bit
+        uint32_t b = x, nx, ny;
+        if (y >= kTileLaneBits && e + 1 < g.n_diag &&
+            ctrl_pair_bits(diags[g.off + e + 1], &nx, &ny) && (y == nx || y == ny) &&
+            x != nx && x != ny)
+          b = y;
+        for (; f < g.n_diag && ctrl_pair_bits(diags[g.off + f], &nx, &ny) && (nx == b || ny == b);
+             ++f) {
+          TileDiag& dg = diags[g.off + f];
+          dg.oq = nx == b ? ny : nx;
+          const size_t at = 2 * (size_t(dg.tab) + 3);  // so = 0, c = 3
+          ph.push_back(pool[at]);
+          ph.push_back(pool[at + 1]);
+        }
+        t.kind = 6;
+        t.lq[0] = b;
+      }
       t.off = g.off + e;
       t.n_diag = f - e;
-      t.lq[0] = uint32_t(bit);
       t.ins[0] = uint32_t(add_matrix(ph.data(), f - e));
       t.cmask = g.cmask;
       t.cval = g.cval;
@@ -674,6 +703,19 @@ void TileBuilder::close_run() {
     }
   }
   ops[h].n_diag = uint32_t(ops.size() - h - 1);
+}
+
+// A lowered diagonal gate that multiplies by one phase exactly the amplitudes whose two
+// in-tile bits are both 1 (every other phase exactly 1 + 0i): true, and its bits in run
+// coordinates.
+bool TileBuilder::ctrl_pair_bits(const TileDiag& d, uint32_t* x, uint32_t* y) const {
+  if (d.m != 2 || ((d.code >> 16) & 3u) != 0u) return false;
+  const uint32_t kind0 = d.code & 15u, kind1 = (d.code >> 4) & 15u;
+  if (kind0 == 0 || kind1 == 0) return false;
+  if ((d.idw & 0xFu) != 0x7u) return false;  // identities at c = 0, 1, 2, not at c = 3
+  *x = kind0 == 1 ? (d.code >> 8) & 15u : kTileLaneBits + kind0 - 2;
+  *y = kind1 == 1 ? (d.code >> 12) & 15u : kTileLaneBits + kind1 - 2;
+  return true;
 }
 
 // The index bit (run coordinates) of a lowered diagonal gate that multiplies by one phase

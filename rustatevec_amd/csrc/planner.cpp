@@ -59,6 +59,11 @@ void tile_pass_features(const TileBuilder& b, const TilePassArg& pa, uint32_t n,
           cnt[g.lq[0] >= kTileLaneBits ? 5 : 4] += float(g.n_diag);
           continue;
         }
+        if (g.kind == 6) {  // phases on two in-tile bits: register-only or with a lane bit
+          for (uint32_t d = 0; d < g.n_diag; ++d)
+            cnt[g.lq[0] >= kTileLaneBits && b.diags[g.off + d].oq >= kTileLaneBits ? 5 : 4] += 1.f;
+          continue;
+        }
         const bool cheap = g.real != 0;
         if (op.kind == 3)
           cnt[cheap ? 13 : 12] += 1.f;

@@ -78,6 +78,7 @@ struct TileBuilder {
   void close_run();
   void finish_small_diag(TileDiag& d, const std::vector<double>& mat);
   int ctrl_phase_bit(const TileDiag& d) const;
+  bool ctrl_pair_bits(const TileDiag& d, uint32_t* x, uint32_t* y) const;
 };
 
 // Tile schedule of one circuit's classified launches on n qubits with tb tile bits

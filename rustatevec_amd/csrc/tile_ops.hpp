@@ -496,6 +496,51 @@ __device__ __forceinline__ void ctrl_phases(T (&xr)[KMAX], T (&xi)[KMAX], uint32
   }
 }
 
+// Kind 6 lane-run member: n phases on two run index bits, the stretch's register bit b and
+// each gate's own bit a (diags[off + d].oq): the amplitudes whose bits a and b are both 1 are
+// multiplied by pool[ins[0] + d] (the identities' multiplies left out, as above).
+template <typename T, int NT, int KMAX>
+__device__ __forceinline__ void pair_phases(T (&xr)[KMAX], T (&xi)[KMAX], uint32_t N,
+                                            const TileOp& g, const TileDiag* __restrict__ diags,
+                                            const T* __restrict__ pool) {
+  constexpr int LK = int(ilog2c(KMAX));
+  const TileDiag* __restrict__ dg = diags + g.off;
+  const T* __restrict__ ph = pool + 2 * size_t(g.ins[0]);
+  const uint32_t n = g.n_diag, b = g.lq[0];
+  uint32_t tid = threadIdx.x;
+  // mul(a, pr, pi): the gate with other bit a and phase pr + i pi
+  auto each = [&](auto&& mul) __attribute__((always_inline)) {
+#pragma unroll 1
+    for (uint32_t d = 0; d < n; ++d) {
+      const uint32_t a = dg[d].oq;
+      mul(a, ph[2 * d], ph[2 * d + 1]);
+    }
+  };
+  // registers k that have bit p set (compile-time) and whose lane or register bit a is 1
+  auto regs = [&](auto P, uint32_t a, T pr, T pi) __attribute__((always_inline)) {
+    if (a >= kTileLaneBits) {
+      const uint32_t q = a - kTileLaneBits;
+      static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = kc.value;
+        if constexpr ((k >> P.value) & 1)
+          if (((uint32_t(k) >> q) & 1u) && threadIdx.x + uint32_t(k) * NT < N)
+            cmul_ip<true>(xr[k], xi[k], pr, pi);
+      });
+    } else if ((tid >> a) & 1u) {
+      static_for<0, KMAX>([&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = kc.value;
+        if constexpr ((k >> P.value) & 1)
+          if (threadIdx.x + uint32_t(k) * NT < N) cmul_ip<true>(xr[k], xi[k], pr, pi);
+      });
+    }
+  };
+  // (the host makes b a register bit)
+  if constexpr (LK > 0)
+    with_kpos<LK>(b - kTileLaneBits, [&](auto P) __attribute__((always_inline)) {
+      each([&](uint32_t a, T pr, T pi) __attribute__((always_inline)) { regs(P, a, pr, pi); });
+    });
+}
+
 // A run of diagonal gates.  Each lane keeps its KMAX amplitudes (e = tid + k NT) in
 // registers for the whole run and multiplies them by every gate's phase in order -- the
 // multiply of diag_chunk, in the same order.
@@ -905,6 +950,10 @@ __device__ __forceinline__ void apply_lane_run(T* sr, T* si, uint32_t N, uint64_
     }
     if (g.kind == 5) {  // phases controlled by outside bits
       ctrl_phases<T, NT, KMAX>(xr, xi, N, base, g, diags, pool);
+      continue;
+    }
+    if (g.kind == 6) {  // phases on two in-tile bits
+      pair_phases<T, NT, KMAX>(xr, xi, N, g, diags, pool);
       continue;
     }
     const T* __restrict__ mat = pool + 2 * g.off;
