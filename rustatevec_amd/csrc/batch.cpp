@@ -92,7 +92,7 @@ int plan(Batch* b, int tb, int blocks) {
   b->tb = tb;
   b->blocks = blocks;
   TileBuilder bld;
-  bld.kernel_tb = uint32_t(tb);  // circuits below the batch's tile keep fixed lane runs
+  bld.free_runs = false;  // the batch kernel runs the interpreter without free lane runs
   std::vector<TilePassArg> args;
   uint32_t n_partials = 0;
   b->order.resize(b->circuits.size());

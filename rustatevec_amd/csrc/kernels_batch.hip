@@ -27,8 +27,12 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
+// Two workgroups per CU (the 64 KiB tiles allow it) need <= 128 VGPRs per lane: without the
+// cap the compiler took 140 and one workgroup per CU ran (512 HEA circuits 53 -> 95 ms).  The
+// interpreter runs without its free lane runs (run_ops<..., false>), which would not fit.
 template <typename T, int MAXTB>
-__global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict__ passes,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(
+    const BatchPass* __restrict__ passes,
                                                     uint32_t n_passes, uint64_t total_work,
                                                     unsigned long long* __restrict__ counter,
                                                     T* __restrict__ amp_pool,
@@ -115,7 +119,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchPass* __restrict_
         }
       }
       __syncthreads();
-      tile::run_ops<T, kThreads, int(NMAX / kThreads)>(sr, si, N, base, ops + bp->op_begin,
+      tile::run_ops<T, kThreads, int(NMAX / kThreads), false>(sr, si, N, base, ops + bp->op_begin,
                                                        bp->n_ops, diags, pool);
       if (flags & kBatchExpZ0) {
         double acc = 0.0;

@@ -149,12 +149,10 @@ void lower_exact_pass(const std::vector<Launch>& in, const std::vector<uint64_t>
   (void)n;
   pa.op_begin = uint32_t(bld.ops.size());
   bld.pass_begin = bld.ops.size();
-  // Lane runs choose their register bits when the pass fills the kernel's tile (a smaller
-  // tile, a batch circuit below the batch's tile size, keeps the fixed layout).
+  // Lane runs choose their register bits where the kernel allows it (TileBuilder::free_runs;
+  // a tile pass kernel's tile is always full, tb bits of the kernel's tb).
   bld.tile_bits = tb;
-  bld.run_lk = (bld.kernel_tb == 0 || bld.kernel_tb == tb) && tb > kTileLaneBits
-                   ? std::min<uint32_t>(tb - kTileLaneBits, 4)
-                   : 0;
+  bld.run_lk = bld.free_runs && tb > kTileLaneBits ? std::min<uint32_t>(tb - kTileLaneBits, 4) : 0;
   for (size_t t = 0; t < take.size(); ++t) {
     const uint64_t i = take[t];
     // the tile bits that the diagonal gates up to the next other op use kHeavyDiag times
@@ -504,6 +502,9 @@ void TileBuilder::emit(const TileOp& op) {
     close_block();
     const uint32_t b = op.lq[0];
     bool in = false;
+    // (a run whose register bits are among tile bits 0..3 loads and stores its amplitudes
+    // 8 or 16 apart in LDS, with bank conflicts; measured on QFT(30) c128, keeping those
+    // bits lane bits instead cost more: 110 -> 122 ms)
     if (open_run >= 0) {
       for (uint32_t j = 0; j < run_nreg; ++j) in = in || run_reg[j] == b;
       if (!in && run_nreg < run_lk) {
@@ -635,6 +636,7 @@ void TileBuilder::close_run() {
   // Stretches of diagonal gates that are one phase controlled by one outside bit, on the
   // same index bit (a controlled phase whose control is outside the tile: QFT's ladders)
   // become kind 5 members: no per-gate lowering variant to decode, the phases side by side.
+  if (!free_runs) return;
   const std::vector<TileOp> mem(ops.begin() + std::ptrdiff_t(h) + 1, ops.end());
   ops.resize(h + 1);
   for (const TileOp& g : mem) {

@@ -65,7 +65,8 @@ struct TileBuilder {
   // rewritten in the run's coordinates (lane bits 0..8, register bits from kTileLaneBits
   // up) when the run closes.
   int64_t open_run = -1;  // index of the open lane-run op, or -1
-  uint32_t kernel_tb = 0;  // tile bits of the kernel running the passes (0: each pass's own)
+  bool free_runs = true;   // runs may choose register bits and get kind 5, 6 members (the
+                           // kernel runs run_ops<..., true>)
   uint32_t run_lk = 0;     // register bits the current pass's runs choose (0: fixed)
   uint32_t tile_bits = 0;  // tile bits of the current pass
   uint32_t run_reg[4] = {0, 0, 0, 0};  // the open run's register bits (tile-local)

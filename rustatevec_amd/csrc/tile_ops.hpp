@@ -899,7 +899,10 @@ __device__ __forceinline__ void run_1q_reg(T (&xr)[KMAX], T (&xi)[KMAX], uint32_
   });
 }
 
-template <typename T, int NT, int KMAX>
+// FREE: the run may have its own register bits (op.m = 1) and kind 5 and 6 members; a
+// kernel short of registers leaves that code out, and its host lowers with the fixed
+// layout and kind 1 members only (TileBuilder::free_runs).
+template <typename T, int NT, int KMAX, bool FREE = true>
 __device__ __forceinline__ void apply_lane_run(T* sr, T* si, uint32_t N, uint64_t base,
                                                const TileOp& op,
                                                const TileOp* __restrict__ mem,
@@ -911,7 +914,7 @@ __device__ __forceinline__ void apply_lane_run(T* sr, T* si, uint32_t N, uint64_
   // positions perm[0..lq[0]), ascending, lq[0] <= LK register bits, N = NT 2^lq[0]) register
   // bit j of k is tile-local bit perm[j] and the lane's bits fill the other positions in
   // order; registers k >= 2^lq[0] hold nothing (u >= N).
-  const bool own = op.m != 0;
+  const bool own = FREE && op.m != 0;
   const uint32_t nreg = op.lq[0];
   uint32_t lane_e = threadIdx.x;
   if (own) {
@@ -948,13 +951,15 @@ __device__ __forceinline__ void apply_lane_run(T* sr, T* si, uint32_t N, uint64_
       diag_gates<T, NT, KMAX>(xr, xi, N, base, diags + g.off, g.n_diag, pool);
       continue;
     }
-    if (g.kind == 5) {  // phases controlled by outside bits
-      ctrl_phases<T, NT, KMAX>(xr, xi, N, base, g, diags, pool);
-      continue;
-    }
-    if (g.kind == 6) {  // phases on two in-tile bits
-      pair_phases<T, NT, KMAX>(xr, xi, N, g, diags, pool);
-      continue;
+    if constexpr (FREE) {
+      if (g.kind == 5) {  // phases controlled by outside bits
+        ctrl_phases<T, NT, KMAX>(xr, xi, N, base, g, diags, pool);
+        continue;
+      }
+      if (g.kind == 6) {  // phases on two in-tile bits
+        pair_phases<T, NT, KMAX>(xr, xi, N, g, diags, pool);
+        continue;
+      }
     }
     const T* __restrict__ mat = pool + 2 * g.off;
     const uint32_t p = g.lq[0], cm = g.insm[0], cv = g.lcval;
@@ -981,7 +986,7 @@ __device__ __forceinline__ void apply_lane_run(T* sr, T* si, uint32_t N, uint64_
 }
 
 // Interpret ops[0..n_ops) on the tile (LDS); ends with a workgroup barrier.
-template <typename T, int NT, int KMAX>
+template <typename T, int NT, int KMAX, bool FREE = true>
 __device__ __forceinline__ void run_ops(T* sr, T* si, uint32_t N, uint64_t base,
                                         const TileOp* __restrict__ ops, uint32_t n_ops,
                                         const TileDiag* __restrict__ diags,
@@ -989,7 +994,7 @@ __device__ __forceinline__ void run_ops(T* sr, T* si, uint32_t N, uint64_t base,
   for (uint32_t o = 0; o < n_ops; ++o) {
     const TileOp op = ops[o];
     if (op.kind == 4) {  // lane run; its op.n_diag members follow it
-      apply_lane_run<T, NT, KMAX>(sr, si, N, base, op, ops + o + 1, diags, pool);
+      apply_lane_run<T, NT, KMAX, FREE>(sr, si, N, base, op, ops + o + 1, diags, pool);
       o += op.n_diag;
       __syncthreads();
       continue;
