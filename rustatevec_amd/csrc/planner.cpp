@@ -30,9 +30,14 @@ Launch classify(const Op& op, int dtype);  // api.cpp
 
 constexpr int kFeatures = 12;
 constexpr int kTileFeatures = 18;
+// The pass models see the state size up to 28 qubits: complex64 states from 2^27 amplitudes
+// up live in HBM beyond the 256 MiB infinity cache, where a pass's time per amplitude no
+// longer depends on n, and the models are trained on 24..28-qubit states -- a larger n
+// would only ask them to extrapolate.
+constexpr uint32_t kPassFeatureMaxN = 28;
 
 // Feature vector of one tile pass (the ops schedule_tiles lowered for it): the state's
-// dtype, tile bits and size, then counts (/16) of diagonal gates by lowering variant
+// dtype, tile bits and size (n capped at kPassFeatureMaxN), then counts (/16) of diagonal gates by lowering variant
 // (phase uniform per tile, indexed by lane bits, by register bits only, m > 2), of lane-run
 // 1-qubit members (lane or register target; general or cheap pair variant), register
 // blocks and their members, LDS ops (standalone 1-qubit and dense 2-3 qubit), permutation
@@ -80,7 +85,7 @@ void tile_pass_features(const TileBuilder& b, const TilePassArg& pa, uint32_t n,
   }
   f[0] = float(dtype);
   f[1] = float(pa.tb) / 13.f;
-  f[2] = float(n) / 32.f;
+  f[2] = float(std::min(n, kPassFeatureMaxN)) / 32.f;
   for (int i = 3; i < kTileFeatures; ++i) f[i] = cnt[i] / 16.f;
 }
 
@@ -110,7 +115,7 @@ constexpr int kMtFeatures = 11;
 // dense groups by width (4, 5, 6; /16), single diagonal ops (/32), merged diagonal runs
 // (/16) and their members (/64), the entries of the per-tile tables the merged runs build
 // and of the pass's phase table (/2048), flags for a 5- or 6-qubit widest group (the
-// kernel's occupancy steps) and the state size n/32.
+// kernel's occupancy steps) and the state size n/32 (n capped at kPassFeatureMaxN).
 void mt_pass_features(const MtPlan& plan, size_t i, uint32_t n, float* f) {
   const MtPassArg& pa = plan.passes[i];
   std::fill(f, f + kMtFeatures, 0.f);
@@ -130,7 +135,7 @@ void mt_pass_features(const MtPlan& plan, size_t i, uint32_t n, float* f) {
   f[7] = float(pa.tab_entries) / 2048;
   f[8] = pa.kmax == 5 ? 1.f : 0.f;
   f[9] = pa.kmax == 6 ? 1.f : 0.f;
-  f[10] = float(n) / 32.f;
+  f[10] = float(std::min(n, kPassFeatureMaxN)) / 32.f;
 }
 
 void mt_schedule_features(const Circuit& c, std::vector<float>* rows, std::vector<Launch>* other) {
