@@ -453,18 +453,28 @@ def main():
             runs.append((f"passes_uniform_{f}", pcf,
                          {"precision": rs.PREC_MFMA_BF16X3, "planned": True}))
         runs += fixed_forms
-        legs_ = {}
+        # every program built and judged first, then timed in rounds over all of them (each
+        # program's best round): clock and cache drift over the run reach every form alike
+        progs_ = []
         for name, c_, kw in runs:
             p_ = rs.Program(st_, c_, **kw)
-            s_ = timed_steps(b, st_, p_, 2 if nq_ <= 28 else 1, 1, reset_)
             reset_()
             p_.launch()
             err_, fid_ = b.compare(ref_, st_)
+            progs_.append((name, p_, err_, fid_))
+        best_s = {name: float("inf") for name, _, _, _ in progs_}
+        for r_ in range(3 if nq_ <= 28 else 2):
+            for name, p_, _, _ in progs_:
+                best_s[name] = min(best_s[name],
+                                   timed_steps(b, st_, p_, 1, 1 if r_ == 0 else 0, reset_))
+        legs_ = {}
+        for name, p_, err_, fid_ in progs_:
+            s_ = best_s[name]
             legs_[name] = {"passes": p_.num_kernels(), "circuit_s": round(s_, 5),
                            "speedup_vs_plain_c128": round(plain_s_ / s_, 2),
                            "max_abs_err": err_, "one_minus_fidelity": 1 - fid_}
             log(f"  mix({nq_}) {name}: {s_:.4f} s, err {err_:.1e}")
-            del p_
+        del progs_
         lp = legs_["learned_passes"]
         fixed = {k: v["circuit_s"] for k, v in legs_.items()
                  if not k.startswith("passes_") and k != "learned_passes"}
