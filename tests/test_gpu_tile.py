@@ -201,3 +201,46 @@ def test_lowering_forms(backend):
         want = oracle.execute(oracle.CPU_PARALLEL, n, [ops[i] for i in order], dtype=npdt,
                               start=start)
         assert max_abs_diff(got, want) == 0.0, (n, tile)
+
+
+def test_controlled_phase_members(backend):
+    """Controlled phases get lane-run members of their own -- a phase controlled by a bit
+    outside the tile (one test of the tile's base per gate) and stretches of phases on two
+    in-tile bits sharing one register bit -- and lane runs choose their register bits from
+    the 1-qubit ops and diagonal gates around them.  Ladders of CP / CZ in random partner
+    order, diagonals with a phase on another pattern (not lowered that way) and 1-qubit gates
+    between them, from a random start: bitwise against the oracle on the recorded order."""
+    rng = np.random.default_rng(7)
+    for n, tile, dtype in ((14, 10, rs.C128), (15, 12, rs.C128), (15, 11, rs.C64),
+                           (16, 13, rs.C64)):
+        c = rs.Circuit(n)
+        for j in rng.permutation(n)[: n // 2 + 2]:
+            j = int(j)
+            c.h(j)
+            for k in rng.permutation(n):
+                k = int(k)
+                if k == j:
+                    continue
+                r = rng.uniform()
+                if r < 0.6:
+                    c.cphase(k, j, float(rng.uniform(0, 2 * np.pi)))
+                elif r < 0.8:
+                    c.cz(k, j)
+                else:  # phase where bit j is 1 and bit k is 0: another table pattern
+                    c.push(np.diag([1, np.exp(1j * rng.uniform(0, 6)), 1, 1]), [j, k])
+            c.u3(int(rng.integers(n)), 0.4, 0.5, 0.6)
+        ops = c.ops()
+        st = backend.alloc(n, dtype)
+        backend.init_random_normal(st, n + 1)
+        start = backend.download(st)
+        p = rs.Program(st, c, tile_qubits=tile)
+        p.launch()
+        got = backend.download(st)
+        order = order_of(p)
+        del p
+        st.free()
+        assert sorted(order) == list(range(len(ops)))
+        npdt = np.float64 if dtype == rs.C128 else np.float32
+        want = oracle.execute(oracle.CPU_PARALLEL, n, [ops[i] for i in order], dtype=npdt,
+                              start=start)
+        assert max_abs_diff(got, want) == 0.0, (n, tile)
