@@ -148,6 +148,13 @@ def _ptr(a: np.ndarray) -> C.c_void_p:
     return C.c_void_p(a.ctypes.data)
 
 
+def _ctx_alive(backend) -> bool:
+    """False once the backend's context is destroyed: a garbage-collected cycle may finalise
+    a backend before the objects that use it, whose handles are then dropped (the process
+    is ending) instead of being freed through a destroyed context."""
+    return _lib is not None and getattr(backend, "ctx", None) is not None
+
+
 def _check(rc: int, ctx=None):
     if rc != 0:
         msg = ""
@@ -349,11 +356,11 @@ class State:
             self._h = None
 
     def __del__(self):
-        if getattr(self, "_h", None) is not None and _lib is not None:
+        if getattr(self, "_h", None) is not None and _ctx_alive(getattr(self, "backend", None)):
             for p in list(getattr(self, "_programs", ())):
                 p._destroy()
             _lib.qsvh_state_free(self._h)
-            self._h = None
+        self._h = None
 
 
 class Program:
@@ -377,9 +384,10 @@ class Program:
         return self._h
 
     def _destroy(self):
-        if getattr(self, "_h", None) is not None and _lib is not None:
+        state = getattr(self, "state", None)
+        if getattr(self, "_h", None) is not None and _ctx_alive(getattr(state, "backend", None)):
             _lib.qsvh_program_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def launch(self):
         _check(lib().qsvh_program_launch(self._handle()), self.state.backend.ctx)
@@ -669,9 +677,9 @@ class Batch:
         return [int(x) for x in out[: n.value]]
 
     def __del__(self):
-        if getattr(self, "_h", None) is not None and _lib is not None:
+        if getattr(self, "_h", None) is not None and _ctx_alive(getattr(self, "backend", None)):
             _lib.qsvh_batch_destroy(self._h)
-            self._h = None
+        self._h = None
 
 
 def fuse_groups(circuit: Circuit, max_qubits: int = 4):

@@ -102,6 +102,8 @@ hipError_t issue(const Program& p, const Launch& l, const Tuning& t, hipStream_t
       return launch_diag_sub(st, l.m, l.q, l.fix, l.mat.data(), t, s);
     case Launch::kSwapSub:
       return launch_swap_sub(st, l.q, t, s);
+    case Launch::kBitSwap:
+      return launch_bitswap(st, l.img.data(), t, s);
     default:
       return hipErrorInvalidValue;
   }
@@ -186,11 +188,22 @@ int program_build(State* st, const Circuit& c, const qsvh_program_opts* opts, Pr
     return set_err(st->ctx, kInvalidArg, "tile_qubits and mfma_tiles exclude each other");
   if (mt > 0 && opts->subspace)
     return set_err(st->ctx, kInvalidArg, "subspace launches do not combine with mfma_tiles");
+  // Tile passes and fused groups leave a trailing qubit permutation (a QFT's SWAP network)
+  // to bit-swap passes: one or two HBM round trips instead of passes that need each
+  // swapped pair in one tile, and exact data movement instead of matrix-core products.
+  // The per-gate forms keep every gate a launch of its own.
+  const uint64_t n_ops = p->launches.size();
+  std::vector<Launch> perm;
+  uint64_t tail = n_ops;
+  if (opts && (opts->tile_qubits > 0 || mt > 0 || p->precision != QSVH_PREC_EXACT))
+    tail = strip_perm_suffix(p->launches, st->n, &perm);
   int rc = plan_tiles(p, opts ? opts->tile_qubits : 0);
   if (rc != kOk) return rc;
   rc = mt > 0 ? plan_mfma_tiles(p, mt) : plan_mfma(p);
   if (rc != kOk) return rc;
   if (opts && opts->subspace) plan_subspace(p);
+  for (uint64_t i = tail; i < n_ops; ++i) p->order.push_back(i);
+  for (Launch& l : perm) p->launches.push_back(std::move(l));
   return build_arena(p);
 }
 

@@ -89,9 +89,12 @@ struct MqIndex {
 // One kernel launch of the plain executor.  kCtrl1q: 2x2 matrix `mat` on q[0], applied
 // where the state bits q[1..m) equal `fix`'s bits (a controlled 1-qubit gate on its control
 // subspace); kDiagSub: the phase mat[0..1] on the amplitudes whose bits q[0..m) equal
-// `fix`'s (the one non-unit entry of a diagonal gate).
+// `fix`'s (the one non-unit entry of a diagonal gate); kBitSwap: the amplitude at index i
+// moved to the index whose bit img[b] is bit b of i, img an involution of the state bits.
 struct Launch {
-  enum Kind { k1q, kDiag, kMq, kFusedMfma, kTile, kCtrl1q, kDiagSub, kMfmaTile, kSwapSub } kind;
+  enum Kind {
+    k1q, kDiag, kMq, kFusedMfma, kTile, kCtrl1q, kDiagSub, kMfmaTile, kSwapSub, kBitSwap
+  } kind;
   uint32_t m = 0;
   uint32_t q[kMaxGateQubits] = {0, 0, 0, 0, 0, 0};
   // matrix in f64 (interleaved re, im), row-major; for diagonal launches the 2^m phases.
@@ -99,6 +102,21 @@ struct Launch {
   uint64_t arena_off = 0;  // element offset of the matrix in the device arena
   uint64_t mfma_off = 0;   // byte offset of the block matrix in the program's MFMA buffer
   uint64_t fix = 0;        // kCtrl1q / kDiagSub: required values of the fixed state bits
+  std::vector<uint8_t> img;  // kBitSwap: image of each state bit
+};
+
+// The kernel argument of a kBitSwap launch: the involution's bit pairs split into those
+// among a tile's bits (the low bits 0..lo-1 and their partners) and the others.
+struct BitSwapArg {
+  uint32_t lo = 0, n_hi = 0;  // low tile bits; tile bits above them (partners of low bits)
+  uint32_t hi_bit[8] = {};    // state bits of tile position bits lo, lo + 1, ... (ascending)
+  uint32_t n_sp = 0;          // pairs among the tile bits, as tile position bits
+  uint32_t sp_a[8] = {}, sp_b[8] = {};
+  uint32_t n_op = 0;          // pairs among the other bits, as state bits
+  uint32_t op_a[32] = {}, op_b[32] = {};
+  uint32_t n_o = 0;           // the other bits: state bit of tile number bit k (32-bit
+  uint32_t o_bit[64] = {};    // entries: read by scalar loads)
+  uint64_t n_tiles = 0;       // 2^n_o
 };
 
 // ---- launchers (kernels_plain.hip) ---------------------------------------------------
@@ -119,6 +137,9 @@ hipError_t launch_diag_sub(const State& st, uint32_t m, const uint32_t* q, uint6
                            const double* phase, const Tuning& t, hipStream_t s);
 // kSwapSub: SWAP(q[0], q[1]) on its |01>, |10> amplitudes only.
 hipError_t launch_swap_sub(const State& st, const uint32_t* q, const Tuning& t, hipStream_t s);
+// kBitSwap (img: the involution's image of each of the state's n bits; no launch when it
+// is the identity, hipErrorInvalidValue when it is not an involution).
+hipError_t launch_bitswap(const State& st, const uint8_t* img, const Tuning& t, hipStream_t s);
 hipError_t launch_probs(const State& st, void* out, hipStream_t s);
 // f64 sum of |psi_i|^2 * sign(i & mask) (mask 0: the norm) into *d_result.
 hipError_t launch_reduce(const State& st, uint64_t mask, double* d_partials, int max_partials,

@@ -639,6 +639,124 @@ __global__ __launch_bounds__(kBlock) void k_diag_sub(T* __restrict__ amp, uint64
   }
 }
 
+// ------------------------------------------------------------------------ bit swaps
+// The state bits of disjoint pairs exchanged: the amplitude at index i moves to rho(i)
+// (BitSwapArg).  rho is an involution, so amplitudes trade places in pairs and the pass
+// runs in place.  A tile is the 2^(lo + n_hi) amplitudes over the low bits 0..lo-1 and the
+// partners of those bits; rho maps the tile bits among themselves and the other bits among
+// themselves, so the tile of base b goes to the tile of base rho(b).  The workgroup of the
+// smaller base loads both tiles (runs of 2^lo consecutive amplitudes), then writes each,
+// permuted through LDS, to the other's place; a tile that rho maps to itself is permuted
+// in place.  Pure data movement: the values are copied, not computed.
+__device__ __forceinline__ uint64_t swap_bit_pairs(uint64_t x, uint32_t n, const uint32_t* a,
+                                                   const uint32_t* b) {
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint64_t t = ((x >> a[k]) ^ (x >> b[k])) & 1u;
+    x ^= (t << a[k]) | (t << b[k]);
+  }
+  return x;
+}
+
+template <typename T, int LO, bool NT>
+__global__ __launch_bounds__(kBlock) void k_bitswap(T* __restrict__ amp, BitSwapArg a, int xcd) {
+  constexpr int A = amps_per_chunk<T>();
+  constexpr uint32_t kTile = 1u << (2 * LO);  // amplitudes of the largest tile
+  constexpr int NI = int(kTile / A / kBlock);  // chunks per lane and tile, at most
+  static_assert(NI >= 1, "a tile holds at least one chunk per lane");
+  __shared__ T sbuf[2][kTile][2];
+  const uint32_t chunks = (1u << (a.lo + a.n_hi)) / A;
+  const uint32_t lomask = (1u << a.lo) - 1;
+  // LDS slot of tile position p: the high position bits XORed into the low ones, so the
+  // lanes of a wave meet distinct banks both in position order and in permuted order
+  auto slot = [&](uint32_t p) { return p ^ ((p >> a.lo) & lomask); };
+  // the next tile number from u on whose pair this workgroup moves (the pair's smaller
+  // base is its own), with the two bases; n_tiles when there is none
+  auto next = [&](uint64_t u, uint64_t& bu, uint64_t& bv) {
+    for (; u < a.n_tiles; u += gridDim.x) {
+      bu = 0;  // bit k of the tile number to state bit o_bit[k]
+      for (uint32_t k = 0; k < a.n_o; ++k) bu |= ((u >> k) & 1u) << a.o_bit[k];
+      bv = swap_bit_pairs(bu, a.n_op, a.op_a, a.op_b);
+      if (bv >= bu) break;
+    }
+    return u;
+  };
+  // state offset of each of this lane's chunks within a tile (the low bits as they are,
+  // the tile bits above them to their state bits)
+  uint64_t off[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const uint32_t p = (threadIdx.x + uint32_t(i) * kBlock) * A;
+    uint64_t o = p & lomask;
+    for (uint32_t k = 0; k < a.n_hi; ++k) o |= uint64_t((p >> (a.lo + k)) & 1u) << a.hi_bit[k];
+    off[i] = o;
+  }
+  T r0[NI][A], m0[NI][A], r1[NI][A], m1[NI][A];
+  auto load = [&](uint64_t bu, uint64_t bv) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      if (threadIdx.x + uint32_t(i) * kBlock < chunks) {
+        cload<T, A, NT>(amp, bu + off[i], r0[i], m0[i]);
+        if (bv != bu) cload<T, A, NT>(amp, bv + off[i], r1[i], m1[i]);
+      }
+    }
+  };
+  uint64_t bu = 0, bv = 0;
+  uint64_t u = next(xcd_block(xcd), bu, bv);
+  if (u < a.n_tiles) load(bu, bv);
+  while (u < a.n_tiles) {
+    const bool two = bv != bu;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const uint32_t c = threadIdx.x + uint32_t(i) * kBlock;
+      if (c < chunks) {
+#pragma unroll
+        for (int j = 0; j < A; ++j) {
+          const uint32_t s = slot(c * A + j);
+          sbuf[0][s][0] = r0[i][j];
+          sbuf[0][s][1] = m0[i][j];
+          if (two) {
+            sbuf[1][s][0] = r1[i][j];
+            sbuf[1][s][1] = m1[i][j];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // the next pair's loads go out before this pair's stores (their tiles are other tiles)
+    const uint64_t su = bu, sv = bv;
+    u = next(u + gridDim.x, bu, bv);
+    if (u < a.n_tiles) load(bu, bv);
+    // position p of the tile at sv receives position rho(p) of the tile at su, and the
+    // tile at su position rho(p) of the tile at sv (of itself when sv == su)
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const uint32_t c = threadIdx.x + uint32_t(i) * kBlock;
+      if (c < chunks) {
+        uint32_t src[A];
+#pragma unroll
+        for (int j = 0; j < A; ++j)
+          src[j] = slot(uint32_t(swap_bit_pairs(c * A + j, a.n_sp, a.sp_a, a.sp_b)));
+        T r[A], m[A];
+#pragma unroll
+        for (int j = 0; j < A; ++j) {
+          r[j] = sbuf[0][src[j]][0];
+          m[j] = sbuf[0][src[j]][1];
+        }
+        cstore<T, A, NT>(amp, sv + off[i], r, m);
+        if (two) {
+#pragma unroll
+          for (int j = 0; j < A; ++j) {
+            r[j] = sbuf[1][src[j]][0];
+            m[j] = sbuf[1][src[j]][1];
+          }
+          cstore<T, A, NT>(amp, su + off[i], r, m);
+        }
+      }
+    }
+    __syncthreads();  // the LDS is read before the next pair is written
+  }
+}
+
 // ------------------------------------------------------------------- host helpers
 // A dispatch holds fewer than 2^32 work-items (the packet's grid size is 32-bit), so
 // grids are capped at 2^31 work-items -- a multiple of 8 blocks, keeping the XCD order --
@@ -1004,7 +1122,73 @@ hipError_t launch_swap_sub_t(const State& st, const uint32_t* q, const Tuning& t
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_bitswap_t(const State& st, const uint8_t* img, const Tuning& t,
+                            hipStream_t s) {
+  constexpr int LO = 5;  // tiles of up to 2^10 amplitudes (at least 2^5 in a run)
+  const uint32_t n = st.n;
+  for (uint32_t b = 0; b < n; ++b)
+    if (img[b] >= n || img[img[b]] != b) return hipErrorInvalidValue;
+  BitSwapArg a;
+  // the most low bits whose tile (they and their partners above them) fits the LDS tile
+  uint32_t lo = n < uint32_t(2 * LO) ? n : uint32_t(2 * LO);
+  for (;; --lo) {
+    uint32_t hi = 0;
+    for (uint32_t b = 0; b < lo; ++b) hi += img[b] >= lo ? 1u : 0u;
+    if (lo + hi <= uint32_t(2 * LO)) break;
+  }
+  a.lo = lo;
+  uint32_t pos[64];  // tile position bit of each state bit; ~0u, ~1u: outside the tile
+  for (uint32_t b = 0; b < n; ++b) pos[b] = b < a.lo ? b : ~0u;
+  for (uint32_t x = a.lo; x < n; ++x)  // ascending: the partners of low bits above them
+    if (img[x] < a.lo) {
+      a.hi_bit[a.n_hi] = x;
+      pos[x] = a.lo + a.n_hi++;
+    }
+  // tile number bits: ascending, each followed by its partner, so that the workgroups at
+  // work together cover runs of neighbouring tiles on both sides of their pairs
+  for (uint32_t x = a.lo; x < n; ++x) {
+    if (pos[x] != ~0u) continue;
+    a.o_bit[a.n_o++] = x;
+    pos[x] = ~1u;
+    if (img[x] != x && pos[img[x]] == ~0u) {
+      a.o_bit[a.n_o++] = img[x];
+      pos[img[x]] = ~1u;
+    }
+  }
+  for (uint32_t b = 0; b < n; ++b) {
+    if (img[b] <= b) continue;
+    if (pos[b] < 64) {
+      a.sp_a[a.n_sp] = pos[b];
+      a.sp_b[a.n_sp++] = pos[img[b]];
+    } else {
+      a.op_a[a.n_op] = b;
+      a.op_b[a.n_op++] = img[b];
+    }
+  }
+  if (a.n_sp == 0 && a.n_op == 0) return hipSuccess;
+  a.n_tiles = uint64_t(1) << a.n_o;
+  // a workgroup per tile (one whose pair belongs to the other tile exits at once), or the
+  // tuning's grid-stride cap
+  uint64_t g = a.n_tiles < kMaxStreamBlocks ? a.n_tiles : kMaxStreamBlocks;
+  if (t.stream_max_blocks > 0 && g > uint64_t(t.stream_max_blocks)) g = t.stream_max_blocks;
+  const unsigned grid = unsigned(g);
+  T* amp = static_cast<T*>(st.amp);
+  if (nontemporal_for(st, t, true))
+    k_bitswap<T, LO, true><<<grid, kBlock, 0, s>>>(amp, a, t.xcd);
+  else
+    k_bitswap<T, LO, false><<<grid, kBlock, 0, s>>>(amp, a, t.xcd);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_bitswap(const State& st, const uint8_t* img, const Tuning& t,
+                          hipStream_t s) {
+  if (st.n > 64) return hipErrorInvalidValue;
+  return st.dtype == 0 ? launch_bitswap_t<double>(st, img, t, s)
+                       : launch_bitswap_t<float>(st, img, t, s);
+}
 
 hipError_t launch_swap_sub(const State& st, const uint32_t* q, const Tuning& t, hipStream_t s) {
   if (st.n < 2) return hipErrorInvalidValue;

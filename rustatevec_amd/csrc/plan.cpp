@@ -1353,6 +1353,86 @@ int plan_mfma_tiles(Program* p, int tb) {
   return rc;
 }
 
+// Where each bit of the sub-index goes when l's matrix is a permutation of the basis that
+// moves bits: every column c has one entry exactly 1 + 0i, at row sigma(c) = the sum of
+// bit j of c shifted to tau[j], and exact zeros elsewhere (y[sigma(c)] = x[c]).
+static bool qubit_perm_of(const Launch& l, uint32_t* tau) {
+  if (l.kind != Launch::kMq || l.m < 1 || l.m > kMaxGateQubits) return false;
+  const uint32_t d = 1u << l.m;
+  std::vector<uint32_t> sigma(d);
+  for (uint32_t c = 0; c < d; ++c) {
+    uint32_t ones = 0;
+    for (uint32_t r = 0; r < d; ++r) {
+      const double re = l.mat[2 * (size_t(r) * d + c)], im = l.mat[2 * (size_t(r) * d + c) + 1];
+      if (im != 0.0 || (re != 0.0 && re != 1.0)) return false;
+      if (re == 1.0) {
+        sigma[c] = r;
+        ++ones;
+      }
+    }
+    if (ones != 1) return false;
+  }
+  uint32_t used = 0;
+  for (uint32_t j = 0; j < l.m; ++j) {
+    const uint32_t v = sigma[1u << j];
+    if (v == 0 || (v & (v - 1)) || (used & v)) return false;
+    used |= v;
+    tau[j] = uint32_t(__builtin_ctz(v));
+  }
+  for (uint32_t c = 0; c < d; ++c) {
+    uint32_t want = 0;
+    for (uint32_t j = 0; j < l.m; ++j) want |= ((c >> j) & 1u) << tau[j];
+    if (sigma[c] != want) return false;
+  }
+  return true;
+}
+
+uint64_t strip_perm_suffix(std::vector<Launch>& ls, uint32_t n, std::vector<Launch>* out) {
+  uint64_t first = ls.size();
+  uint32_t tau[kMaxGateQubits];
+  while (first > 0 && qubit_perm_of(ls[first - 1], tau)) --first;
+  if (first == ls.size()) return first;
+  // dest[b]: the state bit that the data of bit b ends at, the launches applied in order
+  std::vector<uint32_t> dest(n);
+  for (uint32_t b = 0; b < n; ++b) dest[b] = b;
+  for (uint64_t i = first; i < ls.size(); ++i) {
+    const Launch& l = ls[i];
+    qubit_perm_of(l, tau);
+    std::vector<uint32_t> g(n);
+    for (uint32_t b = 0; b < n; ++b) g[b] = b;
+    for (uint32_t j = 0; j < l.m; ++j) g[l.q[j]] = l.q[tau[j]];
+    for (uint32_t b = 0; b < n; ++b) dest[b] = g[dest[b]];
+  }
+  ls.resize(first);
+  // dest = beta . alpha with alpha, beta involutions: on each cycle c_0 -> c_1 -> ... of
+  // length k, alpha(c_i) = c_{-i mod k} and beta(c_i) = c_{1-i mod k}
+  std::vector<uint8_t> alpha(n), beta(n);
+  std::vector<bool> seen(n, false);
+  for (uint32_t b = 0; b < n; ++b) {
+    if (seen[b]) continue;
+    std::vector<uint32_t> cyc;
+    for (uint32_t x = b; !seen[x]; x = dest[x]) {
+      seen[x] = true;
+      cyc.push_back(x);
+    }
+    const size_t k = cyc.size();
+    for (size_t i = 0; i < k; ++i) {
+      alpha[cyc[i]] = uint8_t(cyc[(k - i) % k]);
+      beta[cyc[i]] = uint8_t(cyc[(k + 1 - i) % k]);
+    }
+  }
+  for (const std::vector<uint8_t>* inv : {&alpha, &beta}) {
+    bool ident = true;
+    for (uint32_t b = 0; b < n; ++b) ident = ident && (*inv)[b] == b;
+    if (ident) continue;
+    Launch l;
+    l.kind = Launch::kBitSwap;
+    l.img = *inv;
+    out->push_back(std::move(l));
+  }
+  return first;
+}
+
 void release_extra(Program* p) {
   for (void* b : p->owned) hipFree(b);
   p->owned.clear();

@@ -96,6 +96,8 @@ void tile_schedule_features(const Circuit& c, int dtype, uint32_t tb, std::vecto
   std::vector<Launch> in;
   in.reserve(c.ops.size());
   for (const Op& op : c.ops) in.push_back(classify(op, dtype));
+  std::vector<Launch> perm;  // the trailing qubit permutation, as the program runs it
+  strip_perm_suffix(in, c.n, &perm);
   TileBuilder bld;
   std::vector<TilePassArg> passes;
   std::vector<Launch> out;
@@ -107,6 +109,7 @@ void tile_schedule_features(const Circuit& c, int dtype, uint32_t tb, std::vecto
   other->clear();
   for (Launch& l : out)
     if (l.kind != Launch::kTile) other->push_back(std::move(l));
+  for (Launch& l : perm) other->push_back(std::move(l));
 }
 
 constexpr int kMtFeatures = 11;
@@ -142,6 +145,8 @@ void mt_schedule_features(const Circuit& c, std::vector<float>* rows, std::vecto
   std::vector<Launch> in;
   in.reserve(c.ops.size());
   for (const Op& op : c.ops) in.push_back(classify(op, 1));
+  std::vector<Launch> perm;
+  strip_perm_suffix(in, c.n, &perm);
   MtPlan plan;
   build_mt_plan(in, c.n, 12, &plan);
   rows->assign(plan.passes.size() * kMtFeatures, 0.f);
@@ -150,6 +155,24 @@ void mt_schedule_features(const Circuit& c, std::vector<float>* rows, std::vecto
   other->clear();
   for (Launch& l : plan.launches)
     if (l.kind != Launch::kMfmaTile) other->push_back(std::move(l));
+  for (Launch& l : perm) other->push_back(std::move(l));
+}
+
+void group_features(uint32_t n, uint32_t k, const uint32_t* q, bool diag, double density,
+                    float* f);
+
+// Group-model features of a launch no pass holds; a bit-swap pass (one read and one write
+// of every amplitude) is priced as a dense 1-qubit gate on the top qubit.
+static void other_features(const Launch& l, uint32_t n, float* f) {
+  if (l.kind == Launch::kBitSwap) {
+    const uint32_t top = n ? n - 1 : 0;
+    group_features(n, 1, &top, false, 1.0, f);
+    return;
+  }
+  uint32_t q[kMaxGateQubits];
+  std::copy(l.q, l.q + l.m, q);
+  std::sort(q, q + l.m);
+  group_features(n, l.m, q, l.kind == Launch::kDiag, 1.0, f);
 }
 
 // Feature vector of a group: k qubits (ascending), diagonal flag, density of the composed
@@ -443,10 +466,7 @@ int plan_form(Ctx* ctx, Mlp& group_mlp, Mlp& tile_mlp, Mlp* mt_mlp, const Circui
     std::vector<Candidate> cands;
     for (const Launch& l : other) {
       Candidate cd;
-      uint32_t q[kMaxGateQubits];
-      std::copy(l.q, l.q + l.m, q);
-      std::sort(q, q + l.m);
-      group_features(c.n, l.m, q, l.kind == Launch::kDiag, 1.0, cd.f);
+      other_features(l, c.n, cd.f);
       cands.push_back(cd);
     }
     std::vector<double> oc;
@@ -484,10 +504,7 @@ int plan_form(Ctx* ctx, Mlp& group_mlp, Mlp& tile_mlp, Mlp* mt_mlp, const Circui
       std::vector<Candidate> cands;
       for (const Launch& l : other) {
         Candidate cd;
-        uint32_t q[kMaxGateQubits];
-        std::copy(l.q, l.q + l.m, q);
-        std::sort(q, q + l.m);
-        group_features(c.n, l.m, q, l.kind == Launch::kDiag, 1.0, cd.f);
+        other_features(l, c.n, cd.f);
         cands.push_back(cd);
       }
       std::vector<double> oc;

@@ -97,6 +97,11 @@ void plan_subspace(Program* p);
 int plan_mfma(Program* p);
 // Runs of fused groups as MFMA tile passes of 2^tb amplitudes (split-BF16 precision).
 int plan_mfma_tiles(Program* p, int tb);
+// Trailing qubit permutations (dense launches whose matrix only moves the bits of the
+// sub-index, such as the SWAP network that ends a QFT, alone or fused into groups) are
+// removed from `ls`; their product, a permutation of the state bits, is appended to `out`
+// as at most two kBitSwap launches.  Returns the index of the first launch removed.
+uint64_t strip_perm_suffix(std::vector<Launch>& ls, uint32_t n, std::vector<Launch>* out);
 // The host side of plan_mfma_tiles: the passes, their ops and matrix pool, and the launch
 // sequence (kMfmaTile launches and the launches no pass holds).
 struct MtPlan {

@@ -41,13 +41,22 @@ def test_features_count_every_lowered_op():
 
 
 def test_features_of_the_workloads():
-    # QFT(30) complex128 at 12 tile bits: 6 passes holding its 435 CP, 30 H and 15 SWAP
+    # QFT(30) complex128 at 12 tile bits: 4 passes holding its 435 CP and 30 H; the 15
+    # trailing SWAPs are one bit-swap launch outside the passes
     rows, other = planner.tile_pass_features(rs.qft(30), rs.C128, 12)
-    assert rows.shape == (6, planner.N_TILE_FEATURES) and other == 0
+    assert rows.shape == (4, planner.N_TILE_FEATURES) and other == 1
     tot = np.rint(rows[:, 3:] * 16).sum(axis=0).astype(int)
     assert tot[0:4].sum() == 435            # diagonal gates, all lowering variants
     assert tot[4:8].sum() + tot[9:11].sum() + tot[11] == 30  # H: run members, block members, LDS
-    assert tot[12] == 15                    # SWAP permutations
+    assert tot[12] == 0                     # no permutation op left in the passes
+    # with a gate after them the swaps are no longer a trailing permutation: 6 passes hold
+    # the 15 SWAPs as permutation ops
+    ops = rs.qft(30).ops()
+    c = rs.Circuit.from_ops(30, ops + [(rs.gate_matrix("h"), [0])])
+    rows, other = planner.tile_pass_features(c, rs.C128, 12)
+    assert rows.shape == (6, planner.N_TILE_FEATURES) and other == 0
+    tot = np.rint(rows[:, 3:] * 16).sum(axis=0).astype(int)
+    assert tot[12] == 15
     # a gate a tile cannot hold (4-qubit dense) stays outside the passes
     c = rs.Circuit(14)
     q = np.linalg.qr(np.random.default_rng(1).normal(size=(16, 16)))[0]
