@@ -112,11 +112,11 @@ struct BitSwapArg {
   uint32_t hi_bit[8] = {};    // state bits of tile position bits lo, lo + 1, ... (ascending)
   uint32_t n_sp = 0;          // pairs among the tile bits, as tile position bits
   uint32_t sp_a[8] = {}, sp_b[8] = {};
-  uint32_t n_op = 0;          // pairs among the other bits, as state bits
+  uint32_t n_op = 0;          // pairs among the other bits, as state bits (a < b)
   uint32_t op_a[32] = {}, op_b[32] = {};
-  uint32_t n_o = 0;           // the other bits: state bit of tile number bit k (32-bit
-  uint32_t o_bit[64] = {};    // entries: read by scalar loads)
-  uint64_t n_tiles = 0;       // 2^n_o
+  uint32_t n_rest = 0;        // the other bits without a partner (32-bit entries: read by
+  uint32_t rest_bit[64] = {}; // scalar loads)
+  uint64_t n_items = 0;       // pairs of tiles: 2^n_rest (2^p + (2^p - 1) 2^(p-1)), p = n_op
 };
 
 // ---- launchers (kernels_plain.hip) ---------------------------------------------------

@@ -644,8 +644,8 @@ __global__ __launch_bounds__(kBlock) void k_diag_sub(T* __restrict__ amp, uint64
 // (BitSwapArg).  rho is an involution, so amplitudes trade places in pairs and the pass
 // runs in place.  A tile is the 2^(lo + n_hi) amplitudes over the low bits 0..lo-1 and the
 // partners of those bits; rho maps the tile bits among themselves and the other bits among
-// themselves, so the tile of base b goes to the tile of base rho(b).  The workgroup of the
-// smaller base loads both tiles (runs of 2^lo consecutive amplitudes), then writes each,
+// themselves, so the tile of base b goes to the tile of base rho(b).  One workgroup per
+// pair of tiles loads both (runs of 2^lo consecutive amplitudes), then writes each,
 // permuted through LDS, to the other's place; a tile that rho maps to itself is permuted
 // in place.  Pure data movement: the values are copied, not computed.
 __device__ __forceinline__ uint64_t swap_bit_pairs(uint64_t x, uint32_t n, const uint32_t* a,
@@ -669,16 +669,31 @@ __global__ __launch_bounds__(kBlock) void k_bitswap(T* __restrict__ amp, BitSwap
   // LDS slot of tile position p: the high position bits XORed into the low ones, so the
   // lanes of a wave meet distinct banks both in position order and in permuted order
   auto slot = [&](uint32_t p) { return p ^ ((p >> a.lo) & lomask); };
-  // the next tile number from u on whose pair this workgroup moves (the pair's smaller
-  // base is its own), with the two bases; n_tiles when there is none
-  auto next = [&](uint64_t u, uint64_t& bu, uint64_t& bv) {
-    for (; u < a.n_tiles; u += gridDim.x) {
-      bu = 0;  // bit k of the tile number to state bit o_bit[k]
-      for (uint32_t k = 0; k < a.n_o; ++k) bu |= ((u >> k) & 1u) << a.o_bit[k];
-      bv = swap_bit_pairs(bu, a.n_op, a.op_a, a.op_b);
-      if (bv >= bu) break;
+  // Pair w of tiles (the bases bu, bv = rho(bu)): the low n_rest bits of w are the other
+  // bits without a partner; the rest, wl, numbers the values (x, y) of the pairs' bits:
+  // wl < 2^p, the tiles with x = y everywhere (rho maps each to itself), x = wl; beyond,
+  // d = x ^ y != 0 and x with its bit at the lowest set bit of d cleared (the other tile
+  // of the pair has it set), so every pair of tiles has one number.
+  auto pair_of = [&](uint64_t w, uint64_t& bu, uint64_t& bv) {
+    const uint32_t p = a.n_op;
+    const uint64_t wl = w >> a.n_rest;
+    uint64_t d = 0, x = wl;
+    if (p > 0 && wl >= (uint64_t(1) << p)) {
+      const uint64_t r = wl - (uint64_t(1) << p);
+      d = 1 + (r >> (p - 1));
+      const uint64_t x1 = r & ((uint64_t(1) << (p - 1)) - 1);
+      const uint32_t j = uint32_t(__builtin_ctzll(d));
+      x = ((x1 >> j) << (j + 1)) | (x1 & ((uint64_t(1) << j) - 1));
     }
-    return u;
+    uint64_t base = 0;
+    for (uint32_t k = 0; k < a.n_rest; ++k) base |= ((w >> k) & 1u) << a.rest_bit[k];
+    bu = base;
+    bv = base;
+    for (uint32_t k = 0; k < p; ++k) {
+      const uint64_t xk = (x >> k) & 1u, yk = xk ^ ((d >> k) & 1u);
+      bu |= (xk << a.op_a[k]) | (yk << a.op_b[k]);
+      bv |= (yk << a.op_a[k]) | (xk << a.op_b[k]);
+    }
   };
   // state offset of each of this lane's chunks within a tile (the low bits as they are,
   // the tile bits above them to their state bits)
@@ -701,9 +716,12 @@ __global__ __launch_bounds__(kBlock) void k_bitswap(T* __restrict__ amp, BitSwap
     }
   };
   uint64_t bu = 0, bv = 0;
-  uint64_t u = next(xcd_block(xcd), bu, bv);
-  if (u < a.n_tiles) load(bu, bv);
-  while (u < a.n_tiles) {
+  uint64_t w = xcd_block(xcd);
+  if (w < a.n_items) {
+    pair_of(w, bu, bv);
+    load(bu, bv);
+  }
+  while (w < a.n_items) {
     const bool two = bv != bu;
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
@@ -724,8 +742,11 @@ __global__ __launch_bounds__(kBlock) void k_bitswap(T* __restrict__ amp, BitSwap
     __syncthreads();
     // the next pair's loads go out before this pair's stores (their tiles are other tiles)
     const uint64_t su = bu, sv = bv;
-    u = next(u + gridDim.x, bu, bv);
-    if (u < a.n_tiles) load(bu, bv);
+    w += gridDim.x;
+    if (w < a.n_items) {
+      pair_of(w, bu, bv);
+      load(bu, bv);
+    }
     // position p of the tile at sv receives position rho(p) of the tile at su, and the
     // tile at su position rho(p) of the tile at sv (of itself when sv == su)
 #pragma unroll
@@ -1138,39 +1159,32 @@ hipError_t launch_bitswap_t(const State& st, const uint8_t* img, const Tuning& t
     if (lo + hi <= uint32_t(2 * LO)) break;
   }
   a.lo = lo;
-  uint32_t pos[64];  // tile position bit of each state bit; ~0u, ~1u: outside the tile
+  uint32_t pos[64];  // tile position bit of each state bit, or ~0u outside the tile
   for (uint32_t b = 0; b < n; ++b) pos[b] = b < a.lo ? b : ~0u;
   for (uint32_t x = a.lo; x < n; ++x)  // ascending: the partners of low bits above them
     if (img[x] < a.lo) {
       a.hi_bit[a.n_hi] = x;
       pos[x] = a.lo + a.n_hi++;
     }
-  // tile number bits: ascending, each followed by its partner, so that the workgroups at
-  // work together cover runs of neighbouring tiles on both sides of their pairs
-  for (uint32_t x = a.lo; x < n; ++x) {
-    if (pos[x] != ~0u) continue;
-    a.o_bit[a.n_o++] = x;
-    pos[x] = ~1u;
-    if (img[x] != x && pos[img[x]] == ~0u) {
-      a.o_bit[a.n_o++] = img[x];
-      pos[img[x]] = ~1u;
-    }
-  }
   for (uint32_t b = 0; b < n; ++b) {
-    if (img[b] <= b) continue;
-    if (pos[b] < 64) {
-      a.sp_a[a.n_sp] = pos[b];
-      a.sp_b[a.n_sp++] = pos[img[b]];
-    } else {
+    if (pos[b] != ~0u) {
+      if (img[b] > b) {
+        a.sp_a[a.n_sp] = pos[b];
+        a.sp_b[a.n_sp++] = pos[img[b]];
+      }
+    } else if (img[b] == b) {
+      a.rest_bit[a.n_rest++] = b;
+    } else if (img[b] > b) {
       a.op_a[a.n_op] = b;
       a.op_b[a.n_op++] = img[b];
     }
   }
   if (a.n_sp == 0 && a.n_op == 0) return hipSuccess;
-  a.n_tiles = uint64_t(1) << a.n_o;
-  // a workgroup per tile (one whose pair belongs to the other tile exits at once), or the
-  // tuning's grid-stride cap
-  uint64_t g = a.n_tiles < kMaxStreamBlocks ? a.n_tiles : kMaxStreamBlocks;
+  const uint32_t p = a.n_op;
+  a.n_items = (uint64_t(1) << a.n_rest) *
+              ((uint64_t(1) << p) + (p ? ((uint64_t(1) << p) - 1) << (p - 1) : 0));
+  // a workgroup per pair of tiles, or the tuning's grid-stride cap
+  uint64_t g = a.n_items < kMaxStreamBlocks ? a.n_items : kMaxStreamBlocks;
   if (t.stream_max_blocks > 0 && g > uint64_t(t.stream_max_blocks)) g = t.stream_max_blocks;
   const unsigned grid = unsigned(g);
   T* amp = static_cast<T*>(st.amp);
