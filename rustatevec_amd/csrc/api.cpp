@@ -346,7 +346,7 @@ int qsvh_tune_set(qsvh_ctx* h, const char* key, int64_t value) {
     t.mfma_blocks = int(value);
   } else if (k == "mfma_layout" && value >= 0 && value <= 2) {
     t.mfma_layout = int(value);
-  } else if (k == "tile_blocks" && value >= 1 && value <= (1ll << 20)) {
+  } else if (k == "tile_blocks" && value >= 0 && value <= (1ll << 20)) {
     t.tile_blocks = int(value);
   } else if (k == "contig" && (value == 0 || value == 1)) {
     t.contig = int(value);

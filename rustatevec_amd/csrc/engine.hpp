@@ -22,9 +22,13 @@ struct Tuning {
   int mfma_blocks = 0;         // workgroups of the persistent MFMA group kernels; 0: by
                                // form (measured: 1 per CU for FP32 at k = 4, else 2 per
                                // CU; the HBM rate drops with more tiles in flight)
+  // workgroups of the tile-pass kernels (exact and MFMA): 0, one per tile; n > 0: n
+  // workgroups looping over the tiles.  One per tile measured 6-10% faster than a
+  // resident-sized loop (whose next tile's loads wait out the previous tile's stores:
+  // the vector-memory wait counts stores too).
+  int tile_blocks = 0;
   int mfma_layout = 0;         // FP32 group kernel's LDS layout: 0 by the group's qubits,
                                // 1 row-major, 2 column-major
-  int tile_blocks = 1024;      // workgroups of the persistent tile-pass kernel
   int contig = 0;              // 1: a thread's unrolled items are adjacent (else strided)
   // 1: blocks b and b + 1 of a streaming launch take neighbouring address ranges of one
   // XCD (the hardware deals blocks to the 8 XCDs round robin, b -> XCD b % 8).

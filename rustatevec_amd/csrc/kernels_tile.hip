@@ -91,8 +91,11 @@ template <typename T, int TB>
 hipError_t launch_tb(const State& st, const TilePassArg& pa, const TileOp* ops,
                      const TileDiag* diags, const void* pool, int max_blocks, hipStream_t s) {
   const uint64_t n_tiles = st.dim() >> TB;
+  // one workgroup per tile unless max_blocks caps the grid (then they loop), and fewer
+  // than 2^32 work-items per dispatch
   uint64_t g = n_tiles;
-  if (g > uint64_t(max_blocks)) g = max_blocks;
+  if (max_blocks > 0 && g > uint64_t(max_blocks)) g = max_blocks;
+  if (g > (uint64_t(1) << 31) / kThreads) g = (uint64_t(1) << 31) / kThreads;
   k_tile<T, TB><<<unsigned(g), kThreads, 0, s>>>(static_cast<T*>(st.amp), n_tiles, pa, ops,
                                                 diags, static_cast<const T*>(pool));
   return hipGetLastError();
