@@ -1,9 +1,10 @@
 """Trailing qubit permutations in tile-pass and fused programs: the SWAP network that ends a
 circuit (a QFT's), alone or fused into groups whose matrices only move bits, runs as one or
 two bit-swap passes (each exchanges the state bits of disjoint pairs, in place).  A SWAP
-moves amplitudes without arithmetic, so the result is the input permuted exactly: the
-oracle's dense SWAP (products by exact 1 and 0, crates/qsv-core/src/backend/cpu.rs:221-247)
-and a numpy index permutation agree with it bit for bit, in complex128 and complex64, at
+moves amplitudes without arithmetic, so the result is the input permuted exactly: a numpy
+index permutation agrees with it bit for bit, and the oracle's dense SWAP (products by exact
+1 and 0, crates/qsv-core/src/backend/cpu.rs:221-247) in value (the two differ at most in
+the sign of a zero amplitude; the random start states have none), in complex128 and complex64, at
 state sizes from two qubits to a few tiles' bits, for involutions (one pass) and for
 longer cycles of qubits (two passes)."""
 import numpy as np
