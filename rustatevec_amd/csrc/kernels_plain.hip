@@ -11,6 +11,7 @@
 // CpuBackend (crates/qsv-core/src/backend/cpu.rs:71-114,221-247).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "amp.hpp"
@@ -410,6 +411,103 @@ __global__ __launch_bounds__(kBlock) void k_mq_lane(T* __restrict__ amp, uint64_
       yi[j] = acci;
     }
     cstore<T, A, NT>(amp, a, yr, yi);
+  }
+}
+
+// mq_block for complex128 gates on a qubit below 3, whose one-block-per-thread gathers
+// would fetch half or a quarter of each 128-byte line per load: a wave takes the 64
+// consecutive amplitudes of a run (index bits 0..5 are its lanes) for each value of the
+// gate's NH qubits at 6 and above (one load each), gathers every member of its blocks from
+// the lanes holding them (the gate qubits below 6 are lane bits; ds_bpermute), and
+// computes its own rows with k_mq_reg's operations in the same order, so the results are
+// bitwise those of k_mq_reg.  The lane's rows (one per value of the high qubits) are
+// selected from the kernel arguments once.  Launched for m = 2, NH = 1.
+template <int M, int NH>
+struct MqLow {
+  uint32_t hsort[NH > 0 ? NH : 1];  // the high gate qubits, ascending
+  uint64_t hoff[1 << NH];           // offset of high sub-index h (bit i: hsort[i])
+  uint32_t q[M];                    // the gate's qubits in matrix-index order
+  uint32_t hrank[M];                // of a high qubit q[j]: its bit in h
+  uint32_t c_lane[1 << M];          // member c: its lane bits (the low gate qubits)
+  uint32_t c_h[1 << M];             // member c: its high sub-index
+  uint32_t glmask;                  // lane bits of the low gate qubits
+};
+
+template <int M, int NH, bool NT>
+__global__ __launch_bounds__(kBlock) void k_mq_low(double* __restrict__ amp, uint64_t n_runs,
+                                                   MqLow<M, NH> ix, MatArg<double, M> g,
+                                                   int xcd) {
+  constexpr int D = 1 << M, H = 1 << NH;
+  const uint32_t lane = threadIdx.x & 63;
+  // row of each of the lane's amplitudes: bit j from lane bit q[j] or from h
+  double er[H][D], ei[H][D];
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+      r |= (ix.q[j] < 6 ? (lane >> ix.q[j]) & 1u : (uint32_t(h) >> ix.hrank[j]) & 1u) << j;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      double vr = g.r[c], vi = g.i[c];
+#pragma unroll
+      for (int rr = 1; rr < D; ++rr)
+        if (r == uint32_t(rr)) {
+          vr = g.r[rr * D + c];
+          vi = g.i[rr * D + c];
+        }
+      er[h][c] = vr;
+      ei[h][c] = vi;
+    }
+  }
+  const uint32_t keep = lane & ~ix.glmask;
+  const uint64_t nw = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+  for (uint64_t w = (xcd_block(xcd) * blockDim.x + threadIdx.x) >> 6; w < n_runs; w += nw) {
+    uint64_t base = w << 6;
+#pragma unroll
+    for (int i = 0; i < NH; ++i) {
+      const uint32_t b = ix.hsort[i];
+      base = ((base >> b) << (b + 1)) | (base & ((uint64_t(1) << b) - 1));
+    }
+    base += lane;
+    uint32_t v[H][4];  // the lane's amplitude for each high sub-index, as 32-bit words
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      double xr[1], xi[1];
+      cload<double, 1, NT>(amp, base + ix.hoff[h], xr, xi);
+      __builtin_memcpy(v[h], xr, 8);
+      __builtin_memcpy(v[h] + 2, xi, 8);
+    }
+    // member c of each of the lane's blocks: held by lane keep | c_lane[c], in its v[c_h[c]]
+    double cr[D], ci[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const int src = int((keep | ix.c_lane[c]) << 2);
+      uint32_t u[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        uint32_t val = v[0][k];
+#pragma unroll
+        for (int h = 1; h < H; ++h)
+          if (ix.c_h[c] == uint32_t(h)) val = v[h][k];
+        u[k] = uint32_t(__builtin_amdgcn_ds_bpermute(src, int(val)));
+      }
+      __builtin_memcpy(&cr[c], u, 8);
+      __builtin_memcpy(&ci[c], u + 2, 8);
+    }
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      double accr = 0.0, acci = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const double pr = er[h][c] * cr[c] - ei[h][c] * ci[c];
+        const double pi = er[h][c] * ci[c] + ei[h][c] * cr[c];
+        accr = accr + pr;
+        acci = acci + pi;
+      }
+      const double yr[1] = {accr}, yi[1] = {acci};
+      cstore<double, 1, NT>(amp, base + ix.hoff[h], yr, yi);
+    }
   }
 }
 
@@ -929,6 +1027,46 @@ MqIdx<M> make_idx(const uint32_t* q) {
   return idx;
 }
 
+template <int M, int NH>
+hipError_t launch_mq_low(const State& st, const uint32_t* q, const MatArg<double, M>& g,
+                         const Tuning& t, hipStream_t s) {
+  MqLow<M, NH> ix{};
+  int nh = 0;
+  for (int j = 0; j < M; ++j)
+    if (q[j] >= 6) ix.hsort[nh++] = q[j];
+  std::sort(ix.hsort, ix.hsort + NH);
+  for (int h = 0; h < (1 << NH); ++h) {
+    uint64_t o = 0;
+    for (int i = 0; i < NH; ++i) o |= uint64_t((h >> i) & 1) << ix.hsort[i];
+    ix.hoff[h] = o;
+  }
+  for (int j = 0; j < M; ++j) {
+    ix.q[j] = q[j];
+    ix.hrank[j] = 0;
+    for (int i = 0; i < NH; ++i)
+      if (ix.hsort[i] == q[j]) ix.hrank[j] = uint32_t(i);
+    if (q[j] < 6) ix.glmask |= 1u << q[j];
+  }
+  for (int c = 0; c < (1 << M); ++c) {
+    uint32_t ln = 0, h = 0;
+    for (int j = 0; j < M; ++j) {
+      const uint32_t bit = uint32_t(c >> j) & 1u;
+      if (q[j] < 6) ln |= bit << q[j];
+      else h |= bit << ix.hrank[j];
+    }
+    ix.c_lane[c] = ln;
+    ix.c_h[c] = h;
+  }
+  const uint64_t runs = st.dim() >> (6 + NH);
+  const unsigned grid = grid_for(runs * 64, 1, t);
+  double* amp = static_cast<double*>(st.amp);
+  if (nontemporal_for(st, t, true))  // whole 1 KiB runs per load
+    k_mq_low<M, NH, true><<<grid, kBlock, 0, s>>>(amp, runs, ix, g, t.xcd);
+  else
+    k_mq_low<M, NH, false><<<grid, kBlock, 0, s>>>(amp, runs, ix, g, t.xcd);
+  return hipGetLastError();
+}
+
 template <typename T, int M>
 hipError_t launch_mq_reg(const State& st, const uint32_t* q, const double* gd,
                          const Tuning& t, hipStream_t s) {
@@ -947,6 +1085,16 @@ hipError_t launch_mq_reg(const State& st, const uint32_t* q, const double* gd,
   // 30 qubits: 4.7 -> 1.5 TB/s), on whole ones it gains 4-7%
   const bool nt = nontemporal_for(st, t, true) &&
                   (uint64_t(1) << idx.sorted[0]) * 2 * sizeof(T) >= 128;
+  // complex128 2-qubit gates on a qubit below 3 and one at 6 or above: whole runs per wave,
+  // members across lanes (30 qubits: SWAP(0, 29) 5.25 -> 6.48 TB/s, CX(1, 15) 5.55 -> 6.44).
+  // With both qubits below 6 (one load and 16 lane exchanges per amplitude) or three
+  // qubits below 6 (32) the exchanges cost more than the partial lines (U(0, 1) 4.74 ->
+  // 4.20, U(0, 1, 2) 3.68 -> 2.20 TB/s), so those keep one block per thread.
+  if constexpr (sizeof(T) == 8 && M == 2) {
+    const bool high = q[0] >= 6 || q[1] >= 6;
+    if (t.mq_lane && idx.sorted[0] < 3 && high && st.n >= 7)
+      return launch_mq_low<M, 1>(st, q, g, t, s);
+  }
   // the lane form measured faster only for complex64 2-qubit gates (5.8-6.0 against
   // 5.4-5.7 TB/s at 30 qubits); for complex128 its exchanges cost more than they save
   if (t.mq_lane && sizeof(T) == 4 && M == 2 && int(idx.sorted[0]) >= ilog2(A) &&

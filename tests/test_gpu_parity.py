@@ -165,10 +165,12 @@ def random_unitary(d, rng):
 
 @pytest.mark.parametrize("dtype", [rs.C128, rs.C64])
 def test_mq_lane_and_block_kernels_agree_on_every_placement(backend, dtype):
-    """Dense 2- and 3-qubit gates run with one chunk per lane (members gathered across
-    lanes 8, 16, 32 apart) by default and one block per thread with "mq_lane" 0; both must
-    give the oracle's result bitwise for every ordered placement of the gate's qubits
-    (low, high, scattered, reversed), on states from below one wave of blocks upward."""
+    """Dense 2- and 3-qubit gates run with one chunk per lane by default (complex64 2-qubit
+    gates: members gathered across lanes 8, 16, 32 apart; complex128 2-qubit gates on a
+    qubit below 3 and one at 6 or above: runs of 64 amplitudes per wave, members gathered by
+    lane permutes) and one block per thread with "mq_lane" 0; both must give the oracle's
+    result bitwise for every ordered placement of the gate's qubits (low, high, scattered,
+    reversed), on states from below one wave of blocks upward."""
     npt = np.float64 if dtype == rs.C128 else np.float32
     rng = np.random.default_rng(12)
     old = backend.tune()["mq_lane"]
