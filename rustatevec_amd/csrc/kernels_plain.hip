@@ -414,8 +414,9 @@ __global__ __launch_bounds__(kBlock) void k_mq_lane(T* __restrict__ amp, uint64_
   }
 }
 
-// mq_block for complex128 gates on a qubit below 3, whose one-block-per-thread gathers
-// would fetch half or a quarter of each 128-byte line per load: a wave takes the 64
+// mq_block for complex128 gates on a qubit below 6 (one block per thread gathers chunks
+// 2^q0 x 16 bytes apart: below 3, half or a quarter of each 128-byte line per load; from 3
+// to 5, lines in 256-1024-byte steps): a wave takes the 64
 // consecutive amplitudes of a run (index bits 0..5 are its lanes) for each value of the
 // gate's NH qubits at 6 and above (one load each), gathers every member of its blocks from
 // the lanes holding them (the gate qubits below 6 are lane bits; ds_bpermute), and
@@ -1085,14 +1086,16 @@ hipError_t launch_mq_reg(const State& st, const uint32_t* q, const double* gd,
   // 30 qubits: 4.7 -> 1.5 TB/s), on whole ones it gains 4-7%
   const bool nt = nontemporal_for(st, t, true) &&
                   (uint64_t(1) << idx.sorted[0]) * 2 * sizeof(T) >= 128;
-  // complex128 2-qubit gates on a qubit below 3 and one at 6 or above: whole runs per wave,
-  // members across lanes (30 qubits: SWAP(0, 29) 5.25 -> 6.48 TB/s, CX(1, 15) 5.55 -> 6.44).
-  // With both qubits below 6 (one load and 16 lane exchanges per amplitude) or three
-  // qubits below 6 (32) the exchanges cost more than the partial lines (U(0, 1) 4.74 ->
-  // 4.20, U(0, 1, 2) 3.68 -> 2.20 TB/s), so those keep one block per thread.
+  // complex128 2-qubit gates on a qubit below 6 and one at 6 or above: two whole 1 KiB runs
+  // per wave, members across lanes (30 qubits: SWAP(0, 29) 5.25 -> 6.48 TB/s, SWAP(4, 25)
+  // 5.80 -> 6.38, CX(4, 20) 5.67 -> 6.27, CX(3, 17) 6.35 -> 6.61).  With both qubits below
+  // 6 (one load and 16 lane exchanges per amplitude) or three qubits below 6 (32) the
+  // exchanges cost more than the partial lines (U(0, 1) 4.74 -> 4.20, U(0, 1, 2) 3.68 ->
+  // 2.20 TB/s), so those keep one block per thread, as do gates with both qubits at 6 or
+  // above (one block per thread already loads whole runs).
   if constexpr (sizeof(T) == 8 && M == 2) {
     const bool high = q[0] >= 6 || q[1] >= 6;
-    if (t.mq_lane && idx.sorted[0] < 3 && high && st.n >= 7)
+    if (t.mq_lane && idx.sorted[0] < 6 && high && st.n >= 7)
       return launch_mq_low<M, 1>(st, q, g, t, s);
   }
   // the lane form measured faster only for complex64 2-qubit gates (5.8-6.0 against

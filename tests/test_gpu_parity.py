@@ -167,7 +167,7 @@ def random_unitary(d, rng):
 def test_mq_lane_and_block_kernels_agree_on_every_placement(backend, dtype):
     """Dense 2- and 3-qubit gates run with one chunk per lane by default (complex64 2-qubit
     gates: members gathered across lanes 8, 16, 32 apart; complex128 2-qubit gates on a
-    qubit below 3 and one at 6 or above: runs of 64 amplitudes per wave, members gathered by
+    qubit below 6 and one at 6 or above: runs of 64 amplitudes per wave, members gathered by
     lane permutes) and one block per thread with "mq_lane" 0; both must give the oracle's
     result bitwise for every ordered placement of the gate's qubits (low, high, scattered,
     reversed), on states from below one wave of blocks upward."""
