@@ -192,18 +192,32 @@ int program_build(State* st, const Circuit& c, const qsvh_program_opts* opts, Pr
   // to bit-swap passes: one or two HBM round trips instead of passes that need each
   // swapped pair in one tile, and exact data movement instead of matrix-core products.
   // The per-gate forms keep every gate a launch of its own.
+  // The MFMA tile form may run the swaps earlier (PermTail), where its passes come out
+  // fewer (fused QFT(30) at k = 4: 5 passes instead of 6, 43.3 -> 42.2 ms).  The exact tile
+  // form keeps them at the end: moved past the swaps, QFT's controlled-phase ladders index
+  // lane bits instead of bits outside the tile, and 3 passes took longer than 4 (93.4 ->
+  // 96.3 ms in complex128).
   const uint64_t n_ops = p->launches.size();
-  std::vector<Launch> perm;
-  uint64_t tail = n_ops;
-  if (opts && (opts->tile_qubits > 0 || mt > 0 || p->precision != QSVH_PREC_EXACT))
-    tail = strip_perm_suffix(p->launches, st->n, &perm);
-  int rc = plan_tiles(p, opts ? opts->tile_qubits : 0);
+  const int tq = opts ? opts->tile_qubits : 0;
+  PermTail pt;
+  uint64_t tail = n_ops, body = n_ops;
+  if (opts && (tq > 0 || mt > 0 || p->precision != QSVH_PREC_EXACT)) {
+    tail = strip_perm_suffix(p->launches, st->n, &pt.swaps, &pt.dest);
+    pt.k = body = p->launches.size();
+    if (!pt.swaps.empty() && mt > 0 && st->n >= 12)
+      pt.k = hoist_perm(p->launches, pt.dest, [&](const std::vector<Launch>& v) {
+        return mt_launch_count(v, st->n);
+      });
+  }
+  int rc = plan_tiles(p, tq, &pt);
   if (rc != kOk) return rc;
-  rc = mt > 0 ? plan_mfma_tiles(p, mt) : plan_mfma(p);
+  rc = mt > 0 ? plan_mfma_tiles(p, mt, &pt) : plan_mfma(p);
   if (rc != kOk) return rc;
   if (opts && opts->subspace) plan_subspace(p);
+  if (!pt.swaps.empty() && pt.k != body)
+    return set_err(st->ctx, kInvalidArg, "internal: moved bit swaps left unplaced");
   for (uint64_t i = tail; i < n_ops; ++i) p->order.push_back(i);
-  for (Launch& l : perm) p->launches.push_back(std::move(l));
+  for (Launch& l : pt.swaps) p->launches.push_back(std::move(l));  // not placed by a planner
   return build_arena(p);
 }
 

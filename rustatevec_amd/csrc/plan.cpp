@@ -776,7 +776,7 @@ void TileBuilder::close_block() {
   }
 }
 
-int plan_tiles(Program* p, int tile_qubits) {
+int plan_tiles(Program* p, int tile_qubits, PermTail* pt) {
   State& st = *p->st;
   p->order.clear();
   if (tile_qubits <= 0) {
@@ -796,7 +796,18 @@ int plan_tiles(Program* p, int tile_qubits) {
   const std::vector<Launch> in = std::move(p->launches);
   p->launches.clear();
   TileBuilder tbld;
-  schedule_tiles(in, st.n, tb, tbld, p->tile_passes, p->launches, p->order);
+  if (pt && !pt->swaps.empty()) {  // launches before k, the bit swaps, the rest
+    const std::vector<Launch> pre(in.begin(), in.begin() + pt->k);
+    const std::vector<Launch> suf(in.begin() + pt->k, in.end());
+    schedule_tiles(pre, st.n, tb, tbld, p->tile_passes, p->launches, p->order);
+    for (Launch& l : pt->swaps) p->launches.push_back(std::move(l));
+    pt->swaps.clear();
+    const size_t o0 = p->order.size();
+    schedule_tiles(suf, st.n, tb, tbld, p->tile_passes, p->launches, p->order);
+    for (size_t i = o0; i < p->order.size(); ++i) p->order[i] += pt->k;
+  } else {
+    schedule_tiles(in, st.n, tb, tbld, p->tile_passes, p->launches, p->order);
+  }
   if (tbld.ops.empty()) return kOk;
   // upload ops, diagonal descriptors and the matrix pool (state precision)
   auto upload = [&](const void* src, size_t bytes, void** dst) {
@@ -1323,14 +1334,25 @@ void build_mt_plan(const std::vector<Launch>& in, uint32_t n, uint32_t tb, MtPla
   }
 }
 
-int plan_mfma_tiles(Program* p, int tb) {
+int plan_mfma_tiles(Program* p, int tb, PermTail* pt) {
   State& st = *p->st;
   if (p->precision != QSVH_PREC_MFMA_BF16X3)
     return set_err(st.ctx, kInvalidArg, "mfma_tiles runs split-BF16 groups (MFMA_BF16X3)");
   if (tb != 12) return set_err(st.ctx, kInvalidArg, "mfma_tiles must be 12");
   if (st.n < uint32_t(tb)) return plan_mfma(p);  // smaller than one tile
   MtPlan plan;
-  build_mt_plan(p->launches, st.n, uint32_t(tb), &plan);
+  if (pt && !pt->swaps.empty()) {  // launches before k, the bit swaps, the rest
+    const std::vector<Launch> pre(p->launches.begin(), p->launches.begin() + pt->k);
+    const std::vector<Launch> suf(p->launches.begin() + pt->k, p->launches.end());
+    build_mt_plan(pre, st.n, uint32_t(tb), &plan);
+    for (Launch& l : pt->swaps) plan.launches.push_back(std::move(l));
+    pt->swaps.clear();
+    const size_t o0 = plan.order.size();
+    build_mt_plan(suf, st.n, uint32_t(tb), &plan);
+    for (size_t i = o0; i < plan.order.size(); ++i) plan.order[i] += pt->k;
+  } else {
+    build_mt_plan(p->launches, st.n, uint32_t(tb), &plan);
+  }
   // p->order is the identity here (tile_qubits excludes mfma_tiles): launch i is op i
   p->order = std::move(plan.order);
   p->launches = std::move(plan.launches);
@@ -1387,7 +1409,8 @@ static bool qubit_perm_of(const Launch& l, uint32_t* tau) {
   return true;
 }
 
-uint64_t strip_perm_suffix(std::vector<Launch>& ls, uint32_t n, std::vector<Launch>* out) {
+uint64_t strip_perm_suffix(std::vector<Launch>& ls, uint32_t n, std::vector<Launch>* out,
+                           std::vector<uint32_t>* dest_out) {
   uint64_t first = ls.size();
   uint32_t tau[kMaxGateQubits];
   while (first > 0 && qubit_perm_of(ls[first - 1], tau)) --first;
@@ -1404,6 +1427,7 @@ uint64_t strip_perm_suffix(std::vector<Launch>& ls, uint32_t n, std::vector<Laun
     for (uint32_t b = 0; b < n; ++b) dest[b] = g[dest[b]];
   }
   ls.resize(first);
+  if (dest_out) *dest_out = dest;
   // dest = beta . alpha with alpha, beta involutions: on each cycle c_0 -> c_1 -> ... of
   // length k, alpha(c_i) = c_{-i mod k} and beta(c_i) = c_{1-i mod k}
   std::vector<uint8_t> alpha(n), beta(n);
@@ -1431,6 +1455,37 @@ uint64_t strip_perm_suffix(std::vector<Launch>& ls, uint32_t n, std::vector<Laun
     out->push_back(std::move(l));
   }
   return first;
+}
+
+uint64_t hoist_perm(std::vector<Launch>& body, const std::vector<uint32_t>& dest,
+                    const std::function<size_t(const std::vector<Launch>&)>& count) {
+  const size_t m = body.size();
+  std::vector<Launch> moved = body;
+  for (Launch& l : moved)
+    for (uint32_t j = 0; j < l.m; ++j) l.q[j] = dest[l.q[j]];
+  // qubits the body acts on from k on
+  std::vector<uint64_t> sup(m + 1, 0);
+  for (size_t k = m; k-- > 0;) sup[k] = sup[k + 1] | qubit_mask(body[k]);
+  uint64_t best_k = m;
+  size_t best = count(body);
+  for (size_t k = 0; k < m; ++k) {
+    if (k > 0 && popcount64(sup[k]) == popcount64(sup[k - 1])) continue;
+    const std::vector<Launch> pre(body.begin(), body.begin() + k);
+    const std::vector<Launch> suf(moved.begin() + k, moved.end());
+    const size_t c = (k ? count(pre) : 0) + count(suf);
+    if (c < best) {
+      best = c;
+      best_k = k;
+    }
+  }
+  for (size_t k = best_k; k < m; ++k) body[k] = std::move(moved[k]);
+  return best_k;
+}
+
+size_t mt_launch_count(const std::vector<Launch>& in, uint32_t n) {
+  MtPlan plan;
+  build_mt_plan(in, n, 12, &plan);
+  return plan.launches.size();
 }
 
 void release_extra(Program* p) {

@@ -96,7 +96,7 @@ void tile_schedule_features(const Circuit& c, int dtype, uint32_t tb, std::vecto
   std::vector<Launch> in;
   in.reserve(c.ops.size());
   for (const Op& op : c.ops) in.push_back(classify(op, dtype));
-  std::vector<Launch> perm;  // the trailing qubit permutation, as the program runs it
+  std::vector<Launch> perm;  // the trailing qubit permutation, run after the passes
   strip_perm_suffix(in, c.n, &perm);
   TileBuilder bld;
   std::vector<TilePassArg> passes;
@@ -145,10 +145,17 @@ void mt_schedule_features(const Circuit& c, std::vector<float>* rows, std::vecto
   std::vector<Launch> in;
   in.reserve(c.ops.size());
   for (const Op& op : c.ops) in.push_back(classify(op, 1));
-  std::vector<Launch> perm;
-  strip_perm_suffix(in, c.n, &perm);
+  std::vector<Launch> perm;  // the trailing qubit permutation, placed as the program places it
+  std::vector<uint32_t> dest;
+  strip_perm_suffix(in, c.n, &perm, &dest);
+  uint64_t k = in.size();
+  if (!perm.empty())
+    k = hoist_perm(in, dest, [&](const std::vector<Launch>& v) {
+      return mt_launch_count(v, c.n);
+    });
   MtPlan plan;
-  build_mt_plan(in, c.n, 12, &plan);
+  build_mt_plan(std::vector<Launch>(in.begin(), in.begin() + k), c.n, 12, &plan);
+  build_mt_plan(std::vector<Launch>(in.begin() + k, in.end()), c.n, 12, &plan);
   rows->assign(plan.passes.size() * kMtFeatures, 0.f);
   for (size_t i = 0; i < plan.passes.size(); ++i)
     mt_pass_features(plan, i, c.n, rows->data() + i * kMtFeatures);

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -91,17 +92,39 @@ void schedule_tiles(const std::vector<Launch>& in, uint32_t n, uint32_t tb, Tile
                     std::vector<TilePassArg>& passes, std::vector<Launch>& out,
                     std::vector<uint64_t>& order);
 
-// Passes that rewrite the plain launch list (plan.cpp).
-int plan_tiles(Program* p, int tile_qubits);
+// A trailing qubit permutation taken out of a circuit and the point k at which its bit
+// swaps run: before launch k of the rest, whose launches from k on act on the permuted
+// qubits (the data of state bit b is at bit dest[b] after the swaps).  The same values
+// result: each moved gate does its arithmetic on the same amplitudes, at other indices.
+struct PermTail {
+  std::vector<Launch> swaps;   // the kBitSwap launches; emptied once a planner placed them
+  std::vector<uint32_t> dest;
+  uint64_t k = 0;
+};
+
+// Passes that rewrite the plain launch list (plan.cpp).  A tile planner given a PermTail
+// with swaps schedules the launches before k and from k separately, the swaps between
+// (k at the end: the swaps after the passes).
+int plan_tiles(Program* p, int tile_qubits, PermTail* pt = nullptr);
 void plan_subspace(Program* p);
 int plan_mfma(Program* p);
 // Runs of fused groups as MFMA tile passes of 2^tb amplitudes (split-BF16 precision).
-int plan_mfma_tiles(Program* p, int tb);
+int plan_mfma_tiles(Program* p, int tb, PermTail* pt = nullptr);
 // Trailing qubit permutations (dense launches whose matrix only moves the bits of the
 // sub-index, such as the SWAP network that ends a QFT, alone or fused into groups) are
-// removed from `ls`; their product, a permutation of the state bits, is appended to `out`
-// as at most two kBitSwap launches.  Returns the index of the first launch removed.
-uint64_t strip_perm_suffix(std::vector<Launch>& ls, uint32_t n, std::vector<Launch>* out);
+// removed from `ls`; their product, a permutation of the state bits (dest, if given), is
+// appended to `out` as at most two kBitSwap launches.  Returns the index of the first
+// launch removed.
+uint64_t strip_perm_suffix(std::vector<Launch>& ls, uint32_t n, std::vector<Launch>* out,
+                           std::vector<uint32_t>* dest = nullptr);
+// The point at which a trailing permutation (dest) runs among `body`: the end, the start,
+// or a point where the qubits the rest of the body acts on shrink -- whichever gives the
+// fewest launches by `count` (prefix plus moved suffix; the end on ties).  The body's
+// launches from the returned k on are moved to the permuted qubits.
+uint64_t hoist_perm(std::vector<Launch>& body, const std::vector<uint32_t>& dest,
+                    const std::function<size_t(const std::vector<Launch>&)>& count);
+// Launches of the MFMA tile form of `in` (12-bit tiles).
+size_t mt_launch_count(const std::vector<Launch>& in, uint32_t n);
 // The host side of plan_mfma_tiles: the passes, their ops and matrix pool, and the launch
 // sequence (kMfmaTile launches and the launches no pass holds).
 struct MtPlan {

@@ -117,6 +117,53 @@ def test_qft_tile_programs_with_swap_suffix(backend, tile):
         assert got == kernels, (n, got)
 
 
+def low_then_high_then_reversal(n, seed):
+    """H everywhere, gates on qubits 0..7, gates on the top 8 qubits (with controlled
+    phases from the low ones), then the qubit reversal."""
+    rng = np.random.default_rng(seed)
+    c = rs.Circuit(n)
+    for q in range(n):
+        c.h(q)
+    for _ in range(40):
+        a, b = rng.choice(8, 2, replace=False)
+        c.cx(int(a), int(b)).u3(int(rng.integers(8)), *rng.uniform(0, 6, 3))
+    for _ in range(60):
+        a, b = rng.choice(np.arange(n - 8, n), 2, replace=False)
+        c.cx(int(a), int(b)).u3(int(rng.integers(n - 8, n)), *rng.uniform(0, 6, 3))
+        c.cphase(int(rng.integers(0, 6)), int(rng.integers(n - 8, n)), float(rng.uniform(0, 6)))
+    for q in range(n // 2):
+        c.swap(q, n - 1 - q)
+    return c
+
+
+@pytest.mark.parametrize("n", [16, 20])
+def test_mfma_tiles_with_the_swaps_moved_ahead(backend, n):
+    # fused at k = 4 as MFMA tile passes, the trailing permutation runs where the passes
+    # come out fewest (here before the high part, whose groups then act on low qubits):
+    # the launches the planner features count, within the fused forms' error bar
+    from rustatevec_amd import planner
+    circ = low_then_high_then_reversal(n, 1)
+    fused = rs.fuse(backend, circ, 4)
+    rows, other = planner.mt_pass_features(fused)
+    h = rs.gate_matrix("h")
+    kept, _ = planner.mt_pass_features(rs.Circuit.from_ops(n, fused.ops() + [(h, [0])]))
+    assert len(rows) < len(kept)
+    plain = backend.alloc(n, rs.C128)
+    backend.init_random_normal(plain, 2)
+    rs.Program(plain, circ).launch()
+    st = backend.alloc(n, rs.C64)
+    backend.init_random_normal(st, 2)
+    p = rs.Program(st, fused, precision=rs.PREC_MFMA_BF16X3, mfma_tiles=12)
+    assert p.num_kernels() == len(rows) + other
+    assert sorted(p.order()) == list(range(len(fused)))
+    p.launch()
+    err, fid = backend.compare(plain, st)
+    del p
+    st.free()
+    plain.free()
+    assert err < 1e-6 and 1.0 - fid < 1e-8, (err, 1.0 - fid)
+
+
 def test_circuits_ending_in_swaps_complex64_tiles(backend):
     rng = np.random.default_rng(5)
     for n, tile in ((13, 11), (15, 12), (16, 13)):
