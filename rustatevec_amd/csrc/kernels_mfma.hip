@@ -26,6 +26,7 @@
 // outside the tile fixed per tile) are applied to it in order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "amp.hpp"
@@ -378,11 +379,14 @@ hipError_t launch_k(const State& st, const TileMap& map, const void* mats, int p
   const int cus = st.ctx && st.ctx->cus > 0 ? st.ctx->cus : 256;
   // Split-BF16 at k = 4: one workgroup per tile (its A fragments are 4 KiB, cheap to load
   // per tile, and a looping workgroup's next loads wait out its previous stores): 10-21%
-  // faster than the persistent grid at 30 qubits.  Wider groups (16-64 KiB of fragments)
-  // and FP32 keep a persistent grid of one or two workgroups per CU.
+  // faster than the persistent grid at 30 qubits; at k = 5 (16 KiB of fragments) four
+  // tiles per workgroup, 1-7% faster.  6-qubit groups (64 KiB) and FP32 keep a persistent
+  // grid of one or two workgroups per CU.
+  const uint64_t persistent = uint64_t(precision == 1 && K == 4 ? 1 : 2) * cus;
   const uint64_t cap = t.mfma_blocks > 0 ? uint64_t(t.mfma_blocks)
                        : precision == 2 && K == 4 ? (uint64_t(1) << 31) / kBlock
-                                                  : uint64_t(precision == 1 && K == 4 ? 1 : 2) * cus;
+                       : precision == 2 && K == 5 ? std::max(n_tiles / 4, persistent)
+                                                  : persistent;
   const uint64_t g = n_tiles < cap ? n_tiles : cap;
   float* amp = static_cast<float*>(st.amp);
   if (precision == 1) {
