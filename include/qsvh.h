@@ -93,8 +93,8 @@ int qsvh_abi_version(void);
  * are a grid apart; 1: adjacent), "xcd" (1: consecutive blocks of a streaming kernel take
  * neighbouring address ranges within one XCD; 0: launch order); "mfma_blocks"
  * (workgroups of the matrix-core group kernels; 0, the default: one per tile for split-BF16
- * 4-qubit groups, one per 4 tiles for split-BF16 5-qubit groups, 1 per CU for FP32 4-qubit
- * groups, else 2 per CU); "tile_blocks" (workgroups of the tile-pass
+ * 4-qubit groups, one per 4 tiles for split-BF16 5-qubit groups, one per 64 tiles for FP32
+ * 6-qubit groups, 1 per CU for FP32 4-qubit groups, else 2 per CU); "tile_blocks" (workgroups of the tile-pass
  * kernels, exact and MFMA; 0, the default: one per tile; n > 0: n looping over tiles);
  * "mfma_layout" (LDS layout of the FP32 group kernel: 0 chosen from the group's qubits,
  * 1 row-major, 2 column-major).

@@ -380,12 +380,14 @@ hipError_t launch_k(const State& st, const TileMap& map, const void* mats, int p
   // Split-BF16 at k = 4: one workgroup per tile (its A fragments are 4 KiB, cheap to load
   // per tile, and a looping workgroup's next loads wait out its previous stores): 10-21%
   // faster than the persistent grid at 30 qubits; at k = 5 (16 KiB of fragments) four
-  // tiles per workgroup, 1-7% faster.  6-qubit groups (64 KiB) and FP32 keep a persistent
-  // grid of one or two workgroups per CU.
+  // tiles per workgroup, 1-7% faster.  6-qubit groups (64 KiB) keep a persistent grid of
+  // two workgroups per CU; FP32 one (4-qubit) or two per CU, except 6-qubit groups: one
+  // workgroup per 64 tiles, 2-3% faster.
   const uint64_t persistent = uint64_t(precision == 1 && K == 4 ? 1 : 2) * cus;
   const uint64_t cap = t.mfma_blocks > 0 ? uint64_t(t.mfma_blocks)
                        : precision == 2 && K == 4 ? (uint64_t(1) << 31) / kBlock
                        : precision == 2 && K == 5 ? std::max(n_tiles / 4, persistent)
+                       : precision == 1 && K == 6 ? std::max(n_tiles / 64, persistent)
                                                   : persistent;
   const uint64_t g = n_tiles < cap ? n_tiles : cap;
   float* amp = static_cast<float*>(st.amp);
