@@ -105,6 +105,11 @@ def main():
     ap.add_argument("--mfma-only", action="store_true",
                     help="only 5 launches each of one dense 6-qubit group on a 30q c64 "
                          "state in FP32 and in split-BF16x3 MFMA, for counter runs")
+    ap.add_argument("--mfma4-only", action="store_true",
+                    help="only 5 launches each of one dense 4-qubit group on a 30q c64 state "
+                         "in split-BF16x3 MFMA, on qubits {3, 9, 20, 29} and {10, 15, 22, 27}, "
+                         "with the default grid (one workgroup per tile) and then with the "
+                         "persistent grid of 2 workgroups per CU, for profiler runs")
     ap.add_argument("--mt-only", action="store_true",
                     help="only 3 launches of QFT(30) fused at k = 4 as split-BF16 MFMA tile "
                          "passes (2^12-amplitude tiles), c64, for profiler runs")
@@ -219,6 +224,26 @@ def main():
             p = rs.Program(st, g6, precision=prec)
             rec[name + "_ms"] = [round(p.time(), 4) for _ in range(5)]
             del p
+        print(json.dumps(rec), flush=True)
+        st.free()
+        b.close()
+        return
+
+    if args.mfma4_only:
+        b = rs.HipBackend(0)
+        st = b.alloc(30, rs.C64)
+        b.init_random_normal(st, 1)
+        u4 = np.linalg.qr(np.random.default_rng(1).normal(size=(16, 16))
+                          + 1j * np.random.default_rng(2).normal(size=(16, 16)))[0]
+        cus = b.device_info().get("compute_units", 256)
+        rec = {}
+        for blocks, name in ((0, "one_per_tile"), (2 * cus, "two_per_cu")):
+            b.tune(mfma_blocks=blocks)
+            for qs in ([3, 9, 20, 29], [10, 15, 22, 27]):
+                p = rs.Program(st, rs.Circuit(30).push(u4, qs), precision=rs.PREC_MFMA_BF16X3)
+                rec[f"{name}_q{qs[0]}_{qs[-1]}_ms"] = [round(p.time(), 4) for _ in range(5)]
+                del p
+        b.tune(mfma_blocks=0)
         print(json.dumps(rec), flush=True)
         st.free()
         b.close()
